@@ -1,0 +1,758 @@
+// Host orchestration of the query kernels and the C ABI entry points for
+// opening, querying and closing an index.
+#include <errno.h>
+#include <float.h>
+#include <stdio.h>
+#include <string.h>
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <memory>
+#include <mutex>
+
+#include "hkg_engine.h"
+#include "hkg_kernels.h"
+
+namespace hkg {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+const std::string &last_error() { return g_err; }
+
+// ---------------------------------------------------------------------------
+// device memory helpers
+// ---------------------------------------------------------------------------
+template <class T>
+static hkg_status dalloc(T **p, size_t count, uint64_t *acct)
+{
+  *p = nullptr;
+  if (count == 0) return HKG_OK;
+  hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    set_error(std::string("hipMalloc(") + std::to_string(count * sizeof(T)) + " B): " + hipGetErrorString(e));
+    *p = nullptr;
+    return HKG_ENOMEM;
+  }
+  if (acct) *acct += count * sizeof(T);
+  return HKG_OK;
+}
+
+template <class T>
+static void dfree(T *&p)
+{
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+// host file -> device, through two pinned staging buffers
+static hkg_status upload_file(const std::string &path, void *dst, uint64_t bytes, hipStream_t st)
+{
+  FILE *f = fopen(path.c_str(), "rb");
+  if (!f) {
+    set_error(path + ": " + strerror(errno));
+    return HKG_EIO;
+  }
+  fseek(f, 0, SEEK_END);
+  uint64_t sz = (uint64_t)ftell(f);
+  fseek(f, 0, SEEK_SET);
+  if (sz != bytes) {
+    fclose(f);
+    set_error(path + ": size " + std::to_string(sz) + " != expected " + std::to_string(bytes));
+    return HKG_EFORMAT;
+  }
+  const size_t chunk = 64ull << 20;
+  void *pin[2] = {nullptr, nullptr};
+  hipEvent_t ev[2];
+  if (hipHostMalloc(&pin[0], chunk, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&pin[1], chunk, hipHostMallocDefault) != hipSuccess) {
+    fclose(f);
+    if (pin[0]) (void)hipHostFree(pin[0]);
+    set_error("hipHostMalloc failed");
+    return HKG_ENOMEM;
+  }
+  (void)hipEventCreate(&ev[0]);
+  (void)hipEventCreate(&ev[1]);
+  hkg_status rc = HKG_OK;
+  bool used[2] = {false, false};
+  uint64_t off = 0;
+  int b = 0;
+  while (off < bytes) {
+    size_t n = (size_t)std::min<uint64_t>(chunk, bytes - off);
+    if (used[b]) (void)hipEventSynchronize(ev[b]);
+    if (fread(pin[b], 1, n, f) != n) {
+      set_error(path + ": short read");
+      rc = HKG_EIO;
+      break;
+    }
+    if (hipMemcpyAsync((char *)dst + off, pin[b], n, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipEventRecord(ev[b], st) != hipSuccess) {
+      set_error("hipMemcpyAsync failed while uploading " + path);
+      rc = HKG_EDEVICE;
+      break;
+    }
+    used[b] = true;
+    off += n;
+    b ^= 1;
+  }
+  (void)hipStreamSynchronize(st);
+  fclose(f);
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  (void)hipHostFree(pin[0]);
+  (void)hipHostFree(pin[1]);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// index lifetime
+// ---------------------------------------------------------------------------
+Index::~Index()
+{
+  (void)hipSetDevice(dev.device);
+  bufs.clear();
+  dfree(dev.X);
+  dfree(dev.sax);
+  dfree(dev.ids);
+  dfree(dev.leaf_pos);
+  dfree(dev.leaf_size);
+  dfree(dev.leaf_seg_off);
+  dfree(dev.seg_end);
+  dfree(dev.seg_ind);
+  dfree(dev.Xb);
+  dfree(dev.xnorm2);
+  dfree(dev.xerr);
+  dfree(idmap64);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+hkg_status Index::init_device(int device)
+{
+  dev.device = device;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    set_error("no HIP device available");
+    return HKG_EDEVICE;
+  }
+  if (device < 0 || device >= ndev) {
+    set_error("device ordinal out of range");
+    return HKG_EINVAL;
+  }
+  HKG_HIP_TRY(hipSetDevice(device));
+  HKG_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  HKG_HIP_TRY(upload_breakpoints_dev());
+  return HKG_OK;
+}
+
+// leaf tables (CSR of segments) from a host tree
+hkg_status Index::upload_tree()
+{
+  const HostTree &t = dev.tree;
+  const uint32_t nl = (uint32_t)t.leaves.size();
+  std::vector<uint64_t> pos;
+  std::vector<uint32_t> size, off;
+  std::vector<int16_t> se;
+  std::vector<float4> ind;
+  pos.reserve(nl);
+  size.reserve(nl);
+  off.reserve(nl + 1);
+  uint32_t maxleaf = 0;
+  for (uint32_t i = 0; i < nl; ++i) {
+    const HostNode &nd = t.nodes[t.leaves[i]];
+    pos.push_back(nd.file_pos);
+    size.push_back(nd.size);
+    maxleaf = std::max(maxleaf, nd.size);
+    off.push_back((uint32_t)se.size());
+    for (uint32_t s = 0; s < nd.nseg; ++s) {
+      se.push_back(t.seg_end[nd.seg_off + s]);
+      const float *g = &t.ind[4 * (nd.seg_off + s)];
+      ind.push_back(make_float4(g[0], g[1], g[2], g[3]));
+    }
+  }
+  off.push_back((uint32_t)se.size());
+  dev.nleaf = nl;
+  dev.max_leaf = maxleaf;
+  hkg_status rc;
+  if ((rc = dalloc(&dev.leaf_pos, nl, &dev.device_bytes)) != HKG_OK) return rc;
+  if ((rc = dalloc(&dev.leaf_size, nl, &dev.device_bytes)) != HKG_OK) return rc;
+  if ((rc = dalloc(&dev.leaf_seg_off, nl + 1, &dev.device_bytes)) != HKG_OK) return rc;
+  if ((rc = dalloc(&dev.seg_end, se.size(), &dev.device_bytes)) != HKG_OK) return rc;
+  if ((rc = dalloc(&dev.seg_ind, ind.size(), &dev.device_bytes)) != HKG_OK) return rc;
+  HKG_HIP_TRY(hipMemcpy(dev.leaf_pos, pos.data(), nl * 8, hipMemcpyHostToDevice));
+  HKG_HIP_TRY(hipMemcpy(dev.leaf_size, size.data(), nl * 4, hipMemcpyHostToDevice));
+  HKG_HIP_TRY(hipMemcpy(dev.leaf_seg_off, off.data(), (nl + 1) * 4, hipMemcpyHostToDevice));
+  if (!se.empty()) {
+    HKG_HIP_TRY(hipMemcpy(dev.seg_end, se.data(), se.size() * 2, hipMemcpyHostToDevice));
+    HKG_HIP_TRY(hipMemcpy(dev.seg_ind, ind.data(), ind.size() * sizeof(float4), hipMemcpyHostToDevice));
+  }
+  dev.has_tree = true;
+  return HKG_OK;
+}
+
+hkg_status open_dir(const char *dir, const hkg_options *opt, Index **out)
+{
+  *out = nullptr;
+  if (!dir) {
+    set_error("null index directory");
+    return HKG_EINVAL;
+  }
+  hkg_options o;
+  hkg_default_options(&o);
+  if (opt) o = *opt;
+  std::string d(dir);
+  if (!d.empty() && d.back() != '/') d += '/';
+  std::unique_ptr<Index> ix(new Index());
+  std::string err;
+  hkg_status rc = parse_root_idx(d + "root.idx", ix->dev.tree, err);
+  if (rc != HKG_OK) {
+    set_error(d + "root.idx: " + err);
+    return rc;
+  }
+  const HostTree &t = ix->dev.tree;
+  if (t.L > kMaxLength) {
+    set_error("series longer than " + std::to_string(kMaxLength) + " are not supported");
+    return HKG_EINVAL;
+  }
+  if (t.paa_segments != kPaaSegments || t.sax_bit_cardinality != 8) {
+    set_error("only 16 iSAX segments of 8-bit cardinality are supported");
+    return HKG_EFORMAT;
+  }
+  if (!t.sims) {
+    set_error("index was built without --sims (no leaves_sims.idx)");
+    return HKG_EFORMAT;
+  }
+  if ((rc = ix->init_device(o.device)) != HKG_OK) return rc;
+  ix->dev.n = t.n_series;
+  ix->dev.L = t.L;
+  if ((rc = ix->upload_tree()) != HKG_OK) return rc;
+  const uint64_t n = ix->dev.n, L = ix->dev.L;
+  if ((rc = dalloc(&ix->dev.X, n * L, &ix->dev.device_bytes)) != HKG_OK) return rc;
+  if ((rc = dalloc(&ix->dev.sax, n * kPaaSegments, &ix->dev.device_bytes)) != HKG_OK) return rc;
+  if ((rc = upload_file(d + "leaves_raw.idx", ix->dev.X, n * L * 4, ix->stream)) != HKG_OK) return rc;
+  if ((rc = upload_file(d + "leaves_sims.idx", ix->dev.sax, n * kPaaSegments, ix->stream)) != HKG_OK) return rc;
+  if (o.keep_bf16 && (rc = ix->make_bf16()) != HKG_OK) return rc;
+  if (o.verbose)
+    fprintf(stderr, "hkg: loaded %s: %llu series x %u, %u leaves, %.2f GB in HBM\n", dir,
+            (unsigned long long)n, (unsigned)L, ix->dev.nleaf, ix->dev.device_bytes / 1e9);
+  *out = ix.release();
+  return HKG_OK;
+}
+
+hkg_status open_raw(const float *X, uint64_t n, uint32_t L, const hkg_options *opt, Index **out)
+{
+  *out = nullptr;
+  if (!X || n == 0 || L == 0 || L % 8 != 0 || L > kMaxLength || n >= (1ull << 32)) {
+    set_error("open_raw: need n in [1, 2^32) rows and L a positive multiple of 8, at most " +
+              std::to_string(kMaxLength));
+    return HKG_EINVAL;
+  }
+  hkg_options o;
+  hkg_default_options(&o);
+  if (opt) o = *opt;
+  std::unique_ptr<Index> ix(new Index());
+  hkg_status rc;
+  if ((rc = ix->init_device(o.device)) != HKG_OK) return rc;
+  ix->dev.n = n;
+  ix->dev.L = L;
+  if ((rc = dalloc(&ix->dev.X, n * L, &ix->dev.device_bytes)) != HKG_OK) return rc;
+  HKG_HIP_TRY(hipMemcpy(ix->dev.X, X, n * L * 4, hipMemcpyHostToDevice));
+  if (o.keep_bf16 && (rc = ix->make_bf16()) != HKG_OK) return rc;
+  *out = ix.release();
+  return HKG_OK;
+}
+
+hkg_status Index::set_ids(const uint64_t *ids, uint64_t n)
+{
+  if (n != dev.n) {
+    set_error("id map length differs from the number of series");
+    return HKG_EINVAL;
+  }
+  // the device keys carry 32-bit ids; the 64-bit map translates them back
+  std::vector<uint32_t> rank(n);
+  // the key's low word must order like the public id: use the rank of the id
+  std::vector<uint64_t> sorted(ids, ids + n);
+  std::sort(sorted.begin(), sorted.end());
+  if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) {
+    set_error("id map has duplicate ids");
+    return HKG_EINVAL;
+  }
+  for (uint64_t i = 0; i < n; ++i)
+    rank[i] = (uint32_t)(std::lower_bound(sorted.begin(), sorted.end(), ids[i]) - sorted.begin());
+  hkg_status rc;
+  (void)hipSetDevice(dev.device);
+  dfree(dev.ids);
+  dfree(idmap64);
+  if ((rc = dalloc(&dev.ids, n, &dev.device_bytes)) != HKG_OK) return rc;
+  if ((rc = dalloc(&idmap64, n, &dev.device_bytes)) != HKG_OK) return rc;
+  HKG_HIP_TRY(hipMemcpy(dev.ids, rank.data(), n * 4, hipMemcpyHostToDevice));
+  HKG_HIP_TRY(hipMemcpy(idmap64, sorted.data(), n * 8, hipMemcpyHostToDevice));
+  dev.host_ids.assign(ids, ids + n);
+  return HKG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-batch device buffers
+// ---------------------------------------------------------------------------
+QueryBuffers::~QueryBuffers()
+{
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  dfree(Q);
+  dfree(kq);
+  dfree(paa);
+  dfree(psum);
+  dfree(qn2);
+  dfree(lb);
+  dfree(best);
+  dfree(bound);
+  dfree(cand);
+  dfree(cand_cnt);
+  dfree(work);
+  dfree(misc);
+  dfree(counters);
+  dfree(out_ids);
+  dfree(out_d2);
+}
+
+hkg_status Index::get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuffers **out)
+{
+  const uint32_t L = dev.L, nleaf = dev.nleaf;
+  const uint64_t key = ((uint64_t)B << 40) ^ ((uint64_t)kmax << 20) ^ cap;
+  auto it = bufs.find(key);
+  if (it != bufs.end()) {
+    *out = it->second.get();
+    return HKG_OK;
+  }
+  // keep at most a few buffer sets alive
+  if (bufs.size() >= 4) bufs.clear();
+  std::unique_ptr<QueryBuffers> b(new QueryBuffers());
+  b->B = B;
+  b->kmax = kmax;
+  b->cap = cap;
+  b->work_cap = (uint64_t)B * std::max<uint32_t>(nleaf, 1) > 0xFFFFFFFFull
+                    ? 0xFFFFFFFFu
+                    : (uint32_t)((uint64_t)B * std::max<uint32_t>(nleaf, 1));
+  uint64_t acct = 0;
+  hkg_status rc;
+#define A(ptr, cnt) \
+  if ((rc = dalloc(&b->ptr, (cnt), &acct)) != HKG_OK) return rc;
+  A(Q, (uint64_t)B * L);
+  A(kq, B);
+  A(paa, (uint64_t)B * kPaaSegments);
+  A(psum, (uint64_t)B * 2 * (L + 1));
+  A(qn2, B);
+  A(lb, (uint64_t)B * std::max<uint32_t>(nleaf, 1));
+  A(best, B);
+  A(bound, B);
+  A(cand, (uint64_t)B * cap);
+  A(cand_cnt, B);
+  A(work, dev.has_tree ? b->work_cap : 1);
+  A(misc, 8);
+  A(counters, kNumCounters);
+  A(out_ids, (uint64_t)B * kmax);
+  A(out_d2, (uint64_t)B * kmax);
+#undef A
+  b->bytes = acct;
+  *out = b.get();
+  bufs[key] = std::move(b);
+  return HKG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// one batch through a path (device pointers; results left in b->out_*)
+// ---------------------------------------------------------------------------
+static uint32_t persistent_grid()
+{
+  static uint32_t g = 0;
+  if (!g) {
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    g = (uint32_t)std::max(cus, 1) * 3;
+  }
+  return g;
+}
+
+void Index::enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st)
+{
+  const uint32_t L = dev.L, nleaf = dev.nleaf;
+  uint32_t *nwork = b.misc + 0, *work_next = b.misc + 1, *overflow = b.misc + 2;
+  launch_reset(st, nq, b.bound, b.cand_cnt);
+  launch_fill(st, b.misc, 8, 0);
+  launch_fill(st, reinterpret_cast<uint32_t *>(b.counters), 2 * kNumCounters, 0);
+  launch_prep(st, b.Q, nq, L, b.paa, b.psum, b.qn2);
+  launch_eapca(st, nq, nleaf, L, dev.leaf_seg_off, dev.seg_end, dev.seg_ind, b.psum, b.lb);
+  launch_best_leaf(st, nq, nleaf, b.lb, b.best);
+  launch_scan_best(st, nq, dev.X, L, b.Q, b.kq, b.best, dev.leaf_pos, dev.leaf_size, dev.ids, b.cap, b.cand,
+                   b.cand_cnt, b.bound, b.counters);
+  launch_collect(st, nq, nleaf, b.lb, b.best, b.bound, dev.leaf_size, b.work, b.work_cap, nwork, b.counters);
+  launch_filter_refine(st, persistent_grid(), dev.X, dev.sax, L, b.Q, b.paa, b.kq, dev.leaf_pos, dev.leaf_size,
+                       dev.ids, b.work, nwork, b.work_cap, work_next, b.cap, b.cand, b.cand_cnt, b.bound,
+                       b.counters);
+  launch_select(st, nq, b.kmax, b.kq, b.cap, b.cand, b.cand_cnt, idmap64, b.out_ids, b.out_d2, overflow);
+}
+
+uint32_t scan_grid_x(uint32_t nq)
+{
+  uint32_t groups = (nq + kScanQ - 1) / kScanQ;
+  uint32_t total = persistent_grid();
+  uint32_t gx = total / std::max<uint32_t>(groups, 1);
+  return std::max<uint32_t>(gx, 64);
+}
+
+void Index::enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st)
+{
+  uint32_t *overflow = b.misc + 2;
+  launch_reset(st, nq, b.bound, b.cand_cnt);
+  launch_fill(st, b.misc, 8, 0);
+  launch_fill(st, reinterpret_cast<uint32_t *>(b.counters), 2 * kNumCounters, 0);
+  launch_scan(st, scan_grid_x(nq), dev.X, dev.n, dev.L, b.Q, nq, b.kq, dev.ids, b.cap, b.cand, b.cand_cnt,
+              b.bound);
+  launch_select(st, nq, b.kmax, b.kq, b.cap, b.cand, b.cand_cnt, idmap64, b.out_ids, b.out_d2, overflow);
+}
+
+// capacity of the per-query candidate buffer for a path
+uint32_t Index::cand_cap(hkg_path path, uint32_t nq, uint32_t kmax) const
+{
+  if (path == HKG_PATH_SCAN || !dev.has_tree) return scan_grid_x(nq) * kmax + 256;
+  return std::max<uint32_t>(8192, 16 * kmax);
+}
+
+hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax,
+                            bool use_graph, hipStream_t st, QueryBuffers **used, Counters *ctr)
+{
+  const bool scan = (path == HKG_PATH_SCAN) || !dev.has_tree;
+  const uint32_t cap = cand_cap(scan ? HKG_PATH_SCAN : path, nq, kmax);
+  QueryBuffers *b = nullptr;
+  hkg_status rc = get_buffers(nq, kmax, cap, &b);
+  if (rc != HKG_OK) return rc;
+  HKG_HIP_TRY(hipMemcpyAsync(b->Q, dQ, (size_t)nq * dev.L * 4, hipMemcpyDeviceToDevice, st));
+  HKG_HIP_TRY(hipMemcpyAsync(b->kq, dk, (size_t)nq * 4, hipMemcpyDeviceToDevice, st));
+  if (use_graph && !scan) {
+    if (!b->graph_exec || b->graph_stream != st) {
+      if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+      b->graph_exec = nullptr;
+      hipGraph_t g;
+      HKG_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      enqueue_index_batch(*b, nq, st);
+      HKG_HIP_TRY(hipStreamEndCapture(st, &g));
+      hipError_t e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HKG_HIP_TRY(e);
+      b->graph_stream = st;
+    }
+    HKG_HIP_TRY(hipGraphLaunch(b->graph_exec, st));
+  } else if (scan) {
+    enqueue_scan_batch(*b, nq, st);
+  } else {
+    enqueue_index_batch(*b, nq, st);
+  }
+  HKG_HIP_TRY(hipGetLastError());
+  if (ctr) {
+    HKG_HIP_TRY(hipMemcpyAsync(ctr->v, b->counters, sizeof(ctr->v), hipMemcpyDeviceToHost, st));
+    HKG_HIP_TRY(hipMemcpyAsync(&ctr->overflow, b->misc + 2, 4, hipMemcpyDeviceToHost, st));
+    HKG_HIP_TRY(hipMemcpyAsync(&ctr->nwork, b->misc + 0, 4, hipMemcpyDeviceToHost, st));
+  }
+  *used = b;
+  return HKG_OK;
+}
+
+// overflowed queries (candidate buffer too small) are answered again by the
+// full scan, whose buffer is sized so that it cannot overflow
+hkg_status Index::fix_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax, hipStream_t st, uint32_t *retries)
+{
+  std::vector<uint32_t> cnt(nq);
+  HKG_HIP_TRY(hipMemcpyAsync(cnt.data(), b.cand_cnt, nq * 4, hipMemcpyDeviceToHost, st));
+  HKG_HIP_TRY(hipStreamSynchronize(st));
+  for (uint32_t q = 0; q < nq; ++q) {
+    if (cnt[q] <= b.cap) continue;
+    ++*retries;
+    QueryBuffers *sb = nullptr;
+    Counters c;
+    hkg_status rc = run_batch(HKG_PATH_SCAN, b.Q + (uint64_t)q * dev.L, b.kq + q, 1, kmax, false, st, &sb, &c);
+    if (rc != HKG_OK) return rc;
+    HKG_HIP_TRY(hipMemcpyAsync(b.out_ids + (uint64_t)q * kmax, sb->out_ids, kmax * 8, hipMemcpyDeviceToDevice, st));
+    HKG_HIP_TRY(hipMemcpyAsync(b.out_d2 + (uint64_t)q * kmax, sb->out_d2, kmax * 4, hipMemcpyDeviceToDevice, st));
+    HKG_HIP_TRY(hipStreamSynchronize(st));
+  }
+  return HKG_OK;
+}
+
+hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, uint32_t kmax,
+                             const hkg_query_opts *qo, uint64_t *dids, float *dd2, hipStream_t st,
+                             hkg_query_stats *stats)
+{
+  hkg_query_opts o;
+  hkg_default_query_opts(&o);
+  if (qo) o = *qo;
+  if (kmax == 0 || kmax > (uint32_t)kMaxK) {
+    set_error("k must be in [1, 256]");
+    return HKG_EINVAL;
+  }
+  if (nq == 0) return HKG_OK;
+  hkg_path path = o.path;
+  if (path == HKG_PATH_AUTO) path = HKG_PATH_BATCHED;
+  if (path == HKG_PATH_BF16 && !dev.Xb) {
+    hkg_status rc = make_bf16();
+    if (rc != HKG_OK) return rc;
+  }
+  if (!dev.has_tree && path != HKG_PATH_SCAN) path = HKG_PATH_SCAN;
+  uint32_t batch = o.batch;
+  if (batch == 0) batch = path == HKG_PATH_INDEX ? 1 : (path == HKG_PATH_SCAN ? 64 : 1024);
+  if (path == HKG_PATH_INDEX) batch = 1;
+  if (batch > 16384) batch = 16384;  // grid.y of the per-query launches
+  HKG_HIP_TRY(hipSetDevice(dev.device));
+  if (!st) st = stream;
+  hipEvent_t e0, e1;
+  HKG_HIP_TRY(hipEventCreate(&e0));
+  HKG_HIP_TRY(hipEventCreate(&e1));
+  HKG_HIP_TRY(hipEventRecord(e0, st));
+  Counters total;
+  uint32_t retries = 0;
+  for (uint32_t q0 = 0; q0 < nq; q0 += batch) {
+    uint32_t n = std::min(batch, nq - q0);
+    QueryBuffers *b = nullptr;
+    Counters c;
+    hkg_status rc;
+    if (path == HKG_PATH_BF16)
+      rc = run_bf16_batch(dQ + (uint64_t)q0 * dev.L, dk + q0, n, kmax, st, &b, &c);
+    else
+      rc = run_batch(path, dQ + (uint64_t)q0 * dev.L, dk + q0, n, kmax, path == HKG_PATH_BATCHED && n == batch, st,
+                     &b, &c);
+    if (rc != HKG_OK) return rc;
+    HKG_HIP_TRY(hipStreamSynchronize(st));
+    if (c.overflow) {
+      rc = fix_overflow(*b, n, kmax, st, &retries);
+      if (rc != HKG_OK) return rc;
+    }
+    HKG_HIP_TRY(hipMemcpyAsync(dids + (uint64_t)q0 * kmax, b->out_ids, (size_t)n * kmax * 8,
+                               hipMemcpyDeviceToDevice, st));
+    HKG_HIP_TRY(hipMemcpyAsync(dd2 + (uint64_t)q0 * kmax, b->out_d2, (size_t)n * kmax * 4,
+                               hipMemcpyDeviceToDevice, st));
+    for (int i = 0; i < kNumCounters; ++i) total.v[i] += c.v[i];
+    total.nwork += c.nwork;
+  }
+  HKG_HIP_TRY(hipEventRecord(e1, st));
+  HKG_HIP_TRY(hipStreamSynchronize(st));
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (stats) {
+    memset(stats, 0, sizeof(*stats));
+    stats->queries = nq;
+    const bool scan = path == HKG_PATH_SCAN;
+    stats->leaves_total = (uint64_t)nq * dev.nleaf;
+    stats->leaves_after_eapca = scan ? 0 : total.v[kCtrLeaves] + nq;
+    stats->series_total = (uint64_t)nq * dev.n;
+    stats->series_lb_checked = scan ? 0 : total.v[kCtrSaxChecked] + total.v[kCtrBf16];
+    stats->series_refined = scan ? (uint64_t)nq * dev.n : total.v[kCtrRefined];
+    stats->bytes_algorithmic = scan ? (uint64_t)((nq + kScanQ - 1) / kScanQ) * dev.n * dev.L * 4
+                                    : total.v[kCtrSaxChecked] * 16 + total.v[kCtrBf16] * dev.L * 2 +
+                                          total.v[kCtrRefined] * dev.L * 4;
+    stats->device_ms = ms;
+    stats->overflow_retries = retries;
+  }
+  return HKG_OK;
+}
+
+hkg_status Index::knn_host(const float *Q, uint32_t nq, const uint32_t *k, uint32_t kmax, const hkg_query_opts *qo,
+                           uint64_t *ids, float *d2, hkg_query_stats *stats)
+{
+  if (!Q || !ids || !d2) {
+    set_error("null argument");
+    return HKG_EINVAL;
+  }
+  if (nq == 0) return HKG_OK;
+  HKG_HIP_TRY(hipSetDevice(dev.device));
+  std::vector<uint32_t> kv(nq, kmax);
+  if (k) {
+    for (uint32_t i = 0; i < nq; ++i) {
+      if (k[i] == 0 || k[i] > kmax) {
+        set_error("per-query k must be in [1, kmax]");
+        return HKG_EINVAL;
+      }
+      kv[i] = k[i];
+    }
+  }
+  float *dQ = nullptr, *dd2 = nullptr;
+  uint32_t *dk = nullptr;
+  uint64_t *dids = nullptr;
+  hkg_status rc;
+  if ((rc = dalloc(&dQ, (uint64_t)nq * dev.L, nullptr)) != HKG_OK) return rc;
+  if ((rc = dalloc(&dk, nq, nullptr)) != HKG_OK) {
+    dfree(dQ);
+    return rc;
+  }
+  if ((rc = dalloc(&dids, (uint64_t)nq * kmax, nullptr)) != HKG_OK || (rc = dalloc(&dd2, (uint64_t)nq * kmax, nullptr)) != HKG_OK) {
+    dfree(dQ);
+    dfree(dk);
+    dfree(dids);
+    return rc;
+  }
+  hipError_t e1 = hipMemcpy(dQ, Q, (size_t)nq * dev.L * 4, hipMemcpyHostToDevice);
+  hipError_t e2 = hipMemcpy(dk, kv.data(), (size_t)nq * 4, hipMemcpyHostToDevice);
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    set_error("query upload failed");
+    rc = HKG_EDEVICE;
+  } else {
+    rc = knn_device(dQ, nq, dk, kmax, qo, dids, dd2, stream, stats);
+  }
+  if (rc == HKG_OK) {
+    if (hipMemcpy(ids, dids, (size_t)nq * kmax * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(d2, dd2, (size_t)nq * kmax * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+      set_error("result download failed");
+      rc = HKG_EDEVICE;
+    }
+  }
+  dfree(dQ);
+  dfree(dk);
+  dfree(dids);
+  dfree(dd2);
+  return rc;
+}
+
+}  // namespace hkg
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+using hkg::Index;
+
+struct hkg_index {
+  Index impl;
+};
+
+extern "C" {
+
+void hkg_default_options(hkg_options *opt)
+{
+  if (!opt) return;
+  memset(opt, 0, sizeof(*opt));
+}
+
+void hkg_default_query_opts(hkg_query_opts *q)
+{
+  if (!q) return;
+  memset(q, 0, sizeof(*q));
+  q->path = HKG_PATH_AUTO;
+}
+
+const char *hkg_status_str(hkg_status s)
+{
+  switch (s) {
+  case HKG_OK: return "ok";
+  case HKG_EINVAL: return "invalid argument";
+  case HKG_ENOMEM: return "out of memory";
+  case HKG_EIO: return "i/o error";
+  case HKG_EFORMAT: return "malformed index";
+  case HKG_EDEVICE: return "device error";
+  case HKG_EBUSY: return "busy";
+  case HKG_ENOTREADY: return "not ready";
+  }
+  return "unknown status";
+}
+
+const char *hkg_last_error(void) { return hkg::last_error().c_str(); }
+
+void hkg_breakpoints256(float *out255) { hkg::host_breakpoints(out255); }
+
+hkg_status hkg_open_dir(const char *index_dir, const hkg_options *opt, hkg_index **out)
+{
+  if (!out) return HKG_EINVAL;
+  Index *ix = nullptr;
+  hkg_status rc = hkg::open_dir(index_dir, opt, &ix);
+  *out = reinterpret_cast<hkg_index *>(ix);
+  return rc;
+}
+
+hkg_status hkg_open_raw(const float *X, uint64_t n, uint32_t L, const hkg_options *opt, hkg_index **out)
+{
+  if (!out) return HKG_EINVAL;
+  Index *ix = nullptr;
+  hkg_status rc = hkg::open_raw(X, n, L, opt, &ix);
+  *out = reinterpret_cast<hkg_index *>(ix);
+  return rc;
+}
+
+hkg_status hkg_set_ids(hkg_index *ix, const uint64_t *ids, uint64_t n)
+{
+  if (!ix || !ids) return HKG_EINVAL;
+  return reinterpret_cast<Index *>(ix)->set_ids(ids, n);
+}
+
+hkg_status hkg_inspect_dir(const char *index_dir, hkg_index_info *info)
+{
+  if (!index_dir || !info) return HKG_EINVAL;
+  std::string d(index_dir);
+  if (!d.empty() && d.back() != '/') d += '/';
+  hkg::HostTree t;
+  std::string err;
+  hkg_status rc = hkg::parse_root_idx(d + "root.idx", t, err);
+  if (rc != HKG_OK) {
+    hkg::set_error(d + "root.idx: " + err);
+    return rc;
+  }
+  struct stat sr, ss;
+  if (stat((d + "leaves_raw.idx").c_str(), &sr) != 0 || stat((d + "leaves_sims.idx").c_str(), &ss) != 0) {
+    hkg::set_error(d + ": leaves_raw.idx or leaves_sims.idx missing");
+    return HKG_EIO;
+  }
+  if ((uint64_t)sr.st_size != t.n_series * t.L * 4 ||
+      (uint64_t)ss.st_size != t.n_series * (uint64_t)t.paa_segments) {
+    hkg::set_error(d + ": leaf file sizes do not match root.idx");
+    return HKG_EFORMAT;
+  }
+  memset(info, 0, sizeof(*info));
+  info->n_series = t.n_series;
+  info->length = t.L;
+  info->paa_segments = (uint32_t)t.paa_segments;
+  info->sax_bit_cardinality = (uint32_t)t.sax_bit_cardinality;
+  info->max_leaf_size = t.max_leaf_size;
+  info->n_leaves = t.leaves.size();
+  info->n_nodes = t.nodes.size();
+  info->has_tree = 1;
+  return HKG_OK;
+}
+
+hkg_status hkg_info(const hkg_index *ix, hkg_index_info *info)
+{
+  if (!ix || !info) return HKG_EINVAL;
+  const Index *p = reinterpret_cast<const Index *>(ix);
+  memset(info, 0, sizeof(*info));
+  info->n_series = p->dev.n;
+  info->length = p->dev.L;
+  info->paa_segments = hkg::kPaaSegments;
+  info->sax_bit_cardinality = 8;
+  info->max_leaf_size = p->dev.tree.max_leaf_size;
+  info->n_leaves = p->dev.nleaf;
+  info->n_nodes = p->dev.tree.nodes.size();
+  info->device_bytes = p->dev.device_bytes;
+  info->has_tree = p->dev.has_tree;
+  info->has_ids = p->dev.ids != nullptr;
+  return HKG_OK;
+}
+
+void hkg_close(hkg_index *ix)
+{
+  delete reinterpret_cast<Index *>(ix);
+}
+
+hkg_status hkg_knn(hkg_index *ix, const float *Q, uint32_t nq, const uint32_t *k, uint32_t kmax,
+                   const hkg_query_opts *qo, uint64_t *ids, float *d2, hkg_query_stats *stats)
+{
+  if (!ix) return HKG_EINVAL;
+  return reinterpret_cast<Index *>(ix)->knn_host(Q, nq, k, kmax, qo, ids, d2, stats);
+}
+
+hkg_status hkg_knn_device(hkg_index *ix, const float *dQ, uint32_t nq, const uint32_t *dk, uint32_t kmax,
+                          const hkg_query_opts *qo, uint64_t *dids, float *dd2, void *stream,
+                          hkg_query_stats *stats)
+{
+  if (!ix || !dQ || !dk || !dids || !dd2) return HKG_EINVAL;
+  return reinterpret_cast<Index *>(ix)->knn_device(dQ, nq, dk, kmax, qo, dids, dd2, (hipStream_t)stream, stats);
+}
+
+}  // extern "C"

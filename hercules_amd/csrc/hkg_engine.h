@@ -1,0 +1,81 @@
+// Engine objects behind the C ABI handles.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+
+#include "hkg_internal.h"
+
+namespace hkg {
+
+const std::string &last_error();
+
+struct Counters {
+  unsigned long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t overflow = 0;
+  uint32_t nwork = 0;
+};
+
+// Device scratch of one batch shape.  All per-query arrays are B long.
+struct QueryBuffers {
+  uint32_t B = 0, kmax = 0, cap = 0, work_cap = 0;
+  uint64_t bytes = 0;
+  float *Q = nullptr;           // B x L
+  uint32_t *kq = nullptr;       // B
+  float *paa = nullptr;         // B x 16
+  double *psum = nullptr;       // B x 2(L+1)
+  double *qn2 = nullptr;        // B
+  float *lb = nullptr;          // B x nleaf
+  uint32_t *best = nullptr;     // B
+  uint32_t *bound = nullptr;    // B, FP32 bits of the current k-th distance bound
+  uint64_t *cand = nullptr;     // B x cap candidate keys
+  uint32_t *cand_cnt = nullptr; // B
+  uint64_t *work = nullptr;     // work_cap (query, leaf) items
+  uint32_t *misc = nullptr;     // [0] nwork [1] work_next [2] overflow
+  unsigned long long *counters = nullptr;
+  uint64_t *out_ids = nullptr;  // B x kmax
+  float *out_d2 = nullptr;      // B x kmax
+  // BF16 path scratch
+  uint32_t *bf_cnt = nullptr;   // per query survivor counters
+  uint64_t *bf_surv = nullptr;  // survivors (query << 32 | row)
+  uint32_t bf_cap = 0;
+  uint16_t *Qb = nullptr;       // B x L bf16 bits
+  float *qerr = nullptr;        // B
+  // captured stage sequence of the batched path
+  hipGraphExec_t graph_exec = nullptr;
+  hipStream_t graph_stream = nullptr;
+  ~QueryBuffers();
+};
+
+struct Index {
+  DevIndex dev;
+  hipStream_t stream = nullptr;
+  uint64_t *idmap64 = nullptr;  // rank -> public id (when an id map is attached)
+  std::map<uint64_t, std::unique_ptr<QueryBuffers>> bufs;
+
+  ~Index();
+  hkg_status init_device(int device);
+  hkg_status upload_tree();
+  hkg_status set_ids(const uint64_t *ids, uint64_t n);
+  hkg_status make_bf16();
+  hkg_status get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuffers **out);
+  uint32_t cand_cap(hkg_path path, uint32_t nq, uint32_t kmax) const;
+  void enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st);
+  void enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st);
+  hkg_status run_batch(hkg_path path, const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax,
+                       bool use_graph, hipStream_t st, QueryBuffers **used, Counters *ctr);
+  hkg_status run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax, hipStream_t st,
+                            QueryBuffers **used, Counters *ctr);
+  hkg_status fix_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax, hipStream_t st, uint32_t *retries);
+  hkg_status knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, uint32_t kmax, const hkg_query_opts *qo,
+                        uint64_t *dids, float *dd2, hipStream_t st, hkg_query_stats *stats);
+  hkg_status knn_host(const float *Q, uint32_t nq, const uint32_t *k, uint32_t kmax, const hkg_query_opts *qo,
+                      uint64_t *ids, float *d2, hkg_query_stats *stats);
+};
+
+hkg_status open_dir(const char *dir, const hkg_options *opt, Index **out);
+hkg_status open_raw(const float *X, uint64_t n, uint32_t L, const hkg_options *opt, Index **out);
+uint32_t scan_grid_x(uint32_t nq);
+
+}  // namespace hkg

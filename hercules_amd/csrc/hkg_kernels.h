@@ -1,0 +1,51 @@
+// Launchers of the query kernels (hkg_kernels.hip).
+#pragma once
+
+#include "hkg_internal.h"
+
+namespace hkg {
+
+constexpr int kRefineTile = 2048;        // floats of chunk sums per wave (refine)
+constexpr int kScanQ = 4;                // queries per block in the full scan
+constexpr int kScanTile = 2048;          // floats of chunk sums per wave (scan)
+constexpr size_t kEapcaLdsBytes = 48 * 1024;
+
+// device counters (unsigned long long each)
+enum Counter : int {
+  kCtrLeaves = 0,      // (query, leaf) pairs after the EAPCA bound (excl. best leaf)
+  kCtrSaxChecked = 1,  // series whose iSAX bound was evaluated
+  kCtrRefined = 2,     // series whose exact distance was computed
+  kCtrBf16 = 3,        // series whose BF16 bound was evaluated
+  kNumCounters = 8
+};
+
+hipError_t upload_breakpoints_dev();
+
+void launch_reset(hipStream_t st, uint32_t nq, uint32_t *bound, uint32_t *cand_cnt);
+void launch_fill(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v);
+void launch_prep(hipStream_t st, const float *Q, uint32_t nq, uint32_t L, float *paa, double *psum,
+                 double *qn2);
+void launch_eapca(hipStream_t st, uint32_t nq, uint32_t nleaf, uint32_t L, const uint32_t *seg_off,
+                  const int16_t *seg_end, const float4 *ind, const double *psum, float *lb);
+void launch_best_leaf(hipStream_t st, uint32_t nq, uint32_t nleaf, const float *lb, uint32_t *best);
+void launch_scan_best(hipStream_t st, uint32_t nq, const float *X, uint32_t L, const float *Q,
+                      const uint32_t *kq, const uint32_t *best, const uint64_t *leaf_pos,
+                      const uint32_t *leaf_size, const uint32_t *ids, uint32_t cap, uint64_t *cand,
+                      uint32_t *cand_cnt, uint32_t *bound, unsigned long long *counters);
+void launch_collect(hipStream_t st, uint32_t nq, uint32_t nleaf, const float *lb, const uint32_t *best,
+                    const uint32_t *bound, const uint32_t *leaf_size, uint64_t *work, uint32_t work_cap,
+                    uint32_t *nwork, unsigned long long *counters);
+void launch_filter_refine(hipStream_t st, uint32_t grid, const float *X, const uint8_t *sax, uint32_t L,
+                          const float *Q, const float *paa, const uint32_t *kq, const uint64_t *leaf_pos,
+                          const uint32_t *leaf_size, const uint32_t *ids, const uint64_t *work,
+                          const uint32_t *nwork, uint32_t work_cap, uint32_t *work_next, uint32_t cap,
+                          uint64_t *cand, uint32_t *cand_cnt, uint32_t *bound,
+                          unsigned long long *counters);
+void launch_scan(hipStream_t st, uint32_t grid_x, const float *X, uint64_t n, uint32_t L, const float *Q,
+                 uint32_t nq, const uint32_t *kq, const uint32_t *ids, uint32_t cap, uint64_t *cand,
+                 uint32_t *cand_cnt, uint32_t *bound);
+void launch_select(hipStream_t st, uint32_t nq, uint32_t kmax, const uint32_t *kq, uint32_t cap,
+                   const uint64_t *cand, const uint32_t *cand_cnt, const uint64_t *idmap64, uint64_t *out_ids,
+                   float *out_d2, uint32_t *overflow);
+
+}  // namespace hkg

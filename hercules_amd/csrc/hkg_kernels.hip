@@ -1,0 +1,726 @@
+// Query kernels of libhkg for gfx950 (MI355X).
+//
+// Compiled with -ffp-contract=off: the squared distance and the iSAX lower
+// bound must round every operation separately, in the reference's order
+// (ts.c:83-109, sax.c:310-479), so their results are bit-identical to the
+// reference's FP32 values.
+//
+// Layout (HBM, leaf order): X[n][L] f32, SAX[n][16] u8, per-leaf tables.
+// A candidate is a 64-bit key  (d2 bits << 32) | public id : FP32 squared
+// distances are non-negative, so the bit pattern orders like the value and
+// sorting keys orders by (d2, id).
+#include "hkg_kernels.h"
+
+#include <float.h>
+
+namespace hkg {
+
+__constant__ float c_bp[256];  // card-256 breakpoints, [255] unused
+
+static const float h_bp[256] = {
+#include "sax_bp256.inc"
+    0.0f};
+
+void host_breakpoints(float *out255)
+{
+  for (int i = 0; i < 255; ++i) out255[i] = h_bp[i];
+}
+
+hipError_t upload_breakpoints_dev()
+{
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_bp), h_bp, sizeof(h_bp));
+}
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+typedef float nf4 __attribute__((ext_vector_type(4)));
+
+// streaming 16-byte load (the rows are touched once per query)
+__device__ __forceinline__ float4 ld_stream(const float *p)
+{
+  nf4 v = __builtin_nontemporal_load(reinterpret_cast<const nf4 *>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// one 8-float chunk of ts_euclidean_distance_SIMD: separate sub and mul
+// roundings, then the two-hadd tree ((d0+d1)+(d2+d3)) + ((d4+d5)+(d6+d7))
+__device__ __forceinline__ float ed_chunk(float4 qa, float4 qb, float4 xa, float4 xb)
+{
+  float d0 = qa.x - xa.x, d1 = qa.y - xa.y, d2 = qa.z - xa.z, d3 = qa.w - xa.w;
+  float d4 = qb.x - xb.x, d5 = qb.y - xb.y, d6 = qb.z - xb.z, d7 = qb.w - xb.w;
+  d0 = d0 * d0; d1 = d1 * d1; d2 = d2 * d2; d3 = d3 * d3;
+  d4 = d4 * d4; d5 = d5 * d5; d6 = d6 * d6; d7 = d7 * d7;
+  return ((d0 + d1) + (d2 + d3)) + ((d4 + d5) + (d6 + d7));
+}
+
+// minidist_paa_to_isax_raw_SIMD for full-cardinality 16-symbol words
+__device__ __forceinline__ float lb_sax_word(const float *paa, uint4 w, const float *bp, float ratio)
+{
+  uint32_t words[4] = {w.x, w.y, w.z, w.w};
+  float t[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    int v = (words[i >> 2] >> ((i & 3) * 8)) & 0xFF;
+    float lo = v == 0 ? -2000000.0f : bp[(v - 1) & 255];
+    float hi = v == 255 ? 2000000.0f : bp[v];
+    float p = paa[i];
+    float r;
+    if (lo > p) {
+      float d = lo - p;
+      r = d * d;
+    } else if (hi < p) {
+      float d = hi - p;
+      r = d * d;
+    } else {
+      r = 0.0f;
+    }
+    t[i] = r;
+  }
+  float w0 = t[0] + t[8], w1 = t[1] + t[9], w2 = t[2] + t[10], w3 = t[3] + t[11];
+  float w4 = t[4] + t[12], w5 = t[5] + t[13], w6 = t[6] + t[14], w7 = t[7] + t[15];
+  float s = ((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7));
+  return s * ratio;
+}
+
+__device__ __forceinline__ bool lb_keep(float lb, uint32_t bound_bits)
+{
+  if (bound_bits >= kInfBits) return true;
+  float b = __uint_as_float(bound_bits);
+  return lb <= b * (1.0f + kLbRel) + kLbAbs;
+}
+
+__device__ __forceinline__ uint64_t make_key(float d2, uint32_t id)
+{
+  return ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)id;
+}
+
+// ---------------------------------------------------------------------------
+// block top-k of distinct distances (256 threads)
+// ---------------------------------------------------------------------------
+struct TopK {
+  uint64_t keys[kTopKCap];  // [0,cnt) sorted list, [256, 256+nstage) staged
+  int cnt;
+  int nstage;
+  uint32_t kth_bits;        // bits of the k-th distinct d2, or kInfBits
+};
+
+__device__ __forceinline__ void topk_reset(TopK &s)
+{
+  if (threadIdx.x == 0) {
+    s.cnt = 0;
+    s.nstage = 0;
+    s.kth_bits = kInfBits;
+  }
+}
+
+// append one candidate (any thread); capacity 256 per merge round
+__device__ __forceinline__ void topk_stage(TopK &s, uint64_t key)
+{
+  int slot = atomicAdd(&s.nstage, 1);
+  s.keys[256 + slot] = key;
+}
+
+// all threads; call after a __syncthreads that follows the staging
+__device__ void topk_merge(TopK &s, uint32_t *scan, int k)
+{
+  const int tid = threadIdx.x;
+  const int n = s.cnt, m = s.nstage;
+  if (m == 0) return;  // uniform: every thread read the same values
+  for (int i = tid; i < kTopKCap; i += kBlock) {
+    if ((i >= n && i < 256) || i >= 256 + m) s.keys[i] = ~0ull;
+  }
+  __syncthreads();
+  // bitonic sort of 512 keys, one compare-exchange per thread per stage
+  for (int size = 2; size <= kTopKCap; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      int i = ((tid / stride) * 2 * stride) + (tid % stride);
+      int j = i + stride;
+      uint64_t a = s.keys[i], b = s.keys[j];
+      bool up = (i & size) == 0;
+      if ((a > b) == up) {
+        s.keys[i] = b;
+        s.keys[j] = a;
+      }
+      __syncthreads();
+    }
+  }
+  // keep the first key of every distinct distance (smallest id)
+  uint32_t f0, f1;
+  {
+    int i0 = 2 * tid, i1 = 2 * tid + 1;
+    uint64_t k0 = s.keys[i0], k1 = s.keys[i1];
+    uint64_t km = i0 > 0 ? s.keys[i0 - 1] : ~0ull;
+    f0 = (k0 != ~0ull) && (i0 == 0 || (uint32_t)(km >> 32) != (uint32_t)(k0 >> 32));
+    f1 = (k1 != ~0ull) && ((uint32_t)(k0 >> 32) != (uint32_t)(k1 >> 32));
+    scan[tid] = f0 + f1;
+  }
+  __syncthreads();
+  // inclusive scan of 256 pair-counts (Hillis-Steele, small)
+  for (int off = 1; off < kBlock; off <<= 1) {
+    uint32_t v = tid >= off ? scan[tid - off] : 0;
+    __syncthreads();
+    scan[tid] += v;
+    __syncthreads();
+  }
+  uint32_t excl = tid > 0 ? scan[tid - 1] : 0;
+  uint32_t total = scan[kBlock - 1];
+  uint64_t k0 = s.keys[2 * tid], k1 = s.keys[2 * tid + 1];
+  __syncthreads();
+  // compact into the upper half (it is free now), then move down
+  if (f0 && excl < (uint32_t)k) s.keys[256 + excl] = k0;
+  if (f1 && excl + f0 < (uint32_t)k) s.keys[256 + excl + f0] = k1;
+  __syncthreads();
+  int nc = total < (uint32_t)k ? (int)total : k;
+  if (tid < nc) s.keys[tid] = s.keys[256 + tid];
+  __syncthreads();
+  if (tid == 0) {
+    s.cnt = nc;
+    s.nstage = 0;
+    s.kth_bits = nc == k ? (uint32_t)(s.keys[k - 1] >> 32) : kInfBits;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// cooperative exact distances for a list of rows held in LDS
+// ---------------------------------------------------------------------------
+// Each (row, chunk) pair of a group is one lane-task: lanes of a wave cover
+// consecutive chunks of a row (coalesced 32-byte reads), write the chunk
+// sums into a padded LDS tile, then one lane per row folds its chunk sums
+// in order (the reference's sequential accumulation).  Rows are staged as
+// candidates when d2 <= bound (d2 equal to the bound is kept for the
+// smallest-id rule).
+struct RefineShared {
+  float csum[kBlock / 64][kRefineTile];  // per wave
+};
+
+__device__ void refine_rows(const float *__restrict__ X, uint32_t L, const float *qs,
+                            const uint64_t *rows, int nrows, const uint32_t *__restrict__ ids,
+                            uint32_t bound_bits, TopK &tk, RefineShared &rs, uint32_t &nrefined)
+{
+  const int cpr = (int)(L >> 3);
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  // rows per wave group: the padded chunk tile must fit kRefineTile floats
+  int g = kRefineTile / (cpr + 1);
+  if (g > 64) g = 64;
+  const int per_block = g * (kBlock / 64);
+  float *cs = rs.csum[wave];
+  for (int base = 0; base < nrows; base += per_block) {
+    int wbeg = base + wave * g;
+    int cnt = nrows - wbeg;
+    if (cnt > g) cnt = g;
+    if (cnt < 0) cnt = 0;
+    const int tasks = cnt * cpr;
+    for (int s = lane; s < tasks; s += 64) {
+      int r = s / cpr;
+      int c = s - r * cpr;
+      uint64_t row = rows[wbeg + r];
+      const float *xp = X + row * L + (uint64_t)c * 8;
+      float4 xa = ld_stream(xp);
+      float4 xb = ld_stream(xp + 4);
+      const float4 *qp = reinterpret_cast<const float4 *>(qs + c * 8);
+      cs[r * (cpr + 1) + c] = ed_chunk(qp[0], qp[1], xa, xb);
+    }
+    __syncthreads();
+    if (lane < cnt) {
+      const float *p = cs + lane * (cpr + 1);
+      float acc = 0.0f;
+      for (int c = 0; c < cpr; ++c) acc = acc + p[c];
+      uint64_t row = rows[wbeg + lane];
+      if (__float_as_uint(acc) <= bound_bits) {
+        uint32_t id = ids ? ids[row] : (uint32_t)row;
+        topk_stage(tk, make_key(acc, id));
+      }
+    }
+    nrefined += (lane < cnt) ? 1u : 0u;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+// PAA (sax.c:46-56), prefix sums in FP64 for the EAPCA bound, ||q||^2 (FP64)
+__global__ void __launch_bounds__(64) k_prep(const float *__restrict__ Q, uint32_t L,
+                                             float *__restrict__ paa, double *__restrict__ psum,
+                                             double *__restrict__ qn2)
+{
+  const uint32_t q = blockIdx.x;
+  const float *x = Q + (uint64_t)q * L;
+  const int lane = threadIdx.x;
+  const int per = (int)(L / kPaaSegments);
+  if (lane < kPaaSegments) {
+    float a = 0.0f;
+    for (int i = 0; i < per; ++i) a = a + x[lane * per + i];
+    paa[q * kPaaSegments + lane] = a / (float)per;
+  }
+  if (lane == 0) {
+    double *p1 = psum + (uint64_t)q * 2 * (L + 1);
+    double *p2 = p1 + (L + 1);
+    double s1 = 0.0, s2 = 0.0;
+    p1[0] = 0.0;
+    p2[0] = 0.0;
+    for (uint32_t i = 0; i < L; ++i) {
+      double v = (double)x[i];
+      s1 += v;
+      s2 += v * v;
+      p1[i + 1] = s1;
+      p2[i + 1] = s2;
+    }
+    qn2[q] = s2;
+  }
+}
+
+// LB_EAPCA (hercules_node.c:879-924) for every (query, leaf): the query's
+// per-segment mean and sd come from FP64 prefix sums instead of the
+// reference's FP32 sums; the bound is used only with the certified slack.
+__global__ void __launch_bounds__(kBlock) k_eapca(uint32_t nleaf, uint32_t L,
+                                                  const uint32_t *__restrict__ seg_off,
+                                                  const int16_t *__restrict__ seg_end,
+                                                  const float4 *__restrict__ ind,
+                                                  const double *__restrict__ psum,
+                                                  float *__restrict__ lb)
+{
+  extern __shared__ double sp[];
+  const uint32_t q = blockIdx.y;
+  const double *gp = psum + (uint64_t)q * 2 * (L + 1);
+  const bool use_lds = 2 * (L + 1) * sizeof(double) <= kEapcaLdsBytes;
+  if (use_lds) {
+    for (uint32_t i = threadIdx.x; i < 2 * (L + 1); i += kBlock) sp[i] = gp[i];
+    __syncthreads();
+  }
+  const double *p1 = use_lds ? sp : gp;
+  const double *p2 = p1 + (L + 1);
+  uint32_t leaf = blockIdx.x * kBlock + threadIdx.x;
+  if (leaf >= nleaf) return;
+  uint32_t b = seg_off[leaf], e = seg_off[leaf + 1];
+  double sum = 0.0;
+  int start = 0;
+  for (uint32_t s = b; s < e; ++s) {
+    int end = seg_end[s];
+    double n = (double)(end - start);
+    double mu = (p1[end] - p1[start]) / n;
+    double var = (p2[end] - p2[start]) / n - mu * mu;
+    double sd = var > 0.0 ? sqrt(var) : 0.0;
+    float4 g = ind[s];  // {max mean, min mean, max sd, min sd}
+    double t = 0.0;
+    if ((sd - g.z) * (sd - g.w) > 0.0) {
+      double u = fmin(fabs(sd - (double)g.z), fabs(sd - (double)g.w));
+      t += u * u;
+    }
+    if ((mu - g.x) * (mu - g.y) > 0.0) {
+      double u = fmin(fabs(mu - (double)g.x), fabs(mu - (double)g.y));
+      t += u * u;
+    }
+    sum += t * n;
+    start = end;
+  }
+  lb[(uint64_t)q * nleaf + leaf] = (float)sum;
+}
+
+// per query: the leaf with the smallest bound (ties: smallest leaf index)
+__global__ void __launch_bounds__(kBlock) k_best_leaf(uint32_t nleaf, const float *__restrict__ lb,
+                                                      uint32_t *__restrict__ best)
+{
+  __shared__ uint64_t red[kBlock];
+  const uint32_t q = blockIdx.x;
+  uint64_t m = ~0ull;
+  for (uint32_t i = threadIdx.x; i < nleaf; i += kBlock) {
+    uint64_t key = ((uint64_t)__float_as_uint(lb[(uint64_t)q * nleaf + i]) << 32) | i;
+    m = key < m ? key : m;
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      uint64_t o = red[threadIdx.x + off];
+      if (o < red[threadIdx.x]) red[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) best[q] = (uint32_t)(red[0] & 0xFFFFFFFFu);
+}
+
+__device__ __forceinline__ void load_query(const float *__restrict__ Q, uint32_t q, uint32_t L, float *qs)
+{
+  const float4 *src = reinterpret_cast<const float4 *>(Q + (uint64_t)q * L);
+  float4 *dst = reinterpret_cast<float4 *>(qs);
+  for (uint32_t i = threadIdx.x; i < L / 4; i += kBlock) dst[i] = src[i];
+}
+
+// flush a block-local list into the query's global candidate buffer
+__device__ void flush_topk(TopK &tk, int k, uint32_t q, uint32_t cap, uint64_t *__restrict__ cand,
+                           uint32_t *__restrict__ cand_cnt, uint32_t *__restrict__ bound)
+{
+  __shared__ uint32_t s_base, s_bound;
+  if (threadIdx.x == 0) {
+    if (tk.cnt == k) atomicMin(&bound[q], tk.kth_bits);
+    s_bound = __atomic_load_n(&bound[q], __ATOMIC_RELAXED);
+    // count what survives the current global bound (the list is sorted)
+    int nk = 0;
+    while (nk < tk.cnt && (uint32_t)(tk.keys[nk] >> 32) <= s_bound) ++nk;
+    s_base = nk ? atomicAdd(&cand_cnt[q], (uint32_t)nk) : 0;
+    tk.nstage = nk;  // reuse as the flushed count
+  }
+  __syncthreads();
+  int nk = tk.nstage;
+  for (int i = threadIdx.x; i < nk; i += kBlock) {
+    uint32_t slot = s_base + i;
+    if (slot < cap) cand[(uint64_t)q * cap + slot] = tk.keys[i];
+  }
+  __syncthreads();
+}
+
+// approximate phase: the best leaf of every query, scanned in full
+__global__ void __launch_bounds__(kBlock) k_scan_best(const float *__restrict__ X, uint32_t L,
+                                                      const float *__restrict__ Q,
+                                                      const uint32_t *__restrict__ kq,
+                                                      const uint32_t *__restrict__ best,
+                                                      const uint64_t *__restrict__ leaf_pos,
+                                                      const uint32_t *__restrict__ leaf_size,
+                                                      const uint32_t *__restrict__ ids, uint32_t cap,
+                                                      uint64_t *__restrict__ cand,
+                                                      uint32_t *__restrict__ cand_cnt,
+                                                      uint32_t *__restrict__ bound,
+                                                      unsigned long long *__restrict__ counters)
+{
+  extern __shared__ float qs[];  // L floats
+  __shared__ TopK tk;
+  __shared__ RefineShared rs;
+  __shared__ uint64_t rows[kBlock];
+  __shared__ uint32_t scan[kBlock];
+  const uint32_t q = blockIdx.x;
+  const int k = (int)kq[q];
+  load_query(Q, q, L, qs);
+  topk_reset(tk);
+  const uint32_t leaf = best[q];
+  const uint64_t pos = leaf_pos[leaf];
+  const uint32_t size = leaf_size[leaf];
+  uint32_t nref = 0;
+  __syncthreads();
+  for (uint32_t t0 = 0; t0 < size; t0 += kBlock) {
+    uint32_t nr = size - t0 < (uint32_t)kBlock ? size - t0 : (uint32_t)kBlock;
+    if (threadIdx.x < nr) rows[threadIdx.x] = pos + t0 + threadIdx.x;
+    __syncthreads();
+    refine_rows(X, L, qs, rows, (int)nr, ids, tk.kth_bits, tk, rs, nref);
+    __syncthreads();
+    topk_merge(tk, scan, k);
+  }
+  flush_topk(tk, k, q, cap, cand, cand_cnt, bound);
+  if (threadIdx.x == 0) atomicAdd(&counters[kCtrRefined], (unsigned long long)size);
+}
+
+// work list of (query, leaf) pairs that survive the EAPCA bound
+__global__ void __launch_bounds__(kBlock) k_collect(uint32_t nq, uint32_t nleaf,
+                                                    const float *__restrict__ lb,
+                                                    const uint32_t *__restrict__ best,
+                                                    const uint32_t *__restrict__ bound,
+                                                    const uint32_t *__restrict__ leaf_size,
+                                                    uint64_t *__restrict__ work, uint32_t work_cap,
+                                                    uint32_t *__restrict__ nwork,
+                                                    unsigned long long *__restrict__ counters)
+{
+  const uint32_t q = blockIdx.y;
+  const uint32_t leaf = blockIdx.x * kBlock + threadIdx.x;
+  bool keep = false;
+  if (leaf < nleaf && leaf != best[q]) keep = lb_keep(lb[(uint64_t)q * nleaf + leaf], bound[q]);
+  // wave-aggregated append
+  uint64_t mask = __ballot(keep);
+  int lane = threadIdx.x & 63;
+  uint32_t wbase = 0;
+  int leader = __ffsll((long long)mask) - 1;
+  if (mask) {
+    if (lane == leader) wbase = atomicAdd(nwork, (uint32_t)__popcll(mask));
+    wbase = __shfl(wbase, leader);
+  }
+  if (keep) {
+    uint32_t slot = wbase + (uint32_t)__popcll(mask & ((1ull << lane) - 1));
+    if (slot < work_cap) work[slot] = ((uint64_t)q << 32) | leaf;
+    atomicAdd(&counters[kCtrSaxChecked], (unsigned long long)leaf_size[leaf]);
+  }
+  if (lane == leader && mask) atomicAdd(&counters[kCtrLeaves], (unsigned long long)__popcll(mask));
+}
+
+// persistent (query, leaf) work queue: iSAX filter + exact refinement
+__global__ void __launch_bounds__(kBlock) k_filter_refine(
+    const float *__restrict__ X, const uint8_t *__restrict__ sax, uint32_t L,
+    const float *__restrict__ Q, const float *__restrict__ paa, const uint32_t *__restrict__ kq,
+    const uint64_t *__restrict__ leaf_pos, const uint32_t *__restrict__ leaf_size,
+    const uint32_t *__restrict__ ids, const uint64_t *__restrict__ work,
+    const uint32_t *__restrict__ nwork_p, uint32_t work_cap, uint32_t *__restrict__ work_next, uint32_t cap,
+    uint64_t *__restrict__ cand, uint32_t *__restrict__ cand_cnt, uint32_t *__restrict__ bound,
+    unsigned long long *__restrict__ counters)
+{
+  extern __shared__ float qs[];
+  __shared__ TopK tk;
+  __shared__ RefineShared rs;
+  __shared__ uint64_t rows[kBlock];
+  __shared__ uint32_t scan[kBlock];
+  __shared__ float bp[256];
+  __shared__ float spaa[kPaaSegments];
+  __shared__ uint32_t s_item, s_nsurv, s_bound;
+  for (int i = threadIdx.x; i < 256; i += kBlock) bp[i] = c_bp[i];
+  uint32_t nwork = *nwork_p;
+  if (nwork > work_cap) nwork = work_cap;
+  const float ratio = (float)(L / kPaaSegments);
+  uint32_t nref = 0;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = atomicAdd(work_next, 1u);
+    __syncthreads();
+    const uint32_t item = s_item;
+    if (item >= nwork) break;
+    const uint64_t w = work[item];
+    const uint32_t q = (uint32_t)(w >> 32), leaf = (uint32_t)w;
+    const int k = (int)kq[q];
+    load_query(Q, q, L, qs);
+    if (threadIdx.x < kPaaSegments) spaa[threadIdx.x] = paa[q * kPaaSegments + threadIdx.x];
+    topk_reset(tk);
+    const uint64_t pos = leaf_pos[leaf];
+    const uint32_t size = leaf_size[leaf];
+    for (uint32_t t0 = 0; t0 < size; t0 += kBlock) {
+      if (threadIdx.x == 0) {
+        uint32_t gb = __atomic_load_n(&bound[q], __ATOMIC_RELAXED);
+        s_bound = gb < tk.kth_bits ? gb : tk.kth_bits;
+        s_nsurv = 0;
+      }
+      __syncthreads();
+      const uint32_t bnd = s_bound;
+      const uint32_t r = t0 + threadIdx.x;
+      bool keep = false;
+      if (r < size) {
+        uint4 word = *reinterpret_cast<const uint4 *>(sax + (pos + r) * kPaaSegments);
+        keep = lb_keep(lb_sax_word(spaa, word, bp, ratio), bnd);
+      }
+      uint64_t mask = __ballot(keep);
+      int lane = threadIdx.x & 63;
+      int leader = __ffsll((long long)mask) - 1;
+      uint32_t wbase = 0;
+      if (mask) {
+        if (lane == leader) wbase = atomicAdd(&s_nsurv, (uint32_t)__popcll(mask));
+        wbase = __shfl(wbase, leader);
+      }
+      if (keep) rows[wbase + __popcll(mask & ((1ull << lane) - 1))] = pos + r;
+      __syncthreads();
+      const int ns = (int)s_nsurv;
+      refine_rows(X, L, qs, rows, ns, ids, bnd, tk, rs, nref);
+      __syncthreads();
+      topk_merge(tk, scan, k);
+    }
+    flush_topk(tk, k, q, cap, cand, cand_cnt, bound);
+  }
+  if ((threadIdx.x & 63) == 0 && nref) atomicAdd(&counters[kCtrRefined], (unsigned long long)nref);
+}
+
+// PSCAN: exact full scan, a group of up to kScanQ queries per block,
+// persistent over row tiles
+__global__ void __launch_bounds__(kBlock) k_scan(const float *__restrict__ X, uint64_t n, uint32_t L,
+                                                 const float *__restrict__ Q, uint32_t nq,
+                                                 const uint32_t *__restrict__ kq,
+                                                 const uint32_t *__restrict__ ids, uint32_t cap,
+                                                 uint64_t *__restrict__ cand,
+                                                 uint32_t *__restrict__ cand_cnt,
+                                                 uint32_t *__restrict__ bound)
+{
+  extern __shared__ float qs[];  // kScanQ * L
+  __shared__ TopK tk[kScanQ];
+  __shared__ float csum[kBlock / 64][kScanTile];
+  __shared__ uint32_t scan[kBlock];
+  const uint32_t q0 = blockIdx.y * kScanQ;
+  const int nqg = (int)(nq - q0 < (uint32_t)kScanQ ? nq - q0 : (uint32_t)kScanQ);
+  for (int j = 0; j < nqg; ++j) load_query(Q, q0 + j, L, qs + j * L);
+  for (int j = 0; j < kScanQ; ++j) topk_reset(tk[j]);
+  __syncthreads();
+  const int cpr = (int)(L >> 3);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int g = kScanTile / (kScanQ * (cpr + 1));
+  if (g > 64) g = 64;
+  const uint64_t rows_per_iter = (uint64_t)g * (kBlock / 64);
+  float *cs = csum[wave];
+  uint32_t bnd[kScanQ];
+  for (uint64_t base = (uint64_t)blockIdx.x * rows_per_iter; base < n;
+       base += (uint64_t)gridDim.x * rows_per_iter) {
+#pragma unroll
+    for (int j = 0; j < kScanQ; ++j) {
+      uint32_t gb = j < nqg ? __atomic_load_n(&bound[q0 + j], __ATOMIC_RELAXED) : 0u;
+      bnd[j] = gb < tk[j].kth_bits ? gb : tk[j].kth_bits;
+    }
+    uint64_t wbeg = base + (uint64_t)wave * g;
+    int cnt = wbeg < n ? (int)((n - wbeg) < (uint64_t)g ? (n - wbeg) : (uint64_t)g) : 0;
+    const int tasks = cnt * cpr;
+    for (int s = lane; s < tasks; s += 64) {
+      int r = s / cpr;
+      int c = s - r * cpr;
+      const float *xp = X + (wbeg + r) * L + (uint64_t)c * 8;
+      float4 xa = ld_stream(xp);
+      float4 xb = ld_stream(xp + 4);
+#pragma unroll
+      for (int j = 0; j < kScanQ; ++j) {
+        const float4 *qp = reinterpret_cast<const float4 *>(qs + j * L + c * 8);
+        cs[(j * g + r) * (cpr + 1) + c] = ed_chunk(qp[0], qp[1], xa, xb);
+      }
+    }
+    __syncthreads();
+    if (lane < cnt) {
+      uint64_t row = wbeg + lane;
+      uint32_t id = ids ? ids[row] : (uint32_t)row;
+#pragma unroll
+      for (int j = 0; j < kScanQ; ++j) {
+        if (j < nqg) {
+          const float *p = cs + (j * g + lane) * (cpr + 1);
+          float acc = 0.0f;
+          for (int c = 0; c < cpr; ++c) acc = acc + p[c];
+          if (__float_as_uint(acc) <= bnd[j]) topk_stage(tk[j], make_key(acc, id));
+        }
+      }
+    }
+    __syncthreads();
+    for (int j = 0; j < nqg; ++j) topk_merge(tk[j], scan, (int)kq[q0 + j]);
+  }
+  for (int j = 0; j < nqg; ++j) flush_topk(tk[j], (int)kq[q0 + j], q0 + j, cap, cand, cand_cnt, bound);
+}
+
+// final per-query selection over the candidate buffer
+__global__ void __launch_bounds__(kBlock) k_select(uint32_t kmax, const uint32_t *__restrict__ kq,
+                                                   uint32_t cap, const uint64_t *__restrict__ cand,
+                                                   const uint32_t *__restrict__ cand_cnt,
+                                                   const uint64_t *__restrict__ idmap64,
+                                                   uint64_t *__restrict__ out_ids,
+                                                   float *__restrict__ out_d2,
+                                                   uint32_t *__restrict__ overflow)
+{
+  __shared__ TopK tk;
+  __shared__ uint32_t scan[kBlock];
+  const uint32_t q = blockIdx.x;
+  const int k = (int)kq[q];
+  uint32_t cnt = cand_cnt[q];
+  if (cnt > cap) {
+    if (threadIdx.x == 0) atomicAdd(overflow, 1u);
+    cnt = cap;
+  }
+  topk_reset(tk);
+  __syncthreads();
+  for (uint32_t t0 = 0; t0 < cnt; t0 += kBlock) {
+    uint32_t i = t0 + threadIdx.x;
+    if (i < cnt) {
+      uint64_t key = cand[(uint64_t)q * cap + i];
+      if ((uint32_t)(key >> 32) <= tk.kth_bits) topk_stage(tk, key);
+    }
+    __syncthreads();
+    topk_merge(tk, scan, k);
+  }
+  for (uint32_t r = threadIdx.x; r < kmax; r += kBlock) {
+    uint64_t id = ~0ull;
+    float d = FLT_MAX;
+    if ((int)r < tk.cnt) {
+      uint64_t key = tk.keys[r];
+      d = __uint_as_float((uint32_t)(key >> 32));
+      uint32_t id32 = (uint32_t)key;
+      id = idmap64 ? idmap64[id32] : (uint64_t)id32;
+    }
+    out_ids[(uint64_t)q * kmax + r] = id;
+    out_d2[(uint64_t)q * kmax + r] = d;
+  }
+}
+
+// reset per-query state: bound = +inf, no candidates
+__global__ void k_reset_queries(uint32_t nq, uint32_t *__restrict__ bound, uint32_t *__restrict__ cand_cnt)
+{
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nq) {
+    bound[i] = kInfBits;
+    cand_cnt[i] = 0;
+  }
+}
+
+// small fill kernel (used instead of hipMemsetAsync, which must not be
+// recorded into graphs)
+__global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t n, uint32_t v)
+{
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_reset(hipStream_t st, uint32_t nq, uint32_t *bound, uint32_t *cand_cnt)
+{
+  hipLaunchKernelGGL(k_reset_queries, dim3((nq + 255) / 256), dim3(256), 0, st, nq, bound, cand_cnt);
+}
+
+void launch_fill(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v)
+{
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+}
+
+void launch_prep(hipStream_t st, const float *Q, uint32_t nq, uint32_t L, float *paa, double *psum, double *qn2)
+{
+  hipLaunchKernelGGL(k_prep, dim3(nq), dim3(64), 0, st, Q, L, paa, psum, qn2);
+}
+
+void launch_eapca(hipStream_t st, uint32_t nq, uint32_t nleaf, uint32_t L, const uint32_t *seg_off,
+                  const int16_t *seg_end, const float4 *ind, const double *psum, float *lb)
+{
+  size_t lds = 2 * (L + 1) * sizeof(double);
+  if (lds > kEapcaLdsBytes) lds = 0;
+  hipLaunchKernelGGL(k_eapca, dim3((nleaf + kBlock - 1) / kBlock, nq), dim3(kBlock), lds, st, nleaf, L,
+                     seg_off, seg_end, ind, psum, lb);
+}
+
+void launch_best_leaf(hipStream_t st, uint32_t nq, uint32_t nleaf, const float *lb, uint32_t *best)
+{
+  hipLaunchKernelGGL(k_best_leaf, dim3(nq), dim3(kBlock), 0, st, nleaf, lb, best);
+}
+
+void launch_scan_best(hipStream_t st, uint32_t nq, const float *X, uint32_t L, const float *Q,
+                      const uint32_t *kq, const uint32_t *best, const uint64_t *leaf_pos,
+                      const uint32_t *leaf_size, const uint32_t *ids, uint32_t cap, uint64_t *cand,
+                      uint32_t *cand_cnt, uint32_t *bound, unsigned long long *counters)
+{
+  hipLaunchKernelGGL(k_scan_best, dim3(nq), dim3(kBlock), L * sizeof(float), st, X, L, Q, kq, best,
+                     leaf_pos, leaf_size, ids, cap, cand, cand_cnt, bound, counters);
+}
+
+void launch_collect(hipStream_t st, uint32_t nq, uint32_t nleaf, const float *lb, const uint32_t *best,
+                    const uint32_t *bound, const uint32_t *leaf_size, uint64_t *work, uint32_t work_cap,
+                    uint32_t *nwork, unsigned long long *counters)
+{
+  hipLaunchKernelGGL(k_collect, dim3((nleaf + kBlock - 1) / kBlock, nq), dim3(kBlock), 0, st, nq, nleaf,
+                     lb, best, bound, leaf_size, work, work_cap, nwork, counters);
+}
+
+void launch_filter_refine(hipStream_t st, uint32_t grid, const float *X, const uint8_t *sax, uint32_t L,
+                          const float *Q, const float *paa, const uint32_t *kq, const uint64_t *leaf_pos,
+                          const uint32_t *leaf_size, const uint32_t *ids, const uint64_t *work,
+                          const uint32_t *nwork, uint32_t work_cap, uint32_t *work_next, uint32_t cap,
+                          uint64_t *cand, uint32_t *cand_cnt, uint32_t *bound,
+                          unsigned long long *counters)
+{
+  hipLaunchKernelGGL(k_filter_refine, dim3(grid), dim3(kBlock), L * sizeof(float), st, X, sax, L, Q, paa,
+                     kq, leaf_pos, leaf_size, ids, work, nwork, work_cap, work_next, cap, cand, cand_cnt,
+                     bound, counters);
+}
+
+void launch_scan(hipStream_t st, uint32_t grid_x, const float *X, uint64_t n, uint32_t L, const float *Q,
+                 uint32_t nq, const uint32_t *kq, const uint32_t *ids, uint32_t cap, uint64_t *cand,
+                 uint32_t *cand_cnt, uint32_t *bound)
+{
+  uint32_t groups = (nq + kScanQ - 1) / kScanQ;
+  hipLaunchKernelGGL(k_scan, dim3(grid_x, groups), dim3(kBlock), (size_t)kScanQ * L * sizeof(float), st, X,
+                     n, L, Q, nq, kq, ids, cap, cand, cand_cnt, bound);
+}
+
+void launch_select(hipStream_t st, uint32_t nq, uint32_t kmax, const uint32_t *kq, uint32_t cap,
+                   const uint64_t *cand, const uint32_t *cand_cnt, const uint64_t *idmap64, uint64_t *out_ids,
+                   float *out_d2, uint32_t *overflow)
+{
+  hipLaunchKernelGGL(k_select, dim3(nq), dim3(kBlock), 0, st, kmax, kq, cap, cand, cand_cnt, idmap64, out_ids,
+                     out_d2, overflow);
+}
+
+}  // namespace hkg

@@ -1,0 +1,214 @@
+/*
+ * hkg.h -- C ABI of the MI355X exact k-NN query engine for Hercules indexes.
+ *
+ * Plain C types only (no HIP or torch types).  Every entry point returns an
+ * hkg_status; none calls exit().  A handle is safe to use from one thread at
+ * a time; distinct handles are independent.
+ *
+ * Reference interfaces replaced (file:line in karimaechihabi/hercules):
+ *   hkg_open_dir      hercules_index_read + cache_sax_file
+ *                     (code/hercules/src/hercules_index.c:1793-1905, 2342-2351)
+ *   hkg_knn           the exact query functions exact_de_knn_search,
+ *                     exact_de_knn_search_psq_lf and
+ *                     exact_de_knn_search_psq_lf_sims_adaptive
+ *                     (code/hercules/include/hercules_query_engine.h:203-207,
+ *                     256-261, 284-289), batched over queries and returning
+ *                     the results instead of printing them
+ *   hkg_open_raw +
+ *   hkg_knn(HKG_PATH_SCAN)  PSCAN's parallel_scan / parallel_scan_process_query
+ *                     (code/pscan/src/parallel_scan_utils.c:23-347)
+ *   hkg_build         hercules_index_build + hercules_index_write
+ *                     (code/hercules/src/hercules_index.c:2354-2929,
+ *                     1694-1791): a GPU bulk loader producing the same
+ *                     on-disk format (root.idx, leaves_raw.idx,
+ *                     leaves_sims.idx)
+ *   hkg_server_*      no single counterpart: the query file loop
+ *                     hercules_knn_query_binary_file
+ *                     (code/hercules/src/hercules_file_loaders.c:99-294) as a
+ *                     continuous-batching request loop
+ *
+ * Result semantics follow queue_bounded_sorted_insert
+ * (code/hercules/src/hercules_query_engine.c:4034-4082) and its strict-<
+ * call sites: per query the k smallest DISTINCT FP32 squared distances in
+ * ascending order; among series with equal distance the smallest id is
+ * returned.  Ranks that cannot be filled hold d2 = FLT_MAX and
+ * id = UINT64_MAX.  Squared distances are bit-identical to the reference's
+ * ts_euclidean_distance_SIMD (code/hercules/src/ts.c:83-109).
+ */
+#ifndef HKG_H
+#define HKG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hkg_status {
+  HKG_OK = 0,
+  HKG_EINVAL = 1,   /* bad argument */
+  HKG_ENOMEM = 2,   /* host or device allocation failed */
+  HKG_EIO = 3,      /* file missing, unreadable or short */
+  HKG_EFORMAT = 4,  /* index file malformed or truncated */
+  HKG_EDEVICE = 5,  /* HIP runtime error */
+  HKG_EBUSY = 6,    /* serving loop full (admission refused) */
+  HKG_ENOTREADY = 7 /* serving ticket not finished yet */
+} hkg_status;
+
+typedef struct hkg_index hkg_index;
+typedef struct hkg_server hkg_server;
+
+typedef enum hkg_path {
+  HKG_PATH_AUTO = 0,    /* batched index path (BF16 filter where it pays)   */
+  HKG_PATH_INDEX = 1,   /* plain FP32 index path, one query per launch set  */
+  HKG_PATH_BATCHED = 2, /* batched FP32 index path: persistent (query, leaf)
+                           work queue, stages captured in a hipGraph        */
+  HKG_PATH_SCAN = 3,    /* PSCAN: exact full scan, no lower bounds          */
+  HKG_PATH_BF16 = 4     /* batched index path with the BF16 MFMA certified
+                           lower bound in place of the iSAX filter          */
+} hkg_path;
+
+typedef struct hkg_options {
+  int device;          /* HIP device ordinal                                  */
+  int keep_bf16;       /* also keep a BF16 copy of the series (for HKG_PATH_BF16) */
+  int verbose;         /* print load statistics to stderr                     */
+} hkg_options;
+
+typedef struct hkg_query_opts {
+  hkg_path path;
+  uint32_t batch;      /* queries per batch (0 = library default)             */
+  uint32_t reserved[6];
+} hkg_query_opts;
+
+typedef struct hkg_query_stats {
+  uint64_t queries;
+  uint64_t leaves_total;          /* per query, summed over queries            */
+  uint64_t leaves_after_eapca;    /* candidate leaves (1st-level survivors)    */
+  uint64_t series_total;
+  uint64_t series_lb_checked;     /* series whose lower bound was evaluated    */
+  uint64_t series_refined;        /* series whose exact distance was computed  */
+  uint64_t bytes_algorithmic;     /* HBM bytes the plan had to touch           */
+  double device_ms;               /* device time of the query kernels          */
+  uint32_t overflow_retries;      /* candidate-buffer overflows re-run         */
+  uint32_t reserved[7];
+} hkg_query_stats;
+
+typedef struct hkg_index_info {
+  uint64_t n_series;
+  uint32_t length;
+  uint32_t paa_segments;
+  uint32_t sax_bit_cardinality;
+  uint32_t max_leaf_size;
+  uint64_t n_leaves;
+  uint64_t n_nodes;
+  uint64_t device_bytes;          /* HBM held by this index                    */
+  int has_tree;                   /* 0 for hkg_open_raw (scan only)            */
+  int has_ids;                    /* an original-id map is attached            */
+} hkg_index_info;
+
+/* ---- lifetime ----------------------------------------------------------- */
+void hkg_default_options(hkg_options *opt);
+void hkg_default_query_opts(hkg_query_opts *q);
+
+/* Load a Hercules index directory (root.idx, leaves_raw.idx,
+   leaves_sims.idx, as written with --serial 1 --sims) into HBM as SoA. */
+hkg_status hkg_open_dir(const char *index_dir, const hkg_options *opt, hkg_index **out);
+
+/* A flat dataset for the full scan only (no tree); X is host memory,
+   row-major n x L float32, L % 8 == 0. */
+hkg_status hkg_open_raw(const float *X, uint64_t n, uint32_t L, const hkg_options *opt,
+                        hkg_index **out);
+
+/* Attach an id map (leaf-order row -> public id, n entries, each < 2^32);
+   without one the public id of a series is its leaf-order position
+   (file_pos + offset in leaf, the reference's "Pos"). */
+hkg_status hkg_set_ids(hkg_index *ix, const uint64_t *ids, uint64_t n);
+
+hkg_status hkg_info(const hkg_index *ix, hkg_index_info *info);
+/* Parse and validate an index directory on the host only (no device):
+   root.idx structure and the sizes of leaves_raw.idx / leaves_sims.idx. */
+hkg_status hkg_inspect_dir(const char *index_dir, hkg_index_info *info);
+void hkg_close(hkg_index *ix);
+
+/* ---- queries ------------------------------------------------------------ */
+/* Q: nq x L float32 host rows.  k: per-query k (1..256), or NULL to use
+   kmax for all.  Outputs ids/d2: nq x kmax, host memory. */
+hkg_status hkg_knn(hkg_index *ix, const float *Q, uint32_t nq, const uint32_t *k, uint32_t kmax,
+                   const hkg_query_opts *qo, uint64_t *ids, float *d2, hkg_query_stats *stats);
+
+/* Same, with Q, k, ids (uint64) and d2 already in device memory; runs on
+   the given HIP stream (cast to void*; NULL = the handle's stream) and
+   synchronises it before returning. */
+hkg_status hkg_knn_device(hkg_index *ix, const float *dQ, uint32_t nq, const uint32_t *dk,
+                          uint32_t kmax, const hkg_query_opts *qo, uint64_t *dids, float *dd2,
+                          void *stream, hkg_query_stats *stats);
+
+/* ---- index build (GPU bulk loader, same on-disk format) ---------------- */
+typedef struct hkg_build_opts {
+  uint32_t leaf_size;      /* maximum series per leaf                       */
+  uint32_t init_segments;  /* initial EAPCA segments at the root (1)        */
+  uint32_t paa_segments;   /* iSAX segments (16)                            */
+  int device;
+} hkg_build_opts;
+
+void hkg_default_build_opts(hkg_build_opts *bo);
+/* Build from host rows (n x L float32, original order). */
+hkg_status hkg_build(const float *X, uint64_t n, uint32_t L, const hkg_build_opts *bo,
+                     hkg_index **out);
+/* Build from rows already in device memory (n x L float32, original order);
+   the library takes ownership of dX (frees it with hipFree). */
+hkg_status hkg_build_device(float *dX, uint64_t n, uint32_t L, const hkg_build_opts *bo,
+                            hkg_index **out);
+/* Write a resident index in the reference's on-disk format (directory must
+   exist).  The id map, if any, is written to ids.u64 beside it. */
+hkg_status hkg_write_dir(const hkg_index *ix, const char *index_dir);
+
+/* ---- workload generator (device) ---------------------------------------- */
+/* kind 0: z-normalised random walks; kind 1: N(0,1) -> L2 -> z-normalised.
+   Writes n x L float32 rows for ids first..first+n-1 into device memory. */
+hkg_status hkg_gen_series_device(int kind, uint64_t seed, uint64_t first, uint64_t n, uint32_t L,
+                                 float *dX, void *stream);
+/* Noisy queries q_i = znorm(X[src_i] + sigma_i N(0,1)), src_i drawn from
+   stream i of query_seed; dQ: nq x L device, sigmas/src host. */
+hkg_status hkg_gen_noisy_queries_device(int kind, uint64_t data_seed, uint64_t N,
+                                        uint64_t query_seed, uint32_t nq, uint32_t L,
+                                        const float *sigmas, float *dQ, uint64_t *src_ids);
+
+/* ---- serving loop (continuous batching) --------------------------------- */
+typedef struct hkg_server_opts {
+  uint32_t max_batch;        /* queries admitted per device batch            */
+  uint64_t scratch_bytes;    /* device scratch budget for admitted queries   */
+  uint32_t queue_capacity;   /* pending requests before HKG_EBUSY            */
+  uint32_t reserved[5];
+} hkg_server_opts;
+
+void hkg_default_server_opts(hkg_server_opts *so);
+hkg_status hkg_server_start(hkg_index *ix, const hkg_server_opts *so, hkg_server **out);
+/* Submit one query (host row of L floats); returns a ticket. */
+hkg_status hkg_submit(hkg_server *s, const float *q, uint32_t k, uint64_t *ticket);
+/* Non-blocking: HKG_ENOTREADY until done; then copies k results out and
+   reports submit->complete latency in microseconds. */
+hkg_status hkg_poll(hkg_server *s, uint64_t ticket, uint64_t *ids, float *d2, double *latency_us);
+/* Blocking variant of hkg_poll. */
+hkg_status hkg_wait(hkg_server *s, uint64_t ticket, uint64_t *ids, float *d2, double *latency_us);
+void hkg_server_stop(hkg_server *s);
+
+/* ---- device memory helpers (for callers without their own HIP binding) -- */
+void *hkg_device_alloc(uint64_t bytes);               /* NULL on failure        */
+void hkg_device_free(void *p);
+/* kind: 1 host->device, 2 device->host, 3 device->device (synchronous) */
+hkg_status hkg_memcpy(void *dst, const void *src, uint64_t bytes, int kind);
+hkg_status hkg_device_sync(void);
+
+/* ---- misc --------------------------------------------------------------- */
+const char *hkg_status_str(hkg_status s);
+/* Last error message of the calling thread (static storage). */
+const char *hkg_last_error(void);
+/* The card-256 iSAX breakpoints used by the device kernels (255 floats). */
+void hkg_breakpoints256(float *out255);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HKG_H */

@@ -1,0 +1,64 @@
+"""CPU: the C-ABI library loads and exports every entry point that
+include/hkg.h declares; host-only entry points behave (no device calls)."""
+from __future__ import annotations
+
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "hkg.h")
+
+
+def declared() -> list[str]:
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(hkg_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    for must in ["hkg_open_dir", "hkg_knn", "hkg_knn_device", "hkg_close", "hkg_build", "hkg_write_dir",
+                 "hkg_server_start", "hkg_submit", "hkg_poll", "hkg_server_stop", "hkg_inspect_dir"]:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from hercules_amd import hkg
+    L = hkg.lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", hkg.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r" T (hkg_\w+)$", out, flags=re.M))
+    missing = [n for n in declared() if n not in exported]
+    assert not missing, missing
+    # the Python binding covers the whole ABI, with no extra names
+    assert set(hkg.SIGNATURES) == set(declared())
+    for n in declared():
+        assert getattr(L, n) is not None
+
+
+def test_no_torch_or_hip_types_in_header():
+    code = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    assert not re.search(r"\bhip\w*", code) and "torch" not in code
+    assert "#include <hip" not in code
+
+
+def test_status_strings():
+    from hercules_amd import hkg
+    for s in range(8):
+        assert hkg.lib().hkg_status_str(s).decode() == hkg.STATUS[s]
+
+
+def test_inspect_dir_errors(tmp_path):
+    from hercules_amd import hkg
+    with pytest.raises(hkg.HkgError) as e:
+        hkg.inspect_dir(str(tmp_path))
+    assert e.value.status == 3  # missing root.idx -> i/o error
+    # truncated root.idx -> malformed
+    (tmp_path / "root.idx").write_bytes(b"\x01\x00\x00")
+    with pytest.raises(hkg.HkgError) as e:
+        hkg.inspect_dir(str(tmp_path))
+    assert e.value.status == 4

@@ -1,0 +1,156 @@
+"""GPU parity: the HIP paths against the oracle (bit-exact d2, equal ids)
+and against the reference CLI's own answers.  Run on an MI355X."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+import ref_runner as rr
+
+pytestmark = pytest.mark.gpu
+
+L256 = 256
+
+
+def _say(msg: str) -> None:
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def hkg():
+    from hercules_amd import hkg as m
+    m.lib()
+    return m
+
+
+def row_map(index_dir: str, X: np.ndarray) -> np.ndarray:
+    """Original ordinal of every leaf-order row of leaves_raw.idx, by exact
+    row match (the index stores no ids)."""
+    L = X.shape[1]
+    raw = np.fromfile(os.path.join(index_dir, "leaves_raw.idx"), np.float32).reshape(-1, L)
+    lookup = {X[i].tobytes(): i for i in range(X.shape[0])}
+    return np.array([lookup[r.tobytes()] for r in raw], np.uint64)
+
+
+def assert_same(res_ids, res_d2, ids, d2, what: str) -> None:
+    assert res_d2.dtype == np.float32
+    bad = np.nonzero((res_d2.view(np.uint32) != d2.view(np.uint32)).any(1))[0]
+    assert bad.size == 0, f"{what}: d2 differs for queries {bad[:10]}: {res_d2[bad[0]][:5]} vs {d2[bad[0]][:5]}"
+    bad = np.nonzero((res_ids != ids).any(1))[0]
+    assert bad.size == 0, f"{what}: ids differ for queries {bad[:10]}: {res_ids[bad[0]][:5]} vs {ids[bad[0]][:5]}"
+
+
+# ---- generator ---------------------------------------------------------------
+@pytest.mark.parametrize("kind,L", [(0, 256), (1, 96), (0, 96)])
+def test_generator_bit_exact(hkg, oracle, kind, L):
+    _say(f"generator kind={kind} L={L}")
+    for first, n in [(0, 3000), (99_990_000, 500)]:
+        got = hkg.gen_series(kind, 7, first, n, L)
+        exp = oracle.series(kind, 7, first, n, L)
+        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), (kind, L, first)
+
+
+def test_noisy_queries_bit_exact(hkg, oracle):
+    _say("noisy queries")
+    sig = np.linspace(0.0, 0.2, 37).astype(np.float32)
+    got, src = hkg.gen_noisy_queries(0, 1, 1_000_000, 3, sig, 256)
+    exp, esrc = oracle.noisy_queries(0, 1, 1_000_000, 3, sig, 256)
+    assert np.array_equal(src, esrc)
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+
+
+# ---- full scan (PSCAN) ---------------------------------------------------------
+@pytest.fixture(scope="module")
+def small_data(oracle):
+    X = oracle.series(0, 11, 0, 20000, L256)
+    Q = oracle.series(0, 12, 0, 40, L256)
+    # a few hard/easy queries: noisy copies of dataset rows
+    Qn, _ = oracle.noisy_queries(0, 11, 20000, 13, np.array([0.0, 0.01, 0.1, 0.2] * 5, np.float32), L256)
+    return X, np.concatenate([Q, Qn])
+
+
+@pytest.mark.parametrize("k", [1, 7, 100])
+def test_scan_matches_oracle(hkg, oracle, small_data, k):
+    X, Q = small_data
+    _say(f"scan k={k}")
+    ids, d2 = oracle.knn(X, Q, k)
+    with hkg.HerculesIndex.open_raw(X) as ix:
+        r = ix.knn(Q, k, path="scan")
+    assert_same(r.ids, r.d2, ids, d2, f"scan k={k}")
+
+
+def test_scan_ties_and_padding(hkg, oracle):
+    _say("scan ties")
+    X = oracle.series(0, 21, 0, 3000, L256)
+    X[2500] = X[17]          # exact duplicate: same d2 for two ids
+    X[1200] = X[17]
+    Q = np.ascontiguousarray(X[[17, 5, 2500]] + np.float32(0.001))
+    ids, d2 = oracle.knn(X, Q, 5)
+    assert ids[0, 0] == 17   # smallest id among equal distances
+    with hkg.HerculesIndex.open_raw(X) as ix:
+        r = ix.knn(Q, 5, path="scan")
+        assert_same(r.ids, r.d2, ids, d2, "ties")
+        # k larger than the number of distinct distances: padded ranks
+        small = np.ascontiguousarray(X[:3])
+    with hkg.HerculesIndex.open_raw(np.repeat(small, 2, axis=0)) as ix2:
+        r2 = ix2.knn(small[:1], 8, path="scan")
+    assert (r2.d2[0, 3:] == np.finfo(np.float32).max).all()
+    assert (r2.ids[0, 3:] == np.uint64(2**64 - 1)).all()
+    assert r2.ids[0, 0] == 0 and set(r2.ids[0, :3].tolist()) == {0, 2, 4}
+
+
+# ---- index paths on a reference-built index ------------------------------------
+@pytest.fixture(scope="module")
+def ref_index(tmp_path_factory, small_data):
+    if not rr.available():
+        pytest.skip("reference binaries (oracle/_ref) not built")
+    X, Q = small_data
+    d = tmp_path_factory.mktemp("tiny")
+    rr.write_f32(str(d / "data.bin"), X)
+    rr.write_f32(str(d / "q.bin"), Q)
+    rr.build_index(str(d / "data.bin"), X.shape[0], L256, 500, str(d / "idx"), threads=8)
+    return str(d / "idx"), str(d / "q.bin")
+
+
+@pytest.mark.parametrize("path", ["index", "batched", "scan"])
+@pytest.mark.parametrize("k", [1, 10])
+def test_index_paths_match_oracle(hkg, oracle, small_data, ref_index, path, k):
+    X, Q = small_data
+    idx, _ = ref_index
+    _say(f"index path={path} k={k}")
+    ids, d2 = oracle.knn(X, Q, k)
+    with hkg.HerculesIndex.open(idx) as ix:
+        ix.set_ids(row_map(idx, X))
+        r = ix.knn(Q, k, path=path)
+    assert_same(r.ids, r.d2, ids, d2, f"{path} k={k}")
+
+
+def test_index_matches_reference_cli(hkg, small_data, ref_index):
+    """The reference's own multithreaded SIMS answers (exact d2 printed in
+    hex by the oracle-side interposer) equal the GPU's."""
+    X, Q = small_data
+    idx, qpath = ref_index
+    _say("reference cli")
+    d2_ref, _, _, _ = rr.query(idx, qpath, Q.shape[0], L256, 10, threads=8, sims=True)
+    with hkg.HerculesIndex.open(idx) as ix:
+        r = ix.knn(Q, 10, path="batched")
+    assert np.array_equal(r.d2.view(np.uint32), d2_ref.view(np.uint32))
+
+
+def test_mixed_k_and_determinism(hkg, oracle, small_data, ref_index):
+    X, Q = small_data
+    idx, _ = ref_index
+    _say("mixed k")
+    kv = np.array([1, 3, 10, 100] * (Q.shape[0] // 4 + 1), np.uint32)[: Q.shape[0]]
+    ids, d2 = oracle.knn(X, Q, 100)
+    with hkg.HerculesIndex.open(idx) as ix:
+        ix.set_ids(row_map(idx, X))
+        r1 = ix.knn(Q, kv, path="batched")
+        r2 = ix.knn(Q, kv, path="batched")
+    assert np.array_equal(r1.ids, r2.ids) and np.array_equal(r1.d2, r2.d2)
+    for i, k in enumerate(kv):
+        assert np.array_equal(r1.d2[i, :k].view(np.uint32), d2[i, :k].view(np.uint32)), i
+        assert np.array_equal(r1.ids[i, :k], ids[i, :k]), i
+        assert (r1.d2[i, k:] == np.finfo(np.float32).max).all()
