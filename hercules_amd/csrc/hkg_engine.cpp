@@ -375,24 +375,45 @@ static uint32_t persistent_grid()
   return g;
 }
 
-void Index::enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st)
+void Index::enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf)
 {
   const uint32_t L = dev.L, nleaf = dev.nleaf;
   uint32_t *nwork = b.misc + 0, *work_next = b.misc + 1, *overflow = b.misc + 2;
   launch_reset(st, nq, b.bound, b.cand_cnt);
   launch_fill(st, b.misc, 8, 0);
   launch_fill(st, reinterpret_cast<uint32_t *>(b.counters), 2 * kNumCounters, 0);
+  if (pf) pf->mark(st, -1);
   launch_prep(st, b.Q, nq, L, b.paa, b.psum, b.qn2);
+  if (pf) pf->mark(st, HKG_STAGE_PREP);
   launch_eapca(st, nq, nleaf, L, dev.leaf_seg_off, dev.seg_end, dev.seg_ind, b.psum, b.lb);
+  if (pf) pf->mark(st, HKG_STAGE_EAPCA);
   launch_best_leaf(st, nq, nleaf, b.lb, b.best);
+  if (pf) pf->mark(st, HKG_STAGE_SELECT_LEAF);
   launch_scan_best(st, nq, dev.X, L, b.Q, b.kq, b.best, dev.leaf_pos, dev.leaf_size, dev.ids, b.cap, b.cand,
                    b.cand_cnt, b.bound, b.counters);
+  if (pf) pf->mark(st, HKG_STAGE_APPROX);
   launch_collect(st, nq, nleaf, b.lb, b.best, b.bound, dev.leaf_size, b.work, b.work_cap, nwork, b.counters);
+  if (pf) pf->mark(st, HKG_STAGE_COLLECT);
   launch_filter_refine(st, persistent_grid(), dev.X, dev.sax, L, b.Q, b.paa, b.kq, dev.leaf_pos, dev.leaf_size,
                        dev.ids, b.work, nwork, b.work_cap, work_next, b.cap, b.cand, b.cand_cnt, b.bound,
                        b.counters);
+  if (pf) pf->mark(st, HKG_STAGE_REFINE);
   launch_select(st, nq, b.kmax, b.kq, b.cap, b.cand, b.cand_cnt, idmap64, b.out_ids, b.out_d2, overflow);
+  if (pf) pf->mark(st, HKG_STAGE_TOPK);
 }
+
+// one pass over X per group of kScanQ queries
+uint64_t Index::total_scan_bytes(uint32_t nq, uint32_t batch) const
+{
+  uint64_t passes = 0;
+  for (uint32_t q0 = 0; q0 < nq; q0 += batch) {
+    uint32_t n = std::min(batch, nq - q0);
+    passes += (n + kScanQ - 1) / kScanQ;
+  }
+  return passes * dev.n * dev.L * 4;
+}
+
+uint64_t Index::total_segs() const { return dev.tree.seg_end.size(); }
 
 uint32_t scan_grid_x(uint32_t nq)
 {
@@ -402,15 +423,18 @@ uint32_t scan_grid_x(uint32_t nq)
   return std::max<uint32_t>(gx, 64);
 }
 
-void Index::enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st)
+void Index::enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf)
 {
   uint32_t *overflow = b.misc + 2;
   launch_reset(st, nq, b.bound, b.cand_cnt);
   launch_fill(st, b.misc, 8, 0);
   launch_fill(st, reinterpret_cast<uint32_t *>(b.counters), 2 * kNumCounters, 0);
+  if (pf) pf->mark(st, -1);
   launch_scan(st, scan_grid_x(nq), dev.X, dev.n, dev.L, b.Q, nq, b.kq, dev.ids, b.cap, b.cand, b.cand_cnt,
               b.bound);
+  if (pf) pf->mark(st, HKG_STAGE_SCAN);
   launch_select(st, nq, b.kmax, b.kq, b.cap, b.cand, b.cand_cnt, idmap64, b.out_ids, b.out_d2, overflow);
+  if (pf) pf->mark(st, HKG_STAGE_TOPK);
 }
 
 // capacity of the per-query candidate buffer for a path
@@ -421,7 +445,7 @@ uint32_t Index::cand_cap(hkg_path path, uint32_t nq, uint32_t kmax) const
 }
 
 hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax,
-                            bool use_graph, hipStream_t st, QueryBuffers **used, Counters *ctr)
+                            bool use_graph, hipStream_t st, QueryBuffers **used, Counters *ctr, Prof *pf)
 {
   const bool scan = (path == HKG_PATH_SCAN) || !dev.has_tree;
   const uint32_t cap = cand_cap(scan ? HKG_PATH_SCAN : path, nq, kmax);
@@ -430,13 +454,13 @@ hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, 
   if (rc != HKG_OK) return rc;
   HKG_HIP_TRY(hipMemcpyAsync(b->Q, dQ, (size_t)nq * dev.L * 4, hipMemcpyDeviceToDevice, st));
   HKG_HIP_TRY(hipMemcpyAsync(b->kq, dk, (size_t)nq * 4, hipMemcpyDeviceToDevice, st));
-  if (use_graph && !scan) {
+  if (use_graph && !scan && !pf) {
     if (!b->graph_exec || b->graph_stream != st) {
       if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
       b->graph_exec = nullptr;
       hipGraph_t g;
       HKG_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      enqueue_index_batch(*b, nq, st);
+      enqueue_index_batch(*b, nq, st, nullptr);
       HKG_HIP_TRY(hipStreamEndCapture(st, &g));
       hipError_t e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
@@ -445,9 +469,9 @@ hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, 
     }
     HKG_HIP_TRY(hipGraphLaunch(b->graph_exec, st));
   } else if (scan) {
-    enqueue_scan_batch(*b, nq, st);
+    enqueue_scan_batch(*b, nq, st, pf);
   } else {
-    enqueue_index_batch(*b, nq, st);
+    enqueue_index_batch(*b, nq, st, pf);
   }
   HKG_HIP_TRY(hipGetLastError());
   if (ctr) {
@@ -471,7 +495,8 @@ hkg_status Index::fix_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax, hipS
     ++*retries;
     QueryBuffers *sb = nullptr;
     Counters c;
-    hkg_status rc = run_batch(HKG_PATH_SCAN, b.Q + (uint64_t)q * dev.L, b.kq + q, 1, kmax, false, st, &sb, &c);
+    hkg_status rc =
+        run_batch(HKG_PATH_SCAN, b.Q + (uint64_t)q * dev.L, b.kq + q, 1, kmax, false, st, &sb, &c, nullptr);
     if (rc != HKG_OK) return rc;
     HKG_HIP_TRY(hipMemcpyAsync(b.out_ids + (uint64_t)q * kmax, sb->out_ids, kmax * 8, hipMemcpyDeviceToDevice, st));
     HKG_HIP_TRY(hipMemcpyAsync(b.out_d2 + (uint64_t)q * kmax, sb->out_d2, kmax * 4, hipMemcpyDeviceToDevice, st));
@@ -510,19 +535,32 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
   HKG_HIP_TRY(hipEventCreate(&e1));
   HKG_HIP_TRY(hipEventRecord(e0, st));
   Counters total;
-  uint32_t retries = 0;
+  uint32_t retries = 0, launches = 0;
+  std::unique_ptr<Prof> pf;
+  double stage_ms[HKG_NUM_STAGES] = {0};
+  uint64_t stage_launches[HKG_NUM_STAGES] = {0};
+  if (o.profile) {
+    pf.reset(new Prof());
+    if (!pf->init()) {
+      set_error("hipEventCreate failed");
+      return HKG_EDEVICE;
+    }
+  }
   for (uint32_t q0 = 0; q0 < nq; q0 += batch) {
     uint32_t n = std::min(batch, nq - q0);
     QueryBuffers *b = nullptr;
     Counters c;
     hkg_status rc;
+    if (pf) pf->n = 0;
     if (path == HKG_PATH_BF16)
-      rc = run_bf16_batch(dQ + (uint64_t)q0 * dev.L, dk + q0, n, kmax, st, &b, &c);
+      rc = run_bf16_batch(dQ + (uint64_t)q0 * dev.L, dk + q0, n, kmax, st, &b, &c, pf.get());
     else
       rc = run_batch(path, dQ + (uint64_t)q0 * dev.L, dk + q0, n, kmax, path == HKG_PATH_BATCHED && n == batch, st,
-                     &b, &c);
+                     &b, &c, pf.get());
     if (rc != HKG_OK) return rc;
     HKG_HIP_TRY(hipStreamSynchronize(st));
+    ++launches;
+    if (pf) pf->accumulate(stage_ms, stage_launches);
     if (c.overflow) {
       rc = fix_overflow(*b, n, kmax, st, &retries);
       if (rc != HKG_OK) return rc;
@@ -549,11 +587,21 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     stats->series_total = (uint64_t)nq * dev.n;
     stats->series_lb_checked = scan ? 0 : total.v[kCtrSaxChecked] + total.v[kCtrBf16];
     stats->series_refined = scan ? (uint64_t)nq * dev.n : total.v[kCtrRefined];
-    stats->bytes_algorithmic = scan ? (uint64_t)((nq + kScanQ - 1) / kScanQ) * dev.n * dev.L * 4
-                                    : total.v[kCtrSaxChecked] * 16 + total.v[kCtrBf16] * dev.L * 2 +
-                                          total.v[kCtrRefined] * dev.L * 4;
     stats->device_ms = ms;
     stats->overflow_retries = retries;
+    stats->launches = launches;
+    for (int i = 0; i < HKG_NUM_STAGES; ++i) {
+      stats->stage_ms[i] = stage_ms[i];
+      stats->stage_launches[i] = stage_launches[i];
+    }
+    const uint64_t L4 = (uint64_t)dev.L * 4;
+    stats->stage_bytes[HKG_STAGE_APPROX] = total.v[kCtrApprox] * L4;
+    stats->stage_bytes[HKG_STAGE_REFINE] =
+        total.v[kCtrSaxChecked] * 16 + total.v[kCtrBf16] * dev.L * 2 + total.v[kCtrRefined] * L4;
+    stats->stage_bytes[HKG_STAGE_SCAN] = scan ? (uint64_t)launches * 0 + total_scan_bytes(nq, batch) : 0;
+    stats->stage_bytes[HKG_STAGE_EAPCA] = scan ? 0 : (uint64_t)nq * dev.nleaf * 8 + total_segs() * 18ull * nq;
+    stats->bytes_algorithmic = 0;
+    for (int i = 0; i < HKG_NUM_STAGES; ++i) stats->bytes_algorithmic += stats->stage_bytes[i];
   }
   return HKG_OK;
 }

@@ -17,6 +17,43 @@ struct Counters {
   uint32_t nwork = 0;
 };
 
+// per-stage event timing of one batch (profile mode)
+struct Prof {
+  hipEvent_t ev[12];
+  int stage[12];
+  int n = 0;
+  bool ok = false;
+  bool init()
+  {
+    for (int i = 0; i < 12; ++i)
+      if (hipEventCreate(&ev[i]) != hipSuccess) return false;
+    ok = true;
+    return true;
+  }
+  void mark(hipStream_t st, int s)
+  {
+    if (n < 12) {
+      (void)hipEventRecord(ev[n], st);
+      stage[n++] = s;
+    }
+  }
+  void accumulate(double *ms, uint64_t *launches) const
+  {
+    for (int i = 1; i < n; ++i) {
+      float t = 0.0f;
+      if (hipEventElapsedTime(&t, ev[i - 1], ev[i]) == hipSuccess && stage[i] >= 0) {
+        ms[stage[i]] += t;
+        launches[stage[i]] += 1;
+      }
+    }
+  }
+  ~Prof()
+  {
+    if (ok)
+      for (int i = 0; i < 12; ++i) (void)hipEventDestroy(ev[i]);
+  }
+};
+
 // Device scratch of one batch shape.  All per-query arrays are B long.
 struct QueryBuffers {
   uint32_t B = 0, kmax = 0, cap = 0, work_cap = 0;
@@ -61,12 +98,14 @@ struct Index {
   hkg_status make_bf16();
   hkg_status get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuffers **out);
   uint32_t cand_cap(hkg_path path, uint32_t nq, uint32_t kmax) const;
-  void enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st);
-  void enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st);
+  void enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf);
+  void enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf);
   hkg_status run_batch(hkg_path path, const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax,
-                       bool use_graph, hipStream_t st, QueryBuffers **used, Counters *ctr);
+                       bool use_graph, hipStream_t st, QueryBuffers **used, Counters *ctr, Prof *pf);
   hkg_status run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax, hipStream_t st,
-                            QueryBuffers **used, Counters *ctr);
+                            QueryBuffers **used, Counters *ctr, Prof *pf);
+  uint64_t total_scan_bytes(uint32_t nq, uint32_t batch) const;
+  uint64_t total_segs() const;
   hkg_status fix_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax, hipStream_t st, uint32_t *retries);
   hkg_status knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, uint32_t kmax, const hkg_query_opts *qo,
                         uint64_t *dids, float *dd2, hipStream_t st, hkg_query_stats *stats);
@@ -77,5 +116,7 @@ struct Index {
 hkg_status open_dir(const char *dir, const hkg_options *opt, Index **out);
 hkg_status open_raw(const float *X, uint64_t n, uint32_t L, const hkg_options *opt, Index **out);
 uint32_t scan_grid_x(uint32_t nq);
+hkg_status build_device(float *dX, uint64_t n, uint32_t L, const hkg_build_opts &bo, Index **out);
+hkg_status write_dir(const Index &ix, const char *dir);
 
 }  // namespace hkg

@@ -16,6 +16,7 @@ enum Counter : int {
   kCtrSaxChecked = 1,  // series whose iSAX bound was evaluated
   kCtrRefined = 2,     // series whose exact distance was computed
   kCtrBf16 = 3,        // series whose BF16 bound was evaluated
+  kCtrApprox = 4,      // series of the best leaves scanned in the approximate stage
   kNumCounters = 8
 };
 

@@ -410,7 +410,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_best(const float *__restrict__ 
     topk_merge(tk, scan, k);
   }
   flush_topk(tk, k, q, cap, cand, cand_cnt, bound);
-  if (threadIdx.x == 0) atomicAdd(&counters[kCtrRefined], (unsigned long long)size);
+  if (threadIdx.x == 0) atomicAdd(&counters[kCtrApprox], (unsigned long long)size);
 }
 
 // work list of (query, leaf) pairs that survive the EAPCA bound

@@ -10,7 +10,7 @@ hkg_status Index::make_bf16()
   return HKG_EINVAL;
 }
 hkg_status Index::run_bf16_batch(const float *, const uint32_t *, uint32_t, uint32_t, hipStream_t, QueryBuffers **,
-                                 Counters *)
+                                 Counters *, Prof *)
 {
   set_error("BF16 filter not available in this build");
   return HKG_EINVAL;
@@ -18,29 +18,6 @@ hkg_status Index::run_bf16_batch(const float *, const uint32_t *, uint32_t, uint
 }  // namespace hkg
 
 extern "C" {
-void hkg_default_build_opts(hkg_build_opts *bo)
-{
-  if (!bo) return;
-  bo->leaf_size = 10000;
-  bo->init_segments = 1;
-  bo->paa_segments = 16;
-  bo->device = 0;
-}
-hkg_status hkg_build(const float *, uint64_t, uint32_t, const hkg_build_opts *, hkg_index **)
-{
-  hkg::set_error("index builder not available in this build");
-  return HKG_EINVAL;
-}
-hkg_status hkg_build_device(float *, uint64_t, uint32_t, const hkg_build_opts *, hkg_index **)
-{
-  hkg::set_error("index builder not available in this build");
-  return HKG_EINVAL;
-}
-hkg_status hkg_write_dir(const hkg_index *, const char *)
-{
-  hkg::set_error("index writer not available in this build");
-  return HKG_EINVAL;
-}
 void hkg_default_server_opts(hkg_server_opts *so)
 {
   if (so) memset(so, 0, sizeof(*so));

@@ -78,8 +78,22 @@ typedef struct hkg_options {
 typedef struct hkg_query_opts {
   hkg_path path;
   uint32_t batch;      /* queries per batch (0 = library default)             */
-  uint32_t reserved[6];
+  uint32_t profile;    /* 1: time every stage with events (no graph replay)   */
+  uint32_t reserved[5];
 } hkg_query_opts;
+
+/* stages reported in hkg_query_stats.stage_ms / stage_bytes */
+enum {
+  HKG_STAGE_PREP = 0,     /* PAA and prefix sums of the queries              */
+  HKG_STAGE_EAPCA = 1,    /* EAPCA bound of every (query, leaf)              */
+  HKG_STAGE_SELECT_LEAF = 2, /* best leaf per query                          */
+  HKG_STAGE_APPROX = 3,   /* exact scan of the best leaf (first bound)       */
+  HKG_STAGE_COLLECT = 4,  /* (query, leaf) work list                         */
+  HKG_STAGE_REFINE = 5,   /* iSAX or BF16 filter + exact refinement          */
+  HKG_STAGE_TOPK = 6,     /* final per-query top-k                           */
+  HKG_STAGE_SCAN = 7,     /* PSCAN full scan                                 */
+  HKG_NUM_STAGES = 8
+};
 
 typedef struct hkg_query_stats {
   uint64_t queries;
@@ -91,7 +105,10 @@ typedef struct hkg_query_stats {
   uint64_t bytes_algorithmic;     /* HBM bytes the plan had to touch           */
   double device_ms;               /* device time of the query kernels          */
   uint32_t overflow_retries;      /* candidate-buffer overflows re-run         */
-  uint32_t reserved[7];
+  uint32_t launches;              /* kernel launch sets (batches)              */
+  double stage_ms[8];             /* with profile=1: summed stage times        */
+  uint64_t stage_bytes[8];        /* algorithmic HBM bytes of each stage       */
+  uint64_t stage_launches[8];     /* kernel launches of each stage             */
 } hkg_query_stats;
 
 typedef struct hkg_index_info {

@@ -39,7 +39,7 @@ def build_index(data_path: str, n: int, L: int, leaf: int, index_dir: str, threa
            "--index-path", index_dir, "--serial", "1", "--sims", "--mode", "0",
            "--num-threads", str(threads), "--num-write-threads", str(max(1, threads // 2)),
            "--flush-threshold", str(max(1, threads // 2)), "--initial-db-size", str(min(n, 120000))]
-    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    r = subprocess.run(cmd, capture_output=True, encoding="utf-8", errors="replace", timeout=timeout)
     if r.returncode != 0 or not os.path.exists(index_dir + "root.idx"):
         raise RuntimeError(f"reference build failed rc={r.returncode}: {r.stderr[-2000:]}")
 
@@ -62,7 +62,7 @@ def query(index_dir: str, queries_path: str, nq: int, L: int, k: int, threads: i
         cmd.append("--sims")
     import time
     t0 = time.perf_counter()
-    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    r = subprocess.run(cmd, capture_output=True, encoding="utf-8", errors="replace", timeout=timeout)
     wall = time.perf_counter() - t0
     if r.returncode != 0:
         raise RuntimeError(f"reference query failed rc={r.returncode}: {r.stderr[-2000:]}")
@@ -91,7 +91,7 @@ def pscan(data_path: str, n: int, queries_path: str, nq: int, L: int, k: int, th
            "--queries", os.path.abspath(queries_path), "--queries-size", str(nq), "--timeseries-size", str(L),
            "--k", str(k), "--dbsize", str(min(n, 120000)), "--num-query-threads", str(max(2, threads + 1))]
     t0 = time.perf_counter()
-    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    r = subprocess.run(cmd, capture_output=True, encoding="utf-8", errors="replace", timeout=timeout)
     wall = time.perf_counter() - t0
     if r.returncode != 0:
         raise RuntimeError(f"pscan failed rc={r.returncode}: {r.stderr[-2000:]}")
