@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Benchmark: exact k-NN queries/s on one MI355X over an index resident in HBM.
+
+Workload (defaults): 100M z-normalised random walks of length 256 (FP32,
+102.4 GB in HBM) generated on the device, an EAPCA index built on the device
+by the bulk loader (leaf 10000), and queries that are noisy copies of
+dataset series with sigma ~ U[0, 0.2] and k ~ U{1..100} (seeded).  A "step"
+is one batch of queries answered exactly through the batched index path
+(EAPCA leaf bounds -> best-leaf scan -> iSAX filter + exact refinement ->
+top-k).  Inputs are resident in HBM when the timed region starts.
+
+Prints one JSON line (rank 0).  `python bench.py --gpus N` with N > 1 exits:
+this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse() -> argparse.Namespace:
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--n", type=int, default=100_000_000, help="series in the index")
+    p.add_argument("--length", type=int, default=256)
+    p.add_argument("--batch", type=int, default=1024, help="queries per step")
+    p.add_argument("--leaf", type=int, default=10000)
+    p.add_argument("--path", default="batched", choices=["batched", "index", "scan", "bf16"])
+    p.add_argument("--sigma-max", type=float, default=0.2)
+    p.add_argument("--kmax", type=int, default=100)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-sample", type=int, default=1_000_000, help="rows of the CPU baseline sample")
+    p.add_argument("--parity-queries", type=int, default=8)
+    return p.parse_args()
+
+
+def cpu_baseline(args, Q: np.ndarray, kq: np.ndarray) -> dict:
+    """The reference PSCAN (compiled from its sources into oracle/_ref) on a
+    bounded sample: the first cpu_sample rows of the same dataset (generated
+    by the oracle's CPU generator, identical to the device's), a few of the
+    benchmark's queries; queries/s scaled linearly to the full N (a scan's
+    cost is linear in N)."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    try:
+        import ref_runner as rr
+        from oracle_bind import Oracle
+    except Exception as e:  # pragma: no cover
+        return {"error": f"oracle unavailable: {e}"}
+    if not rr.available():
+        return {"error": "reference binaries not built (oracle/_ref)"}
+    threads = max(2, min(16, os.cpu_count() or 2))
+    o = Oracle()
+    m = min(args.cpu_sample, args.n)
+    nq = 4
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        X = o.series(0, 1, 0, m, args.length, threads=threads)
+        rr.write_f32(os.path.join(d, "data.bin"), X)
+        del X
+        rr.write_f32(os.path.join(d, "q.bin"), Q[:nq])
+        k = int(kq[:nq].max())
+        _, wall, _ = rr.pscan(os.path.join(d, "data.bin"), m, os.path.join(d, "q.bin"), nq, args.length, k,
+                              threads=threads - 1)
+    qps_sample = nq / wall
+    return {"value": qps_sample * m / args.n, "unit": "queries/s", "cores": threads, "kind": "reference",
+            "sample": f"reference PSCAN ({threads - 1} scan workers + 1 reader) over the first {m} rows of the "
+                      f"same dataset, {nq} of the benchmark's queries (k={k}), {wall:.2f} s; queries/s scaled "
+                      f"by {m}/{args.n}"}
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(how the path would shard is described in DESIGN.md)", file=sys.stderr)
+        return 2
+    from hercules_amd import hkg
+
+    L, N, B = args.length, args.n, args.batch
+    t0 = time.perf_counter()
+    dX = hkg.DeviceArray(N * L * 4)
+    hkg.gen_series_device(0, 1, 0, N, L, dX)
+    log(f"generated {N} x {L} in HBM: {time.perf_counter() - t0:.1f} s")
+    t1 = time.perf_counter()
+    ix = hkg.HerculesIndex.build_device(dX, N, L, leaf_size=args.leaf)
+    build_s = time.perf_counter() - t1
+    info = ix.info()
+    log(f"built index: {info['n_leaves']} leaves, {build_s:.1f} s, {info['device_bytes'] / 1e9:.1f} GB")
+
+    steps, warm = args.steps, args.warmup
+    total_q = (steps + warm) * B
+    rng = np.random.default_rng(5)
+    sig = rng.uniform(0.0, args.sigma_max, total_q).astype(np.float32)
+    kq = rng.integers(1, args.kmax + 1, total_q).astype(np.uint32)
+    Qh, src = hkg.gen_noisy_queries(0, 1, N, 5, sig, L)
+    dQ = hkg.DeviceArray(Qh.nbytes)
+    dQ.upload(Qh)
+    dk = hkg.DeviceArray(kq.nbytes)
+    dk.upload(kq)
+    kmax = int(kq.max())
+    dids = hkg.DeviceArray(B * kmax * 8)
+    dd2 = hkg.DeviceArray(B * kmax * 4)
+    # per-step views into the resident query array
+    def run(step: int, profile: bool) -> dict:
+        qv = hkg.DeviceArray.__new__(hkg.DeviceArray)
+        qv.ptr, qv.nbytes = dQ.ptr + step * B * L * 4, B * L * 4
+        kv = hkg.DeviceArray.__new__(hkg.DeviceArray)
+        kv.ptr, kv.nbytes = dk.ptr + step * B * 4, B * 4
+        try:
+            return ix.knn_device(qv, B, kv, kmax, dids, dd2, path=args.path, batch=B, profile=profile)
+        finally:
+            qv.ptr = None
+            kv.ptr = None
+
+    for s in range(warm):
+        st = run(s, True)
+        log(f"warmup step {s}: {st['device_ms']:.1f} ms")
+    hkg.lib().hkg_device_sync()
+    stage_ms = {}
+    stage_bytes = {}
+    stage_launch = {}
+    step_ms = []
+    refined = checked = 0
+    t = time.perf_counter()
+    for s in range(warm, warm + steps):
+        ts = time.perf_counter()
+        st = run(s, True)
+        step_ms.append((time.perf_counter() - ts) * 1e3)
+        for kk, v in st["stage_ms"].items():
+            stage_ms[kk] = stage_ms.get(kk, 0.0) + v
+            stage_bytes[kk] = stage_bytes.get(kk, 0) + st["stage_bytes"][kk]
+            stage_launch[kk] = stage_launch.get(kk, 0) + st["stage_launches"][kk]
+        refined += st["series_refined"]
+        checked += st["series_lb_checked"]
+        log(f"step {s}: {step_ms[-1]:.1f} ms, refined {st['series_refined']}, "
+            f"lb-checked {st['series_lb_checked']}, leaves {st['leaves_after_eapca']}")
+    hkg.lib().hkg_device_sync()
+    elapsed = time.perf_counter() - t
+    qps = steps * B / elapsed
+
+    # dominant kernel and its roofline
+    dom = max(stage_ms, key=lambda k: stage_ms[k])
+    launches = max(stage_launch[dom], 1)
+    avg_ms = stage_ms[dom] / launches
+    bytes_per_launch = stage_bytes[dom] / launches
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    kernel_names = {"refine": "k_filter_refine", "approx": "k_scan_best", "eapca": "k_eapca", "scan": "k_scan",
+                    "topk": "k_select", "collect": "k_collect", "prep": "k_prep", "select_leaf": "k_best_leaf"}
+
+    # parity at full size: index path == full scan on a sample of the queries
+    npar = min(args.parity_queries, B)
+    par = {}
+    if npar:
+        Qs = Qh[:npar]
+        ks = kq[:npar]
+        a = ix.knn(Qs, ks, path=args.path)
+        b = ix.knn(Qs, ks, path="scan")
+        same = bool(np.array_equal(a.ids, b.ids) and np.array_equal(a.d2.view(np.uint32), b.d2.view(np.uint32)))
+        zero = [int(i) for i in range(npar) if sig[i] == 0.0]
+        par = {"index_vs_full_scan_identical": same, "queries_checked": npar}
+        log(f"parity index vs scan on {npar} queries: {same}")
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args, Qh, kq)
+    if cpu:
+        log(f"cpu baseline: {cpu}")
+    out = {
+        "metric": "exact k-NN queries/s at 100M x len256 (HBM-resident index, mixed k and query difficulty)",
+        "value": qps,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": elapsed * 1e3 / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: z-normalised random walks generated on the device (seed 1); queries are dataset "
+                "series plus N(0, sigma^2) noise, sigma ~ U[0, 0.2], renormalised (seed 5)",
+        "config": {"workload": f"{N // 1_000_000}M series x len {L}, leaf {args.leaf}, {B} queries per step, "
+                               f"k ~ U{{1..{args.kmax}}}, path {args.path}",
+                   "n_series": N, "length": L, "batch": B, "leaf_size": args.leaf, "n_leaves": info["n_leaves"],
+                   "build_s": round(build_s, 2)},
+        "roofline": {"bound": "hbm", "kernel": kernel_names.get(dom, dom), "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch},
+        "stages_ms": {k: round(v / steps, 3) for k, v in stage_ms.items() if v > 0},
+        "series_refined_per_query": refined / (steps * B),
+        "series_lb_checked_per_query": checked / (steps * B),
+        "parity": par,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ix.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,603 @@
+// BF16 MFMA certified lower bound: a batched Q.X^T on the matrix cores
+// prunes (query, series) pairs; every survivor is refined exactly in FP32.
+//
+// For a query q and a series x (FP32 values), with q~, x~ their BF16
+// roundings and s the FP32-accumulated MFMA dot product of q~ and x~:
+//   d2(q, x) = |q|^2 + |x|^2 - 2 q.x
+//   |q.x - q~.x~| <= |q - q~| |x| + |q~| |x - x~|          (Cauchy-Schwarz)
+//   |q~.x~ - s|   <= g_L |q~| |x~|,  g_L = 2 L 2^-24        (products of two
+//       BF16 values are exact in FP32; any summation order of L terms)
+// so  LB = |q|^2 + |x|^2 - 2 s - 2 (|q - q~||x| + |q~||x - x~| + g_L |q~||x~|)
+// is a lower bound of d2.  The norms are computed in FP64 and rounded in the
+// safe direction; the FP32 evaluation of LB itself is covered by a further
+// absolute slack, and the comparison with the k-th distance uses the same
+// certified margin as the other bounds.  LB is used only to prune.
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include <hipcub/hipcub.hpp>
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "hkg_engine.h"
+#include "hkg_kernels.h"
+
+namespace hkg {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+namespace {
+
+constexpr int kQPerWave = 64;              // two 32-query MFMA tiles
+constexpr int kQPerBlock = 4 * kQPerWave;  // 256 queries per block
+constexpr int kRowTile = 128;              // series per LDS tile
+constexpr float kEvalSlack = 1.0e-3f;      // FP32 evaluation of the bound
+constexpr uint32_t kBf16SurvivorCap = 524288;  // survivors kept per query (8 B each)
+
+__device__ __forceinline__ uint16_t to_bf16_bits(float f)
+{
+  uint32_t u = __float_as_uint(f);
+  // round to nearest even (finite inputs)
+  uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(r >> 16);
+}
+__device__ __forceinline__ float from_bf16_bits(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ __forceinline__ float rd(double v)
+{
+  float f = (float)v;
+  if ((double)f > v) f = nextafterf(f, -FLT_MAX);
+  return f;
+}
+__device__ __forceinline__ float ru(double v)
+{
+  float f = (float)v;
+  if ((double)f < v) f = nextafterf(f, FLT_MAX);
+  return f;
+}
+
+// rows -> BF16 copy + {|x|^2 (down), |x| (up), |x - x~| (up), |x~| (up)}
+__global__ void k_make_bf16(uint64_t n, uint32_t L, const float *__restrict__ X, uint16_t *__restrict__ Xb,
+                            float4 *__restrict__ st)
+{
+  uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const float *x = X + r * L;
+  uint16_t *xb = Xb + r * L;
+  double n2 = 0.0, e2 = 0.0, b2 = 0.0;
+  for (uint32_t t = 0; t < L; t += 2) {
+    float a = x[t], c = x[t + 1];
+    uint16_t ba = to_bf16_bits(a), bc = to_bf16_bits(c);
+    *reinterpret_cast<uint32_t *>(xb + t) = (uint32_t)ba | ((uint32_t)bc << 16);
+    double da = (double)from_bf16_bits(ba), dc = (double)from_bf16_bits(bc);
+    n2 += (double)a * a + (double)c * c;
+    e2 += ((double)a - da) * ((double)a - da) + ((double)c - dc) * ((double)c - dc);
+    b2 += da * da + dc * dc;
+  }
+  st[r] = make_float4(rd(n2 * (1.0 - 1e-15)), ru(sqrt(n2) * (1.0 + 1e-15)), ru(sqrt(e2) * (1.0 + 1e-15)),
+                      ru(sqrt(b2) * (1.0 + 1e-15)));
+}
+
+// queries -> BF16 + {|q|^2 (down), |q - q~| (up), |q~| (up), 0}
+__global__ void k_query_bf16(uint32_t nq, uint32_t L, const float *__restrict__ Q, uint16_t *__restrict__ Qb,
+                             float4 *__restrict__ st)
+{
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const float *x = Q + (uint64_t)q * L;
+  double n2 = 0.0, e2 = 0.0, b2 = 0.0;
+  for (uint32_t t = 0; t < L; ++t) {
+    float a = x[t];
+    uint16_t b = to_bf16_bits(a);
+    Qb[(uint64_t)q * L + t] = b;
+    double d = (double)from_bf16_bits(b);
+    n2 += (double)a * a;
+    e2 += ((double)a - d) * ((double)a - d);
+    b2 += d * d;
+  }
+  st[q] = make_float4(rd(n2 * (1.0 - 1e-15)), ru(sqrt(e2) * (1.0 + 1e-15)), ru(sqrt(b2) * (1.0 + 1e-15)), 0.0f);
+}
+
+// Filter: block = 4 waves x 64 queries; persistent over row tiles of 128
+// series staged in LDS.  MFMA 32x32x16 BF16: A = 32 queries x 16 dims
+// (lane l: query l%32, dims 8(l/32)..+8), B = 16 dims x 32 series (lane l:
+// series l%32, same dims), D[i][j] in lane (j + 32*((i/4)%2)), register
+// 4*(i/8) + i%4.
+//
+// Block group g (grid.y) serves the queries qperm[256 g .. 256 g + 255] (the
+// batch sorted by first bound, so similar queries share a block) and visits
+// only the row tiles listed for it: tiles that overlap a leaf whose EAPCA
+// bound beats the k-th distance bound of at least one of its queries.
+template <int KS>
+__global__ void __launch_bounds__(256) k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb,
+                                                     const float4 *__restrict__ xst,
+                                                     const uint16_t *__restrict__ Qb,
+                                                     const float4 *__restrict__ qst,
+                                                     const uint32_t *__restrict__ bound,
+                                                     const uint32_t *__restrict__ qperm,
+                                                     const uint32_t *__restrict__ tiles, uint32_t tile_stride,
+                                                     const uint32_t *__restrict__ ntiles_g, uint32_t scap,
+                                                     uint64_t *__restrict__ surv, uint32_t *__restrict__ scnt,
+                                                     const float *__restrict__ thr0, float gamma)
+{
+  constexpr int L = KS * 16;
+  constexpr int kStride = KS * 2 + 1;  // 16-byte vectors per LDS row (one of padding)
+  __shared__ bf16x8 xs[kRowTile * kStride];  // row-major tile
+  __shared__ float s_thr[kQPerBlock], s_qn2[kQPerBlock], s_qe[kQPerBlock], s_qb[kQPerBlock];
+  __shared__ uint8_t s_full[kQPerBlock];  // the query's survivor list is full
+  __shared__ uint32_t s_q[kQPerBlock];     // batch index of every slot
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t qblk = blockIdx.y * kQPerBlock;
+  // per-query constants
+  for (int i = threadIdx.x; i < kQPerBlock; i += 256) {
+    uint32_t slot = qblk + i;
+    uint32_t q = slot < nq ? qperm[slot] : 0xFFFFFFFFu;
+    s_q[i] = q;
+    if (slot < nq) {
+      float4 s = qst[q];
+      s_thr[i] = thr0[q];
+      s_full[i] = 0;
+      s_qn2[i] = s.x;
+      s_qe[i] = s.y;
+      s_qb[i] = s.z;
+    } else {
+      s_thr[i] = -FLT_MAX;  // padding slot: never keeps (and is also excluded below)
+      s_full[i] = 1;
+      s_qn2[i] = 0.0f;
+      s_qe[i] = 0.0f;
+      s_qb[i] = 0.0f;
+    }
+  }
+  __syncthreads();
+  // query operands in registers: 2 tiles x KS steps
+  bf16x8 qa[2][KS];
+  for (int t = 0; t < 2; ++t) {
+    uint32_t q = s_q[wave * kQPerWave + t * 32 + (lane & 31)];
+    const bf16x8 *src = reinterpret_cast<const bf16x8 *>(Qb + (uint64_t)(q < nq ? q : 0) * L);
+#pragma unroll
+    for (int k = 0; k < KS; ++k) qa[t][k] = src[2 * k + (lane >> 5)];
+  }
+  const uint32_t my_tiles = ntiles_g[blockIdx.y];
+  const uint32_t *tlist = tiles + (uint64_t)blockIdx.y * tile_stride;
+  for (uint32_t ti = blockIdx.x; ti < my_tiles; ti += gridDim.x) {
+    const uint64_t r0 = (uint64_t)tlist[ti] * kRowTile;
+    const int rows = (int)(n - r0 < (uint64_t)kRowTile ? n - r0 : (uint64_t)kRowTile);
+    // stage the tile: rows x (L/8) vectors of 16 B
+    const bf16x8 *gsrc = reinterpret_cast<const bf16x8 *>(Xb + r0 * L);
+    for (int v = threadIdx.x; v < kRowTile * KS * 2; v += 256) {
+      int row = v / (KS * 2), col = v - row * (KS * 2);
+      xs[row * kStride + col] = row < rows ? gsrc[v] : bf16x8{};
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int sub = 0; sub < kRowTile / 32; ++sub) {
+      f32x16 acc0 = {}, acc1 = {};
+      const int row = sub * 32 + (lane & 31);
+#pragma unroll
+      for (int k = 0; k < KS; ++k) {
+        bf16x8 xb = xs[row * kStride + 2 * k + (lane >> 5)];
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qa[0][k], xb, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qa[1][k], xb, acc1, 0, 0, 0);
+      }
+      // epilogue: this lane holds series row (lane % 32) of the sub-tile;
+      // for a given register both half-waves hold one query each
+      const bool valid = row < rows;
+      const uint64_t grow = r0 + row;
+      const float4 xsv = valid ? xst[grow] : make_float4(0.f, 0.f, 0.f, 0.f);  // {|x|^2, |x|, |x-x~|, |x~|}
+      const int half = lane >> 5;
+      const uint64_t half_mask = half ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int qi = 8 * (r >> 2) + 4 * half + (r & 3);  // query within the 32-tile
+          const int ql = wave * kQPerWave + t * 32 + qi;
+          const float s = t == 0 ? acc0[r] : acc1[r];
+          const float slack = 2.0f * (s_qe[ql] * xsv.y + s_qb[ql] * xsv.z + gamma * s_qb[ql] * xsv.w) + kEvalSlack;
+          const float lb = (s_qn2[ql] + xsv.x) - 2.0f * s - slack;
+          const uint32_t q = s_q[ql];
+          const bool keep = valid && q < nq && lb <= s_thr[ql] && !s_full[ql];
+          const uint64_t m = __ballot(keep) & half_mask;
+          if (m) {
+            // one counter atomic per half-wave
+            const int leader = __ffsll((long long)m) - 1;
+            const uint32_t cnt = (uint32_t)__popcll(m);
+            uint32_t base = 0;
+            if (lane == leader) {
+              base = atomicAdd(&scnt[q], cnt);
+              if (base + cnt > scap) s_full[ql] = 1;
+            }
+            base = __shfl(base, leader);
+            if (keep) {
+              const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1) & half_mask);
+              if (slot < scap)
+                surv[(uint64_t)q * scap + slot] = ((uint64_t)__float_as_uint(fmaxf(lb, 0.0f)) << 32) | (uint32_t)grow;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+static uint32_t bf16_grid_x(uint32_t groups)
+{
+  int d = 0, cus = 0;
+  (void)hipGetDevice(&d);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d);
+  uint32_t total = (uint32_t)std::max(cus, 1) * 2;
+  return std::max<uint32_t>(1, total / std::max<uint32_t>(groups, 1));
+}
+
+void launch_bf16_filter(hipStream_t st, uint32_t L, uint64_t n, uint32_t nq, const uint16_t *Xb, const float4 *xst,
+                        const uint16_t *Qb, const float4 *qst, const uint32_t *bound, const uint32_t *qperm,
+                        const uint32_t *tiles, uint32_t tile_stride, const uint32_t *ntiles_g, uint32_t scap,
+                        uint64_t *surv, uint32_t *scnt, const float *thr0)
+{
+  const uint32_t groups = (nq + kQPerBlock - 1) / kQPerBlock;
+  const float gamma = 2.0f * (float)L * 5.9604645e-08f;
+  dim3 grid(bf16_grid_x(groups), groups);
+#define HKG_BF16_LAUNCH(KS)                                                                                   \
+  hipLaunchKernelGGL(k_bf16_filter<KS>, grid, dim3(256), 0, st, n, nq, Xb, xst, Qb, qst, bound, qperm, tiles, \
+                     tile_stride, ntiles_g, scap, surv, scnt, thr0, gamma)
+  switch (L) {
+  case 96: HKG_BF16_LAUNCH(6); break;
+  case 128: HKG_BF16_LAUNCH(8); break;
+  default: HKG_BF16_LAUNCH(16); break;
+  }
+#undef HKG_BF16_LAUNCH
+}
+
+// filter threshold of every query: its k-th distance bound with the
+// certified margin
+__global__ void k_bf16_thr_init(uint32_t nq, const uint32_t *__restrict__ bound, float *__restrict__ thr0)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nq) {
+    const uint32_t bb = bound[i];
+    thr0[i] = bb >= kInfBits ? FLT_MAX : __uint_as_float(bb) * (1.0f + kLbRel) + kLbAbs;
+  }
+}
+
+// leaves some query of a block group still needs: EAPCA bound within the
+// query's current k-th distance bound
+__global__ void k_group_leaf_need(uint32_t nq, uint32_t nleaf, const float *__restrict__ lb,
+                                  const uint32_t *__restrict__ bound, const uint32_t *__restrict__ qperm,
+                                  uint8_t *__restrict__ need)
+{
+  const uint32_t leaf = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t g = blockIdx.y;
+  if (leaf >= nleaf) return;
+  uint8_t nd = 0;
+  const uint32_t s0 = g * kQPerBlock;
+  const uint32_t s1 = min(nq, s0 + kQPerBlock);
+  for (uint32_t sl = s0; sl < s1 && !nd; ++sl) {
+    const uint32_t q = qperm[sl];
+    const uint32_t bb = bound[q];
+    const float v = lb[(uint64_t)q * nleaf + leaf];
+    nd = (bb >= kInfBits || v <= __uint_as_float(bb) * (1.0f + kLbRel) + kLbAbs) ? 1 : 0;
+  }
+  need[(uint64_t)g * nleaf + leaf] = nd;
+}
+
+// row tiles of each block group: tiles overlapping a needed leaf
+__global__ void k_group_tiles(uint32_t ntiles, uint32_t nleaf, const uint32_t *__restrict__ tile_leaf0,
+                              const uint32_t *__restrict__ tile_leaf1, const uint8_t *__restrict__ need,
+                              uint32_t *__restrict__ tiles, uint32_t *__restrict__ ntiles_g)
+{
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t g = blockIdx.y;
+  bool keep = false;
+  if (t < ntiles) {
+    for (uint32_t lf = tile_leaf0[t]; lf <= tile_leaf1[t] && !keep; ++lf) keep = need[(uint64_t)g * nleaf + lf] != 0;
+  }
+  const uint64_t m = __ballot(keep);
+  const int lane = threadIdx.x & 63;
+  if (m) {
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&ntiles_g[g], (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (keep) tiles[(uint64_t)g * ntiles + base + __popcll(m & ((1ull << lane) - 1))] = t;
+  }
+}
+
+// batch order by first bound: keys = bound bits, values = batch index
+__global__ void k_iota_bound(uint32_t nq, const uint32_t *__restrict__ bound, uint32_t *__restrict__ keys,
+                             uint32_t *__restrict__ vals)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nq) {
+    keys[i] = bound[i];
+    vals[i] = i;
+  }
+}
+
+hkg_status Index::make_bf16()
+{
+  if (dev.Xb) return HKG_OK;
+  if (dev.L % 16 != 0 || (dev.L != 96 && dev.L != 128 && dev.L != 256)) {
+    set_error("the BF16 filter supports series of length 96, 128 and 256");
+    return HKG_EINVAL;
+  }
+  HKG_HIP_TRY(hipSetDevice(dev.device));
+  const uint64_t n = dev.n;
+  if (hipMalloc((void **)&dev.Xb, n * dev.L * 2) != hipSuccess) {
+    (void)hipGetLastError();
+    dev.Xb = nullptr;
+    set_error("no device memory for the BF16 copy of the series");
+    return HKG_ENOMEM;
+  }
+  if (hipMalloc((void **)&dev.xst, n * sizeof(float4)) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(dev.Xb);
+    dev.Xb = nullptr;
+    set_error("no device memory for the BF16 row norms");
+    return HKG_ENOMEM;
+  }
+  dev.device_bytes += n * dev.L * 2 + n * sizeof(float4);
+  hipLaunchKernelGGL(k_make_bf16, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, n, dev.L, dev.X, dev.Xb,
+                     dev.xst);
+  HKG_HIP_TRY(hipGetLastError());
+  HKG_HIP_TRY(hipStreamSynchronize(stream));
+  if (dev.has_tree) {
+    // first and last leaf overlapping every row tile of the filter
+    const uint32_t ntiles = (uint32_t)((n + kRowTile - 1) / kRowTile);
+    std::vector<uint32_t> l0(ntiles), l1(ntiles);
+    const HostTree &t = dev.tree;
+    uint32_t leaf = 0;
+    for (uint32_t i = 0; i < ntiles; ++i) {
+      uint64_t r0 = (uint64_t)i * kRowTile, r1 = std::min<uint64_t>(r0 + kRowTile, n) - 1;
+      while (leaf + 1 < dev.nleaf && t.nodes[t.leaves[leaf]].file_pos + t.nodes[t.leaves[leaf]].size <= r0) ++leaf;
+      l0[i] = leaf;
+      uint32_t e = leaf;
+      while (e + 1 < dev.nleaf && t.nodes[t.leaves[e]].file_pos + t.nodes[t.leaves[e]].size <= r1) ++e;
+      l1[i] = e;
+    }
+    if (hipMalloc((void **)&dev.tile_leaf0, ntiles * 4) != hipSuccess ||
+        hipMalloc((void **)&dev.tile_leaf1, ntiles * 4) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("no device memory for the tile tables");
+      return HKG_ENOMEM;
+    }
+    dev.ntiles = ntiles;
+    dev.device_bytes += 8ull * ntiles;
+    HKG_HIP_TRY(hipMemcpy(dev.tile_leaf0, l0.data(), ntiles * 4, hipMemcpyHostToDevice));
+    HKG_HIP_TRY(hipMemcpy(dev.tile_leaf1, l1.data(), ntiles * 4, hipMemcpyHostToDevice));
+  }
+  return HKG_OK;
+}
+
+hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax, hipStream_t st,
+                                 QueryBuffers **used, Counters *ctr, Prof *pf, const uint32_t *seed_host)
+{
+  if (!dev.has_tree) {
+    set_error("the BF16 path needs an index (its first bound comes from the best leaf)");
+    return HKG_EINVAL;
+  }
+  const uint32_t L = dev.L, nleaf = dev.nleaf;
+  const uint32_t cap = cand_cap(HKG_PATH_BATCHED, nq, kmax);
+  QueryBuffers *b = nullptr;
+  hkg_status rc = get_buffers(nq, kmax, cap, &b);
+  if (rc != HKG_OK) return rc;
+  const uint32_t want_cap = cap_override ? std::max<uint32_t>(cap_override, 16) : kBf16SurvivorCap;
+  const uint32_t groups_max = (b->B + kQPerBlock - 1) / kQPerBlock;
+  if (b->bf_surv && b->bf_cap != want_cap) b->free_bf16();
+  if (!b->bf_surv) {
+    b->bf_cap = want_cap;
+    b->bf_sort_bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, b->bf_sort_bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                             (const uint32_t *)nullptr, (uint32_t *)nullptr, (int)b->B, 0, 32);
+    if (hipMalloc((void **)&b->bf_surv, (size_t)b->B * b->bf_cap * sizeof(*b->bf_surv)) != hipSuccess ||
+        hipMalloc((void **)&b->bf_cnt, (size_t)b->B * 4) != hipSuccess ||
+        hipMalloc((void **)&b->Qb, (size_t)b->B * L * 2) != hipSuccess ||
+        hipMalloc((void **)&b->qst, (size_t)b->B * sizeof(float4)) != hipSuccess ||
+        hipMalloc((void **)&b->qperm, (size_t)b->B * 4 * 4) != hipSuccess ||
+        hipMalloc(&b->bf_sort_tmp, std::max<size_t>(b->bf_sort_bytes, 16)) != hipSuccess ||
+        hipMalloc((void **)&b->need, (size_t)groups_max * nleaf) != hipSuccess ||
+        hipMalloc((void **)&b->tiles, (size_t)groups_max * dev.ntiles * 4) != hipSuccess ||
+        hipMalloc((void **)&b->ntiles_g, (size_t)groups_max * 4) != hipSuccess ||
+        hipMalloc((void **)&b->thr, (size_t)b->B * 4) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("no device memory for the BF16 path scratch");
+      return HKG_ENOMEM;
+    }
+  }
+  HKG_HIP_TRY(hipMemcpyAsync(b->Q, dQ, (size_t)nq * L * 4, hipMemcpyDeviceToDevice, st));
+  HKG_HIP_TRY(hipMemcpyAsync(b->kq, dk, (size_t)nq * 4, hipMemcpyDeviceToDevice, st));
+  uint32_t *overflow = b->misc + 2;
+  if (seed_host) HKG_HIP_TRY(hipMemcpyAsync(b->seed, seed_host, (size_t)nq * 4, hipMemcpyHostToDevice, st));
+  launch_reset(st, nq, b->bound, b->cand_cnt, seed_host ? b->seed : nullptr);
+  launch_fill(st, b->misc, 8, 0);
+  launch_fill(st, reinterpret_cast<uint32_t *>(b->counters), 2 * kNumCounters, 0);
+  launch_fill(st, b->bf_cnt, nq, 0);
+  if (pf) pf->mark(st, -1);
+  launch_prep(st, b->Q, nq, L, b->paa, b->psum, b->qn2);
+  launch_query_bf16(st, nq, L, b->Q, b->Qb, b->qst);
+  if (pf) pf->mark(st, HKG_STAGE_PREP);
+  launch_eapca(st, nq, nleaf, L, dev.leaf_seg_off, dev.seg_end, dev.seg_ind, b->psum, b->lb);
+  if (pf) pf->mark(st, HKG_STAGE_EAPCA);
+  launch_best_leaf(st, nq, nleaf, b->lb, b->best);
+  if (pf) pf->mark(st, HKG_STAGE_SELECT_LEAF);
+  launch_scan_best(st, nq, dev.X, L, b->Q, b->kq, b->best, dev.leaf_pos, dev.leaf_size, dev.ids, b->cap, b->cand,
+                   b->cand_cnt, b->bound, b->counters);
+  if (pf) pf->mark(st, HKG_STAGE_APPROX);
+  // batch order by first bound, leaves and row tiles each block group needs
+  const uint32_t groups = (nq + kQPerBlock - 1) / kQPerBlock;
+  uint32_t *skeys = b->qperm + b->B, *sidx = b->qperm + 2 * b->B;
+  hipLaunchKernelGGL(k_iota_bound, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->bound, skeys, sidx);
+  HKG_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(b->bf_sort_tmp, b->bf_sort_bytes, skeys, b->qperm + 3 * b->B, sidx,
+                                                 b->qperm, (int)nq, 0, 32, st));
+  launch_fill(st, b->ntiles_g, groups, 0);
+  hipLaunchKernelGGL(k_group_leaf_need, dim3((nleaf + 255) / 256, groups), dim3(256), 0, st, nq, nleaf, b->lb,
+                     b->bound, b->qperm, b->need);
+  hipLaunchKernelGGL(k_group_tiles, dim3((dev.ntiles + 255) / 256, groups), dim3(256), 0, st, dev.ntiles, nleaf,
+                     dev.tile_leaf0, dev.tile_leaf1, b->need, b->tiles, b->ntiles_g);
+  if (pf) pf->mark(st, HKG_STAGE_COLLECT);
+  float *thr0 = reinterpret_cast<float *>(b->thr);
+  hipLaunchKernelGGL(k_bf16_thr_init, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->bound, thr0);
+  launch_bf16_filter(st, L, dev.n, nq, dev.Xb, dev.xst, b->Qb, b->qst, b->bound, b->qperm, b->tiles, dev.ntiles,
+                     b->ntiles_g, b->bf_cap, b->bf_surv, b->bf_cnt, thr0);
+  if (pf) pf->mark(st, HKG_STAGE_REFINE);
+  launch_bf16_refine(st, nq, dev.X, L, b->Q, b->kq, b->kmax, dev.ids, b->cap, b->cand, b->cand_cnt, b->bf_cap,
+                     b->bf_surv, b->bf_cnt, thr0, idmap64, b->out_ids, b->out_d2, overflow, b->counters);
+  if (pf) pf->mark(st, HKG_STAGE_TOPK);
+  HKG_HIP_TRY(hipGetLastError());
+  if (ctr) {
+    HKG_HIP_TRY(hipMemcpyAsync(ctr->v, b->counters, sizeof(ctr->v), hipMemcpyDeviceToHost, st));
+    HKG_HIP_TRY(hipMemcpyAsync(&ctr->overflow, overflow, 4, hipMemcpyDeviceToHost, st));
+    std::vector<uint32_t> nt(groups);
+    HKG_HIP_TRY(hipMemcpyAsync(nt.data(), b->ntiles_g, groups * 4, hipMemcpyDeviceToHost, st));
+    HKG_HIP_TRY(hipStreamSynchronize(st));
+    // (query, series) pairs and row tiles the filter went through
+    for (uint32_t g = 0; g < groups; ++g) {
+      uint64_t qn = std::min<uint32_t>(kQPerBlock, nq - g * kQPerBlock);
+      ctr->v[kCtrBf16] += (unsigned long long)nt[g] * kRowTile * qn;
+      ctr->bf16_rows += (uint64_t)nt[g] * kRowTile;
+    }
+  }
+  *used = b;
+  return HKG_OK;
+}
+
+// Queries whose BF16 survivor list overflowed run the BF16 pass again,
+// seeded with the exact k-th distance of the survivors that were kept (a
+// valid upper bound, far tighter than the first one).  Queries still
+// overflowing after a few rounds are answered by the FP32 index path.
+hkg_status Index::fix_bf16_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax, hipStream_t st, uint32_t *retries)
+{
+  const uint32_t L = dev.L;
+  std::vector<uint32_t> cnt(nq), kv(nq);
+  std::vector<float> d2((size_t)nq * kmax);
+  HKG_HIP_TRY(hipMemcpyAsync(cnt.data(), b.bf_cnt, nq * 4, hipMemcpyDeviceToHost, st));
+  HKG_HIP_TRY(hipMemcpyAsync(kv.data(), b.kq, nq * 4, hipMemcpyDeviceToHost, st));
+  HKG_HIP_TRY(hipMemcpyAsync(d2.data(), b.out_d2, d2.size() * 4, hipMemcpyDeviceToHost, st));
+  HKG_HIP_TRY(hipStreamSynchronize(st));
+  std::vector<uint32_t> todo;
+  for (uint32_t q = 0; q < nq; ++q)
+    if (cnt[q] > b.bf_cap) todo.push_back(q);
+  if (todo.empty()) return HKG_OK;
+  uint32_t m = (uint32_t)todo.size();
+  float *tq = nullptr;
+  uint32_t *tk = nullptr;
+  if (hipMalloc((void **)&tq, (size_t)m * L * 4) != hipSuccess || hipMalloc((void **)&tk, (size_t)m * 4) != hipSuccess) {
+    (void)hipGetLastError();
+    if (tq) (void)hipFree(tq);
+    set_error("no device memory for the BF16 overflow re-run");
+    return HKG_ENOMEM;
+  }
+  std::vector<uint32_t> seed(m);
+  for (uint32_t i = 0; i < m; ++i) {
+    (void)hipMemcpyAsync(tq + (uint64_t)i * L, b.Q + (uint64_t)todo[i] * L, L * 4, hipMemcpyDeviceToDevice, st);
+    (void)hipMemcpyAsync(tk + i, b.kq + todo[i], 4, hipMemcpyDeviceToDevice, st);
+    float kth = d2[(size_t)todo[i] * kmax + kv[todo[i]] - 1];
+    uint32_t bits;
+    memcpy(&bits, &kth, 4);
+    seed[i] = kth < FLT_MAX ? bits : kInfBits;
+  }
+  hkg_status rc = HKG_OK;
+  const bool dbg = getenv("HKG_DEBUG") != nullptr;
+  if (dbg) {
+    uint32_t with_seed = 0, overfull = 0;
+    for (uint32_t i = 0; i < m; ++i) {
+      with_seed += seed[i] < kInfBits;
+      overfull += cnt[todo[i]] != 0xFFFFFFFFu;
+    }
+    fprintf(stderr, "hkg: bf16 re-run: %u queries (%u with an exact seed, %u list overflows)\n", m, with_seed,
+            overfull);
+  }
+  for (int round = 0; round < 4 && m > 0 && rc == HKG_OK; ++round) {
+    ++*retries;
+    const auto t_round = std::chrono::steady_clock::now();
+    QueryBuffers *sb = nullptr;
+    Counters c;
+    std::vector<uint32_t> c2(m);
+    std::vector<float> o2((size_t)m * kmax);
+    if (round < 3) {
+      rc = run_bf16_batch(tq, tk, m, kmax, st, &sb, &c, nullptr, seed.data());
+      if (rc != HKG_OK) break;
+      if (hipMemcpyAsync(c2.data(), sb->bf_cnt, m * 4, hipMemcpyDeviceToHost, st) != hipSuccess) rc = HKG_EDEVICE;
+    } else {
+      // last resort: the FP32 index path (its own overflow handling included)
+      float *td2 = nullptr;
+      uint64_t *tids = nullptr;
+      if (hipMalloc((void **)&tids, (size_t)m * kmax * 8) != hipSuccess ||
+          hipMalloc((void **)&td2, (size_t)m * kmax * 4) != hipSuccess) {
+        (void)hipGetLastError();
+        if (tids) (void)hipFree(tids);
+        rc = HKG_ENOMEM;
+        break;
+      }
+      hkg_query_opts o;
+      hkg_default_query_opts(&o);
+      o.path = HKG_PATH_BATCHED;
+      const auto t_fb = std::chrono::steady_clock::now();
+      rc = knn_device(tq, m, tk, kmax, &o, tids, td2, st, nullptr);
+      if (dbg)
+        fprintf(stderr, "hkg: bf16 re-run: %u queries on the FP32 path, %.1f ms\n", m,
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_fb).count());
+      for (uint32_t i = 0; i < m && rc == HKG_OK; ++i) {
+        (void)hipMemcpyAsync(b.out_ids + (uint64_t)todo[i] * kmax, tids + (uint64_t)i * kmax, kmax * 8,
+                             hipMemcpyDeviceToDevice, st);
+        (void)hipMemcpyAsync(b.out_d2 + (uint64_t)todo[i] * kmax, td2 + (uint64_t)i * kmax, kmax * 4,
+                             hipMemcpyDeviceToDevice, st);
+      }
+      (void)hipStreamSynchronize(st);
+      (void)hipFree(tids);
+      (void)hipFree(td2);
+      break;
+    }
+    if (hipMemcpyAsync(o2.data(), sb->out_d2, o2.size() * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      rc = HKG_EDEVICE;
+      break;
+    }
+    std::vector<uint32_t> again;
+    for (uint32_t i = 0; i < m; ++i) {
+      if (c2[i] > sb->bf_cap) {
+        float kth = o2[(size_t)i * kmax + kv[todo[i]] - 1];
+        uint32_t bits;
+        memcpy(&bits, &kth, 4);
+        if (kth < FLT_MAX && bits < seed[i]) seed[i] = bits;
+        again.push_back(i);
+        continue;
+      }
+      (void)hipMemcpyAsync(b.out_ids + (uint64_t)todo[i] * kmax, sb->out_ids + (uint64_t)i * kmax, kmax * 8,
+                           hipMemcpyDeviceToDevice, st);
+      (void)hipMemcpyAsync(b.out_d2 + (uint64_t)todo[i] * kmax, sb->out_d2 + (uint64_t)i * kmax, kmax * 4,
+                           hipMemcpyDeviceToDevice, st);
+    }
+    for (uint32_t j = 0; j < again.size(); ++j) {
+      uint32_t i = again[j];
+      if (i != j) {
+        (void)hipMemcpyAsync(tq + (uint64_t)j * L, tq + (uint64_t)i * L, L * 4, hipMemcpyDeviceToDevice, st);
+        (void)hipMemcpyAsync(tk + j, tk + i, 4, hipMemcpyDeviceToDevice, st);
+      }
+      seed[j] = seed[i];
+      todo[j] = todo[i];
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) rc = HKG_EDEVICE;
+    if (dbg)
+      fprintf(stderr, "hkg: bf16 re-run round %d: %u queries, %u again, %.1f ms\n", round, m, (uint32_t)again.size(),
+              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_round).count());
+    m = (uint32_t)again.size();
+  }
+  (void)hipFree(tq);
+  (void)hipFree(tk);
+  return rc;
+}
+
+void launch_query_bf16(hipStream_t st, uint32_t nq, uint32_t L, const float *Q, uint16_t *Qb, float4 *qst)
+{
+  hipLaunchKernelGGL(k_query_bf16, dim3((nq + 63) / 64), dim3(64), 0, st, nq, L, Q, Qb, qst);
+}
+
+}  // namespace hkg

@@ -122,8 +122,9 @@ Index::~Index()
   dfree(dev.seg_end);
   dfree(dev.seg_ind);
   dfree(dev.Xb);
-  dfree(dev.xnorm2);
-  dfree(dev.xerr);
+  dfree(dev.xst);
+  dfree(dev.tile_leaf0);
+  dfree(dev.tile_leaf1);
   dfree(idmap64);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -296,6 +297,21 @@ hkg_status Index::set_ids(const uint64_t *ids, uint64_t n)
 // ---------------------------------------------------------------------------
 // per-batch device buffers
 // ---------------------------------------------------------------------------
+void QueryBuffers::free_bf16()
+{
+  dfree(bf_cnt);
+  dfree(bf_surv);
+  dfree(Qb);
+  dfree(qst);
+  dfree(qperm);
+  if (bf_sort_tmp) (void)hipFree(bf_sort_tmp);
+  bf_sort_tmp = nullptr;
+  dfree(need);
+  dfree(tiles);
+  dfree(ntiles_g);
+  dfree(thr);
+}
+
 QueryBuffers::~QueryBuffers()
 {
   if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -309,7 +325,11 @@ QueryBuffers::~QueryBuffers()
   dfree(bound);
   dfree(cand);
   dfree(cand_cnt);
-  dfree(work);
+  dfree(work_keys);
+  dfree(work_vals);
+  if (sort_tmp) (void)hipFree(sort_tmp);
+  dfree(seed);
+  free_bf16();
   dfree(misc);
   dfree(counters);
   dfree(out_ids);
@@ -325,15 +345,22 @@ hkg_status Index::get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuff
     *out = it->second.get();
     return HKG_OK;
   }
-  // keep at most a few buffer sets alive
-  if (bufs.size() >= 4) bufs.clear();
+  // keep at most a few buffer sets alive; never the one a caller still uses
+  if (bufs.size() >= 4) {
+    for (auto e = bufs.begin(); e != bufs.end();) {
+      if (std::find(in_use.begin(), in_use.end(), e->second.get()) == in_use.end()) e = bufs.erase(e);
+      else ++e;
+    }
+  }
   std::unique_ptr<QueryBuffers> b(new QueryBuffers());
   b->B = B;
   b->kmax = kmax;
   b->cap = cap;
-  b->work_cap = (uint64_t)B * std::max<uint32_t>(nleaf, 1) > 0xFFFFFFFFull
-                    ? 0xFFFFFFFFu
-                    : (uint32_t)((uint64_t)B * std::max<uint32_t>(nleaf, 1));
+  if ((uint64_t)B * std::max<uint32_t>(nleaf, 1) >= 0x7FFFFFFFull) {
+    set_error("batch x leaves exceeds 2^31; use a smaller batch");
+    return HKG_EINVAL;
+  }
+  b->work_cap = B * std::max<uint32_t>(nleaf, 1);
   uint64_t acct = 0;
   hkg_status rc;
 #define A(ptr, cnt) \
@@ -344,11 +371,23 @@ hkg_status Index::get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuff
   A(psum, (uint64_t)B * 2 * (L + 1));
   A(qn2, B);
   A(lb, (uint64_t)B * std::max<uint32_t>(nleaf, 1));
-  A(best, B);
+  A(best, (uint64_t)B * kBestLeaves);
   A(bound, B);
   A(cand, (uint64_t)B * cap);
   A(cand_cnt, B);
-  A(work, dev.has_tree ? b->work_cap : 1);
+  const uint64_t pairs = dev.has_tree ? (uint64_t)B * nleaf : 1;
+  A(work_keys, 2 * pairs);
+  A(work_vals, 2 * pairs);
+  A(seed, B);
+  if (dev.has_tree) {
+    b->sort_bytes = sort_work_temp_bytes((uint32_t)pairs);
+    if (hipMalloc(&b->sort_tmp, std::max<size_t>(b->sort_bytes, 16)) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("hipMalloc of the sort scratch failed");
+      return HKG_ENOMEM;
+    }
+    acct += b->sort_bytes;
+  }
   A(misc, 8);
   A(counters, kNumCounters);
   A(out_ids, (uint64_t)B * kmax);
@@ -375,11 +414,11 @@ static uint32_t persistent_grid()
   return g;
 }
 
-void Index::enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf)
+void Index::enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf, bool seeded)
 {
   const uint32_t L = dev.L, nleaf = dev.nleaf;
   uint32_t *nwork = b.misc + 0, *work_next = b.misc + 1, *overflow = b.misc + 2;
-  launch_reset(st, nq, b.bound, b.cand_cnt);
+  launch_reset(st, nq, b.bound, b.cand_cnt, seeded ? b.seed : nullptr);
   launch_fill(st, b.misc, 8, 0);
   launch_fill(st, reinterpret_cast<uint32_t *>(b.counters), 2 * kNumCounters, 0);
   if (pf) pf->mark(st, -1);
@@ -392,10 +431,13 @@ void Index::enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Pr
   launch_scan_best(st, nq, dev.X, L, b.Q, b.kq, b.best, dev.leaf_pos, dev.leaf_size, dev.ids, b.cap, b.cand,
                    b.cand_cnt, b.bound, b.counters);
   if (pf) pf->mark(st, HKG_STAGE_APPROX);
-  launch_collect(st, nq, nleaf, b.lb, b.best, b.bound, dev.leaf_size, b.work, b.work_cap, nwork, b.counters);
+  const uint32_t pairs = nq * nleaf;
+  uint32_t *keys2 = b.work_keys + b.work_cap, *vals2 = b.work_vals + b.work_cap;
+  launch_collect(st, nq, nleaf, b.lb, b.best, b.bound, b.work_keys, b.work_vals, nwork);
+  (void)launch_sort_work(st, b.sort_tmp, b.sort_bytes, b.work_keys, keys2, b.work_vals, vals2, pairs);
   if (pf) pf->mark(st, HKG_STAGE_COLLECT);
   launch_filter_refine(st, persistent_grid(), dev.X, dev.sax, L, b.Q, b.paa, b.kq, dev.leaf_pos, dev.leaf_size,
-                       dev.ids, b.work, nwork, b.work_cap, work_next, b.cap, b.cand, b.cand_cnt, b.bound,
+                       dev.ids, nleaf, b.lb, vals2, nwork, work_next, b.cap, b.cand, b.cand_cnt, b.bound,
                        b.counters);
   if (pf) pf->mark(st, HKG_STAGE_REFINE);
   launch_select(st, nq, b.kmax, b.kq, b.cap, b.cand, b.cand_cnt, idmap64, b.out_ids, b.out_d2, overflow);
@@ -426,7 +468,7 @@ uint32_t scan_grid_x(uint32_t nq)
 void Index::enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf)
 {
   uint32_t *overflow = b.misc + 2;
-  launch_reset(st, nq, b.bound, b.cand_cnt);
+  launch_reset(st, nq, b.bound, b.cand_cnt, nullptr);
   launch_fill(st, b.misc, 8, 0);
   launch_fill(st, reinterpret_cast<uint32_t *>(b.counters), 2 * kNumCounters, 0);
   if (pf) pf->mark(st, -1);
@@ -441,11 +483,13 @@ void Index::enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Pro
 uint32_t Index::cand_cap(hkg_path path, uint32_t nq, uint32_t kmax) const
 {
   if (path == HKG_PATH_SCAN || !dev.has_tree) return scan_grid_x(nq) * kmax + 256;
+  if (cap_override) return std::max<uint32_t>(cap_override, kmax);
   return std::max<uint32_t>(8192, 16 * kmax);
 }
 
 hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax,
-                            bool use_graph, hipStream_t st, QueryBuffers **used, Counters *ctr, Prof *pf)
+                            bool use_graph, hipStream_t st, QueryBuffers **used, Counters *ctr, Prof *pf,
+                            const uint32_t *seed_host)
 {
   const bool scan = (path == HKG_PATH_SCAN) || !dev.has_tree;
   const uint32_t cap = cand_cap(scan ? HKG_PATH_SCAN : path, nq, kmax);
@@ -454,13 +498,18 @@ hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, 
   if (rc != HKG_OK) return rc;
   HKG_HIP_TRY(hipMemcpyAsync(b->Q, dQ, (size_t)nq * dev.L * 4, hipMemcpyDeviceToDevice, st));
   HKG_HIP_TRY(hipMemcpyAsync(b->kq, dk, (size_t)nq * 4, hipMemcpyDeviceToDevice, st));
+  if (seed_host) {
+    HKG_HIP_TRY(hipMemcpyAsync(b->seed, seed_host, (size_t)nq * 4, hipMemcpyHostToDevice, st));
+    HKG_HIP_TRY(hipStreamSynchronize(st));
+    use_graph = false;
+  }
   if (use_graph && !scan && !pf) {
     if (!b->graph_exec || b->graph_stream != st) {
       if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
       b->graph_exec = nullptr;
       hipGraph_t g;
       HKG_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      enqueue_index_batch(*b, nq, st, nullptr);
+      enqueue_index_batch(*b, nq, st, nullptr, false);
       HKG_HIP_TRY(hipStreamEndCapture(st, &g));
       hipError_t e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
@@ -471,7 +520,7 @@ hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, 
   } else if (scan) {
     enqueue_scan_batch(*b, nq, st, pf);
   } else {
-    enqueue_index_batch(*b, nq, st, pf);
+    enqueue_index_batch(*b, nq, st, pf, seed_host != nullptr);
   }
   HKG_HIP_TRY(hipGetLastError());
   if (ctr) {
@@ -483,26 +532,100 @@ hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, 
   return HKG_OK;
 }
 
-// overflowed queries (candidate buffer too small) are answered again by the
-// full scan, whose buffer is sized so that it cannot overflow
+// Overflowed queries (their candidate buffer filled up) run again on the
+// index path, seeded with the k-th distance of the candidates that were
+// kept -- a valid upper bound, usually tight, so the re-run keeps few
+// candidates.  A query still overflowing after a few rounds is answered by
+// the full scan, whose buffer is sized so that it cannot overflow.
 hkg_status Index::fix_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax, hipStream_t st, uint32_t *retries)
 {
-  std::vector<uint32_t> cnt(nq);
+  const uint32_t L = dev.L;
+  std::vector<uint32_t> cnt(nq), kv(nq);
+  std::vector<float> d2((size_t)nq * kmax);
   HKG_HIP_TRY(hipMemcpyAsync(cnt.data(), b.cand_cnt, nq * 4, hipMemcpyDeviceToHost, st));
+  HKG_HIP_TRY(hipMemcpyAsync(kv.data(), b.kq, nq * 4, hipMemcpyDeviceToHost, st));
+  HKG_HIP_TRY(hipMemcpyAsync(d2.data(), b.out_d2, d2.size() * 4, hipMemcpyDeviceToHost, st));
   HKG_HIP_TRY(hipStreamSynchronize(st));
-  for (uint32_t q = 0; q < nq; ++q) {
-    if (cnt[q] <= b.cap) continue;
+  std::vector<uint32_t> todo;
+  for (uint32_t q = 0; q < nq; ++q)
+    if (cnt[q] > b.cap) todo.push_back(q);
+  if (todo.empty()) return HKG_OK;
+  uint32_t m = (uint32_t)todo.size();
+  float *tq = nullptr;
+  uint32_t *tk = nullptr;
+  hkg_status rc = dalloc(&tq, (uint64_t)m * L, nullptr);
+  if (rc != HKG_OK) return rc;
+  if ((rc = dalloc(&tk, m, nullptr)) != HKG_OK) {
+    dfree(tq);
+    return rc;
+  }
+  std::vector<uint32_t> seed(m);
+  for (uint32_t i = 0; i < m; ++i) {
+    uint32_t q = todo[i];
+    (void)hipMemcpyAsync(tq + (uint64_t)i * L, b.Q + (uint64_t)q * L, L * 4, hipMemcpyDeviceToDevice, st);
+    (void)hipMemcpyAsync(tk + i, b.kq + q, 4, hipMemcpyDeviceToDevice, st);
+    float kth = d2[(size_t)q * kmax + kv[q] - 1];
+    uint32_t bits;
+    memcpy(&bits, &kth, 4);
+    seed[i] = kth < FLT_MAX ? bits : kInfBits;
+  }
+  for (int round = 0; round < 4 && rc == HKG_OK; ++round) {
     ++*retries;
     QueryBuffers *sb = nullptr;
     Counters c;
-    hkg_status rc =
-        run_batch(HKG_PATH_SCAN, b.Q + (uint64_t)q * dev.L, b.kq + q, 1, kmax, false, st, &sb, &c, nullptr);
-    if (rc != HKG_OK) return rc;
-    HKG_HIP_TRY(hipMemcpyAsync(b.out_ids + (uint64_t)q * kmax, sb->out_ids, kmax * 8, hipMemcpyDeviceToDevice, st));
-    HKG_HIP_TRY(hipMemcpyAsync(b.out_d2 + (uint64_t)q * kmax, sb->out_d2, kmax * 4, hipMemcpyDeviceToDevice, st));
-    HKG_HIP_TRY(hipStreamSynchronize(st));
+    const bool last = round == 3;
+    rc = run_batch(last ? HKG_PATH_SCAN : HKG_PATH_BATCHED, tq, tk, m, kmax, false, st, &sb, &c, nullptr,
+                   last ? nullptr : seed.data());
+    if (rc != HKG_OK) break;
+    std::vector<uint32_t> c2(m);
+    std::vector<float> o2((size_t)m * kmax);
+    if (hipMemcpyAsync(c2.data(), sb->cand_cnt, m * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(o2.data(), sb->out_d2, o2.size() * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      set_error("overflow re-run readback failed");
+      rc = HKG_EDEVICE;
+      break;
+    }
+    bool again = false;
+    for (uint32_t i = 0; i < m; ++i) {
+      if (c2[i] > sb->cap) {
+        again = true;
+        float kth = o2[(size_t)i * kmax + kv[todo[i]] - 1];
+        uint32_t bits;
+        memcpy(&bits, &kth, 4);
+        if (kth < FLT_MAX && bits < seed[i]) seed[i] = bits;
+        continue;
+      }
+      (void)hipMemcpyAsync(b.out_ids + (uint64_t)todo[i] * kmax, sb->out_ids + (uint64_t)i * kmax, kmax * 8,
+                           hipMemcpyDeviceToDevice, st);
+      (void)hipMemcpyAsync(b.out_d2 + (uint64_t)todo[i] * kmax, sb->out_d2 + (uint64_t)i * kmax, kmax * 4,
+                           hipMemcpyDeviceToDevice, st);
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) {
+      rc = HKG_EDEVICE;
+      break;
+    }
+    if (!again) break;
+    // keep only the queries that overflowed again
+    std::vector<uint32_t> keep;
+    for (uint32_t i = 0; i < m; ++i)
+      if (c2[i] > sb->cap) keep.push_back(i);
+    for (uint32_t j = 0; j < keep.size(); ++j) {
+      uint32_t i = keep[j];
+      (void)hipMemcpyAsync(tq + (uint64_t)j * L, tq + (uint64_t)i * L, L * 4, hipMemcpyDeviceToDevice, st);
+      (void)hipMemcpyAsync(tk + j, tk + i, 4, hipMemcpyDeviceToDevice, st);
+      seed[j] = seed[i];
+      todo[j] = todo[i];
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) {
+      rc = HKG_EDEVICE;
+      break;
+    }
+    m = (uint32_t)keep.size();
   }
-  return HKG_OK;
+  dfree(tq);
+  dfree(tk);
+  return rc;
 }
 
 hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, uint32_t kmax,
@@ -528,6 +651,7 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
   if (batch == 0) batch = path == HKG_PATH_INDEX ? 1 : (path == HKG_PATH_SCAN ? 64 : 1024);
   if (path == HKG_PATH_INDEX) batch = 1;
   if (batch > 16384) batch = 16384;  // grid.y of the per-query launches
+  cap_override = o.cand_cap;
   HKG_HIP_TRY(hipSetDevice(dev.device));
   if (!st) st = stream;
   hipEvent_t e0, e1;
@@ -562,7 +686,10 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     ++launches;
     if (pf) pf->accumulate(stage_ms, stage_launches);
     if (c.overflow) {
-      rc = fix_overflow(*b, n, kmax, st, &retries);
+      in_use.push_back(b);
+      rc = path == HKG_PATH_BF16 ? fix_bf16_overflow(*b, n, kmax, st, &retries)
+                                 : fix_overflow(*b, n, kmax, st, &retries);
+      in_use.pop_back();
       if (rc != HKG_OK) return rc;
     }
     HKG_HIP_TRY(hipMemcpyAsync(dids + (uint64_t)q0 * kmax, b->out_ids, (size_t)n * kmax * 8,
@@ -571,6 +698,7 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
                                hipMemcpyDeviceToDevice, st));
     for (int i = 0; i < kNumCounters; ++i) total.v[i] += c.v[i];
     total.nwork += c.nwork;
+    total.bf16_rows += c.bf16_rows;
   }
   HKG_HIP_TRY(hipEventRecord(e1, st));
   HKG_HIP_TRY(hipStreamSynchronize(st));
@@ -586,6 +714,7 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     stats->leaves_after_eapca = scan ? 0 : total.v[kCtrLeaves] + nq;
     stats->series_total = (uint64_t)nq * dev.n;
     stats->series_lb_checked = scan ? 0 : total.v[kCtrSaxChecked] + total.v[kCtrBf16];
+    stats->bf16_pairs = total.v[kCtrBf16];
     stats->series_refined = scan ? (uint64_t)nq * dev.n : total.v[kCtrRefined];
     stats->device_ms = ms;
     stats->overflow_retries = retries;
@@ -597,7 +726,9 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     const uint64_t L4 = (uint64_t)dev.L * 4;
     stats->stage_bytes[HKG_STAGE_APPROX] = total.v[kCtrApprox] * L4;
     stats->stage_bytes[HKG_STAGE_REFINE] =
-        total.v[kCtrSaxChecked] * 16 + total.v[kCtrBf16] * dev.L * 2 + total.v[kCtrRefined] * L4;
+        path == HKG_PATH_BF16 ? total.bf16_rows * ((uint64_t)dev.L * 2 + 16)
+                              : total.v[kCtrSaxChecked] * 16 + total.v[kCtrRefined] * L4;
+    if (path == HKG_PATH_BF16) stats->stage_bytes[HKG_STAGE_TOPK] = total.v[kCtrRefined] * L4;
     stats->stage_bytes[HKG_STAGE_SCAN] = scan ? (uint64_t)launches * 0 + total_scan_bytes(nq, batch) : 0;
     stats->stage_bytes[HKG_STAGE_EAPCA] = scan ? 0 : (uint64_t)nq * dev.nleaf * 8 + total_segs() * 18ull * nq;
     stats->bytes_algorithmic = 0;

@@ -4,6 +4,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "hkg_internal.h"
 
@@ -15,6 +16,7 @@ struct Counters {
   unsigned long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t overflow = 0;
   uint32_t nwork = 0;
+  uint64_t bf16_rows = 0;    // rows the BF16 filter read (summed over block groups)
 };
 
 // per-stage event timing of one batch (profile mode)
@@ -68,17 +70,29 @@ struct QueryBuffers {
   uint32_t *bound = nullptr;    // B, FP32 bits of the current k-th distance bound
   uint64_t *cand = nullptr;     // B x cap candidate keys
   uint32_t *cand_cnt = nullptr; // B
-  uint64_t *work = nullptr;     // work_cap (query, leaf) items
+  uint32_t *work_keys = nullptr; // 2 x B x nleaf: EAPCA bound bits (sort keys), double buffer
+  uint32_t *work_vals = nullptr; // 2 x B x nleaf: pair index q * nleaf + leaf
+  void *sort_tmp = nullptr;
+  size_t sort_bytes = 0;
+  uint32_t *seed = nullptr;      // B: k-th distance bound seeds (overflow re-runs)
   uint32_t *misc = nullptr;     // [0] nwork [1] work_next [2] overflow
   unsigned long long *counters = nullptr;
   uint64_t *out_ids = nullptr;  // B x kmax
   float *out_d2 = nullptr;      // B x kmax
   // BF16 path scratch
-  uint32_t *bf_cnt = nullptr;   // per query survivor counters
-  uint64_t *bf_surv = nullptr;  // survivors (query << 32 | row)
+  uint32_t *bf_cnt = nullptr;   // B survivor counters
+  uint64_t *bf_surv = nullptr;  // B x bf_cap survivors: (BF16 bound bits << 32) | row
   uint32_t bf_cap = 0;
   uint16_t *Qb = nullptr;       // B x L bf16 bits
-  float *qerr = nullptr;        // B
+  float4 *qst = nullptr;        // B x {|q|^2, |q - q~|, |q~|, 0}
+  uint32_t *qperm = nullptr;    // 4 x B: batch order by first bound, then sort scratch
+  void *bf_sort_tmp = nullptr;
+  size_t bf_sort_bytes = 0;
+  uint8_t *need = nullptr;      // groups x nleaf: leaf needed by a block group
+  uint32_t *tiles = nullptr;    // groups x ntiles: row tiles of each block group
+  uint32_t *ntiles_g = nullptr; // groups
+  uint32_t *thr = nullptr;      // B: filter threshold (float bits)
+  void free_bf16();
   // captured stage sequence of the batched path
   hipGraphExec_t graph_exec = nullptr;
   hipStream_t graph_stream = nullptr;
@@ -89,6 +103,8 @@ struct Index {
   DevIndex dev;
   hipStream_t stream = nullptr;
   uint64_t *idmap64 = nullptr;  // rank -> public id (when an id map is attached)
+  uint32_t cap_override = 0;    // candidate buffer entries per query requested by the caller
+  std::vector<QueryBuffers *> in_use;  // buffer sets the buffer cache must not evict
   std::map<uint64_t, std::unique_ptr<QueryBuffers>> bufs;
 
   ~Index();
@@ -98,15 +114,17 @@ struct Index {
   hkg_status make_bf16();
   hkg_status get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuffers **out);
   uint32_t cand_cap(hkg_path path, uint32_t nq, uint32_t kmax) const;
-  void enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf);
+  void enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf, bool seeded);
   void enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf);
   hkg_status run_batch(hkg_path path, const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax,
-                       bool use_graph, hipStream_t st, QueryBuffers **used, Counters *ctr, Prof *pf);
+                       bool use_graph, hipStream_t st, QueryBuffers **used, Counters *ctr, Prof *pf,
+                       const uint32_t *seed_host = nullptr);
   hkg_status run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax, hipStream_t st,
-                            QueryBuffers **used, Counters *ctr, Prof *pf);
+                            QueryBuffers **used, Counters *ctr, Prof *pf, const uint32_t *seed_host = nullptr);
   uint64_t total_scan_bytes(uint32_t nq, uint32_t batch) const;
   uint64_t total_segs() const;
   hkg_status fix_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax, hipStream_t st, uint32_t *retries);
+  hkg_status fix_bf16_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax, hipStream_t st, uint32_t *retries);
   hkg_status knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, uint32_t kmax, const hkg_query_opts *qo,
                         uint64_t *dids, float *dd2, hipStream_t st, hkg_query_stats *stats);
   hkg_status knn_host(const float *Q, uint32_t nq, const uint32_t *k, uint32_t kmax, const hkg_query_opts *qo,

@@ -84,9 +84,10 @@ struct DevIndex {
   int16_t *seg_end = nullptr;        // total segments
   float4 *seg_ind = nullptr;         // total segments
   // BF16 filter data (optional)
-  uint16_t *Xb = nullptr;      // n x L bf16 bits
-  float *xnorm2 = nullptr;     // n, ||x||^2 (FP64 computed, rounded up)
-  float *xerr = nullptr;       // n, ||x - bf16(x)||_2 (FP64 computed, rounded up)
+  uint16_t *Xb = nullptr;      // n x L bf16 bits (round to nearest even)
+  float4 *xst = nullptr;       // n x {|x|^2 down, |x| up, |x - bf16(x)| up, |bf16(x)| up}
+  uint32_t ntiles = 0;         // 128-row tiles of the BF16 filter
+  uint32_t *tile_leaf0 = nullptr, *tile_leaf1 = nullptr;  // leaves overlapping each tile
   uint64_t device_bytes = 0;
   // host copies kept for writing / info
   HostTree tree;

@@ -9,6 +9,8 @@ constexpr int kRefineTile = 2048;        // floats of chunk sums per wave (refin
 constexpr int kScanQ = 4;                // queries per block in the full scan
 constexpr int kScanTile = 2048;          // floats of chunk sums per wave (scan)
 constexpr size_t kEapcaLdsBytes = 48 * 1024;
+constexpr int kWorkChunk = 8;            // work items taken per atomic
+constexpr int kBestLeaves = 4;           // leaves per query scanned in the approximate stage
 
 // device counters (unsigned long long each)
 enum Counter : int {
@@ -22,7 +24,7 @@ enum Counter : int {
 
 hipError_t upload_breakpoints_dev();
 
-void launch_reset(hipStream_t st, uint32_t nq, uint32_t *bound, uint32_t *cand_cnt);
+void launch_reset(hipStream_t st, uint32_t nq, uint32_t *bound, uint32_t *cand_cnt, const uint32_t *seed);
 void launch_fill(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v);
 void launch_prep(hipStream_t st, const float *Q, uint32_t nq, uint32_t L, float *paa, double *psum,
                  double *qn2);
@@ -34,17 +36,24 @@ void launch_scan_best(hipStream_t st, uint32_t nq, const float *X, uint32_t L, c
                       const uint32_t *leaf_size, const uint32_t *ids, uint32_t cap, uint64_t *cand,
                       uint32_t *cand_cnt, uint32_t *bound, unsigned long long *counters);
 void launch_collect(hipStream_t st, uint32_t nq, uint32_t nleaf, const float *lb, const uint32_t *best,
-                    const uint32_t *bound, const uint32_t *leaf_size, uint64_t *work, uint32_t work_cap,
-                    uint32_t *nwork, unsigned long long *counters);
+                    const uint32_t *bound, uint32_t *keys, uint32_t *vals, uint32_t *nwork);
+size_t sort_work_temp_bytes(uint32_t n);
+hipError_t launch_sort_work(hipStream_t st, void *tmp, size_t tmp_bytes, const uint32_t *keys, uint32_t *keys_out,
+                            const uint32_t *vals, uint32_t *vals_out, uint32_t n);
 void launch_filter_refine(hipStream_t st, uint32_t grid, const float *X, const uint8_t *sax, uint32_t L,
                           const float *Q, const float *paa, const uint32_t *kq, const uint64_t *leaf_pos,
-                          const uint32_t *leaf_size, const uint32_t *ids, const uint64_t *work,
-                          const uint32_t *nwork, uint32_t work_cap, uint32_t *work_next, uint32_t cap,
-                          uint64_t *cand, uint32_t *cand_cnt, uint32_t *bound,
-                          unsigned long long *counters);
+                          const uint32_t *leaf_size, const uint32_t *ids, uint32_t nleaf, const float *lb,
+                          const uint32_t *work, const uint32_t *nwork, uint32_t *work_next, uint32_t cap,
+                          uint64_t *cand, uint32_t *cand_cnt, uint32_t *bound, unsigned long long *counters);
 void launch_scan(hipStream_t st, uint32_t grid_x, const float *X, uint64_t n, uint32_t L, const float *Q,
                  uint32_t nq, const uint32_t *kq, const uint32_t *ids, uint32_t cap, uint64_t *cand,
                  uint32_t *cand_cnt, uint32_t *bound);
+void launch_bf16_refine(hipStream_t st, uint32_t nq, const float *X, uint32_t L, const float *Q, const uint32_t *kq,
+                        uint32_t kmax, const uint32_t *ids, uint32_t cap, const uint64_t *cand,
+                        const uint32_t *cand_cnt, uint32_t scap, const uint64_t *surv, uint32_t *scnt,
+                        const float *thr0, const uint64_t *idmap64, uint64_t *out_ids, float *out_d2,
+                        uint32_t *overflow, unsigned long long *counters);
+void launch_query_bf16(hipStream_t st, uint32_t nq, uint32_t L, const float *Q, uint16_t *Qb, float4 *qst);
 void launch_select(hipStream_t st, uint32_t nq, uint32_t kmax, const uint32_t *kq, uint32_t cap,
                    const uint64_t *cand, const uint32_t *cand_cnt, const uint64_t *idmap64, uint64_t *out_ids,
                    float *out_d2, uint32_t *overflow);

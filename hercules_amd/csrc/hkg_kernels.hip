@@ -13,6 +13,8 @@
 
 #include <float.h>
 
+#include <hipcub/hipcub.hpp>
+
 namespace hkg {
 
 __constant__ float c_bp[256];  // card-256 breakpoints, [255] unused
@@ -321,27 +323,41 @@ __global__ void __launch_bounds__(kBlock) k_eapca(uint32_t nleaf, uint32_t L,
   lb[(uint64_t)q * nleaf + leaf] = (float)sum;
 }
 
-// per query: the leaf with the smallest bound (ties: smallest leaf index)
+// per query: the kBestLeaves leaves with the smallest bounds (ties: smaller
+// leaf index first); missing ones (fewer leaves) are marked 0xFFFFFFFF
 __global__ void __launch_bounds__(kBlock) k_best_leaf(uint32_t nleaf, const float *__restrict__ lb,
                                                       uint32_t *__restrict__ best)
 {
   __shared__ uint64_t red[kBlock];
   const uint32_t q = blockIdx.x;
-  uint64_t m = ~0ull;
-  for (uint32_t i = threadIdx.x; i < nleaf; i += kBlock) {
-    uint64_t key = ((uint64_t)__float_as_uint(lb[(uint64_t)q * nleaf + i]) << 32) | i;
-    m = key < m ? key : m;
-  }
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int off = kBlock / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) {
-      uint64_t o = red[threadIdx.x + off];
-      if (o < red[threadIdx.x]) red[threadIdx.x] = o;
+  uint64_t prev = 0;
+  bool have_prev = false;
+  for (int m = 0; m < kBestLeaves; ++m) {
+    uint64_t mk = ~0ull;
+    for (uint32_t i = threadIdx.x; i < nleaf; i += kBlock) {
+      uint64_t key = ((uint64_t)__float_as_uint(lb[(uint64_t)q * nleaf + i]) << 32) | i;
+      if ((!have_prev || key > prev) && key < mk) mk = key;
     }
+    red[threadIdx.x] = mk;
     __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < off) {
+        uint64_t o = red[threadIdx.x + off];
+        if (o < red[threadIdx.x]) red[threadIdx.x] = o;
+      }
+      __syncthreads();
+    }
+    const uint64_t found = red[0];
+    __syncthreads();
+    if (threadIdx.x == 0) best[q * kBestLeaves + m] = found == ~0ull ? 0xFFFFFFFFu : (uint32_t)found;
+    prev = found;
+    have_prev = true;
+    if (found == ~0ull) {
+      for (int r = m + 1; r < kBestLeaves; ++r)
+        if (threadIdx.x == 0) best[q * kBestLeaves + r] = 0xFFFFFFFFu;
+      break;
+    }
   }
-  if (threadIdx.x == 0) best[q] = (uint32_t)(red[0] & 0xFFFFFFFFu);
 }
 
 __device__ __forceinline__ void load_query(const float *__restrict__ Q, uint32_t q, uint32_t L, float *qs)
@@ -374,7 +390,8 @@ __device__ void flush_topk(TopK &tk, int k, uint32_t q, uint32_t cap, uint64_t *
   __syncthreads();
 }
 
-// approximate phase: the best leaf of every query, scanned in full
+// approximate phase: the best leaves of every query, each scanned in full
+// by its own block (grid.y = rank of the leaf)
 __global__ void __launch_bounds__(kBlock) k_scan_best(const float *__restrict__ X, uint32_t L,
                                                       const float *__restrict__ Q,
                                                       const uint32_t *__restrict__ kq,
@@ -394,9 +411,10 @@ __global__ void __launch_bounds__(kBlock) k_scan_best(const float *__restrict__ 
   __shared__ uint32_t scan[kBlock];
   const uint32_t q = blockIdx.x;
   const int k = (int)kq[q];
+  const uint32_t leaf = best[q * kBestLeaves + blockIdx.y];
+  if (leaf == 0xFFFFFFFFu) return;  // uniform for the block
   load_query(Q, q, L, qs);
   topk_reset(tk);
-  const uint32_t leaf = best[q];
   const uint64_t pos = leaf_pos[leaf];
   const uint32_t size = leaf_size[leaf];
   uint32_t nref = 0;
@@ -413,45 +431,44 @@ __global__ void __launch_bounds__(kBlock) k_scan_best(const float *__restrict__ 
   if (threadIdx.x == 0) atomicAdd(&counters[kCtrApprox], (unsigned long long)size);
 }
 
-// work list of (query, leaf) pairs that survive the EAPCA bound
+// dense work list of every (query, leaf) pair: key = EAPCA bound bits when
+// the pair survives the first bound, else all ones; the value is the pair's
+// index q * nleaf + leaf.  Sorting by key gives a best-first order over all
+// queries of the batch.
 __global__ void __launch_bounds__(kBlock) k_collect(uint32_t nq, uint32_t nleaf,
                                                     const float *__restrict__ lb,
                                                     const uint32_t *__restrict__ best,
                                                     const uint32_t *__restrict__ bound,
-                                                    const uint32_t *__restrict__ leaf_size,
-                                                    uint64_t *__restrict__ work, uint32_t work_cap,
-                                                    uint32_t *__restrict__ nwork,
-                                                    unsigned long long *__restrict__ counters)
+                                                    uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                    uint32_t *__restrict__ nwork)
 {
   const uint32_t q = blockIdx.y;
   const uint32_t leaf = blockIdx.x * kBlock + threadIdx.x;
   bool keep = false;
-  if (leaf < nleaf && leaf != best[q]) keep = lb_keep(lb[(uint64_t)q * nleaf + leaf], bound[q]);
-  // wave-aggregated append
+  const uint32_t idx = q * nleaf + leaf;
+  if (leaf < nleaf) {
+    float v = lb[idx];
+    bool is_best = false;
+#pragma unroll
+    for (int m = 0; m < kBestLeaves; ++m) is_best |= leaf == best[q * kBestLeaves + m];
+    keep = !is_best && lb_keep(v, bound[q]);
+    keys[idx] = keep ? __float_as_uint(v) : 0xFFFFFFFFu;
+    vals[idx] = idx;
+  }
   uint64_t mask = __ballot(keep);
-  int lane = threadIdx.x & 63;
-  uint32_t wbase = 0;
-  int leader = __ffsll((long long)mask) - 1;
-  if (mask) {
-    if (lane == leader) wbase = atomicAdd(nwork, (uint32_t)__popcll(mask));
-    wbase = __shfl(wbase, leader);
-  }
-  if (keep) {
-    uint32_t slot = wbase + (uint32_t)__popcll(mask & ((1ull << lane) - 1));
-    if (slot < work_cap) work[slot] = ((uint64_t)q << 32) | leaf;
-    atomicAdd(&counters[kCtrSaxChecked], (unsigned long long)leaf_size[leaf]);
-  }
-  if (lane == leader && mask) atomicAdd(&counters[kCtrLeaves], (unsigned long long)__popcll(mask));
+  if ((threadIdx.x & 63) == 0 && mask) atomicAdd(nwork, (uint32_t)__popcll(mask));
 }
 
-// persistent (query, leaf) work queue: iSAX filter + exact refinement
+// persistent best-first (query, leaf) work queue: iSAX filter + exact
+// refinement.  An item whose leaf bound no longer beats the query's current
+// k-th distance is skipped when dequeued, and abandoned between tiles.
 __global__ void __launch_bounds__(kBlock) k_filter_refine(
     const float *__restrict__ X, const uint8_t *__restrict__ sax, uint32_t L,
     const float *__restrict__ Q, const float *__restrict__ paa, const uint32_t *__restrict__ kq,
     const uint64_t *__restrict__ leaf_pos, const uint32_t *__restrict__ leaf_size,
-    const uint32_t *__restrict__ ids, const uint64_t *__restrict__ work,
-    const uint32_t *__restrict__ nwork_p, uint32_t work_cap, uint32_t *__restrict__ work_next, uint32_t cap,
-    uint64_t *__restrict__ cand, uint32_t *__restrict__ cand_cnt, uint32_t *__restrict__ bound,
+    const uint32_t *__restrict__ ids, uint32_t nleaf, const float *__restrict__ lb,
+    const uint32_t *__restrict__ work, const uint32_t *__restrict__ nwork_p, uint32_t *__restrict__ work_next,
+    uint32_t cap, uint64_t *__restrict__ cand, uint32_t *__restrict__ cand_cnt, uint32_t *__restrict__ bound,
     unsigned long long *__restrict__ counters)
 {
   extern __shared__ float qs[];
@@ -461,33 +478,62 @@ __global__ void __launch_bounds__(kBlock) k_filter_refine(
   __shared__ uint32_t scan[kBlock];
   __shared__ float bp[256];
   __shared__ float spaa[kPaaSegments];
-  __shared__ uint32_t s_item, s_nsurv, s_bound;
+  __shared__ uint32_t s_item, s_nsurv, s_bound, s_stop;
+  __shared__ uint32_t s_chunk[kWorkChunk];
+  __shared__ int s_nchunk, s_ci;
   for (int i = threadIdx.x; i < 256; i += kBlock) bp[i] = c_bp[i];
-  uint32_t nwork = *nwork_p;
-  if (nwork > work_cap) nwork = work_cap;
+  const uint32_t nwork = *nwork_p;
   const float ratio = (float)(L / kPaaSegments);
   uint32_t nref = 0;
+  unsigned long long nleaves = 0, nchecked = 0;
+  if (threadIdx.x == 0) {
+    s_nchunk = 0;
+    s_ci = 0;
+  }
   for (;;) {
     __syncthreads();
-    if (threadIdx.x == 0) s_item = atomicAdd(work_next, 1u);
+    if (threadIdx.x == 0) {
+      // next live item: take chunks of the sorted list, skip pruned pairs
+      uint32_t found = 0xFFFFFFFFu;
+      for (;;) {
+        if (s_ci >= s_nchunk) {
+          uint32_t base = atomicAdd(work_next, (uint32_t)kWorkChunk);
+          if (base >= nwork) break;
+          int m = (int)min((uint32_t)kWorkChunk, nwork - base);
+          for (int j = 0; j < m; ++j) s_chunk[j] = work[base + j];
+          s_nchunk = m;
+          s_ci = 0;
+        }
+        uint32_t v = s_chunk[s_ci++];
+        uint32_t qq = v / nleaf;
+        if (lb_keep(lb[v], __atomic_load_n(&bound[qq], __ATOMIC_RELAXED))) {
+          found = v;
+          break;
+        }
+      }
+      s_item = found;
+    }
     __syncthreads();
     const uint32_t item = s_item;
-    if (item >= nwork) break;
-    const uint64_t w = work[item];
-    const uint32_t q = (uint32_t)(w >> 32), leaf = (uint32_t)w;
+    if (item == 0xFFFFFFFFu) break;
+    const uint32_t q = item / nleaf, leaf = item - q * nleaf;
+    const float item_lb = lb[item];
     const int k = (int)kq[q];
     load_query(Q, q, L, qs);
     if (threadIdx.x < kPaaSegments) spaa[threadIdx.x] = paa[q * kPaaSegments + threadIdx.x];
     topk_reset(tk);
     const uint64_t pos = leaf_pos[leaf];
     const uint32_t size = leaf_size[leaf];
+    ++nleaves;
     for (uint32_t t0 = 0; t0 < size; t0 += kBlock) {
       if (threadIdx.x == 0) {
         uint32_t gb = __atomic_load_n(&bound[q], __ATOMIC_RELAXED);
         s_bound = gb < tk.kth_bits ? gb : tk.kth_bits;
         s_nsurv = 0;
+        s_stop = lb_keep(item_lb, gb) ? 0u : 1u;
       }
       __syncthreads();
+      if (s_stop) break;
       const uint32_t bnd = s_bound;
       const uint32_t r = t0 + threadIdx.x;
       bool keep = false;
@@ -495,6 +541,7 @@ __global__ void __launch_bounds__(kBlock) k_filter_refine(
         uint4 word = *reinterpret_cast<const uint4 *>(sax + (pos + r) * kPaaSegments);
         keep = lb_keep(lb_sax_word(spaa, word, bp, ratio), bnd);
       }
+      nchecked += (threadIdx.x == 0) ? (unsigned long long)min((uint32_t)kBlock, size - t0) : 0ull;
       uint64_t mask = __ballot(keep);
       int lane = threadIdx.x & 63;
       int leader = __ffsll((long long)mask) - 1;
@@ -513,6 +560,10 @@ __global__ void __launch_bounds__(kBlock) k_filter_refine(
     flush_topk(tk, k, q, cap, cand, cand_cnt, bound);
   }
   if ((threadIdx.x & 63) == 0 && nref) atomicAdd(&counters[kCtrRefined], (unsigned long long)nref);
+  if (threadIdx.x == 0) {
+    if (nleaves) atomicAdd(&counters[kCtrLeaves], nleaves);
+    if (nchecked) atomicAdd(&counters[kCtrSaxChecked], nchecked);
+  }
 }
 
 // PSCAN: exact full scan, a group of up to kScanQ queries per block,
@@ -626,12 +677,142 @@ __global__ void __launch_bounds__(kBlock) k_select(uint32_t kmax, const uint32_t
   }
 }
 
-// reset per-query state: bound = +inf, no candidates
-__global__ void k_reset_queries(uint32_t nq, uint32_t *__restrict__ bound, uint32_t *__restrict__ cand_cnt)
+// BF16 path, last stage: one block per query merges the approximate stage's
+// candidates and the exact distances of the BF16 survivors into the final
+// top-k, refining survivors in two passes guided by a histogram of their
+// bounds: first the ~kRefineFirst smallest bounds (which fixes a k-th
+// distance close to the final one), then only the survivors whose bound is
+// within the current k-th distance.  Every survivor whose bound is within the
+// final k-th distance is refined, so the answer is exact when no survivor was
+// dropped for lack of room; otherwise the query is flagged (scnt = ~0) and
+// the caller answers it again, seeded with this k-th distance.
+constexpr int kSurvBins = 256;
+constexpr uint32_t kRefineFirst = 2048;
+
+__global__ void __launch_bounds__(kBlock) k_bf16_refine(
+    const float *__restrict__ X, uint32_t L, const float *__restrict__ Q, const uint32_t *__restrict__ kq,
+    uint32_t kmax, const uint32_t *__restrict__ ids, uint32_t cap, const uint64_t *__restrict__ cand,
+    const uint32_t *__restrict__ cand_cnt, uint32_t scap, const uint64_t *__restrict__ surv,
+    uint32_t *__restrict__ scnt, const float *__restrict__ thr0, const uint64_t *__restrict__ idmap64,
+    uint64_t *__restrict__ out_ids, float *__restrict__ out_d2, uint32_t *__restrict__ overflow,
+    unsigned long long *__restrict__ counters)
+{
+  extern __shared__ float qs[];
+  __shared__ TopK tk;
+  __shared__ RefineShared rs;
+  __shared__ uint64_t rows[kBlock];
+  __shared__ uint32_t scan[kBlock];
+  __shared__ uint32_t hist[kSurvBins];
+  __shared__ uint32_t s_nsel, s_bin;
+  const uint32_t q = blockIdx.x;
+  const int k = (int)kq[q];
+  load_query(Q, q, L, qs);
+  topk_reset(tk);
+  for (int i = threadIdx.x; i < kSurvBins; i += kBlock) hist[i] = 0;
+  __syncthreads();
+  uint32_t nc = cand_cnt[q];
+  if (nc > cap) nc = cap;
+  for (uint32_t t0 = 0; t0 < nc; t0 += kBlock) {
+    uint32_t i = t0 + threadIdx.x;
+    if (i < nc) {
+      uint64_t key = cand[(uint64_t)q * cap + i];
+      if ((uint32_t)(key >> 32) <= tk.kth_bits) topk_stage(tk, key);
+    }
+    __syncthreads();
+    topk_merge(tk, scan, k);
+  }
+  uint32_t ns = scnt[q];
+  const bool dropped = ns > scap;
+  if (dropped) ns = scap;
+  const uint64_t *sv = surv + (uint64_t)q * scap;
+  const float t0v = thr0[q];
+  const float scale = t0v < FLT_MAX && t0v > 0.0f ? (float)kSurvBins / t0v : 0.0f;
+  // histogram of the survivors' bounds over [0, threshold]
+  for (uint32_t i = threadIdx.x; i < ns; i += kBlock) {
+    const float lbv = __uint_as_float((uint32_t)(sv[i] >> 32));
+    const int bin = min(kSurvBins - 1, (int)(lbv * scale));
+    atomicAdd(&hist[bin], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t cum = 0;
+    int b = 0;
+    for (; b < kSurvBins - 1; ++b) {
+      cum += hist[b];
+      if (cum >= kRefineFirst) break;
+    }
+    s_bin = (uint32_t)b;
+  }
+  __syncthreads();
+  const int first_bin = (int)s_bin;
+  uint32_t nref = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (uint32_t t0 = 0; t0 < ns; t0 += kBlock) {
+      if (threadIdx.x == 0) s_nsel = 0;
+      __syncthreads();
+      const uint32_t bnd = tk.kth_bits;
+      const uint32_t i = t0 + threadIdx.x;
+      bool take = false;
+      uint64_t e = 0;
+      if (i < ns) {
+        e = sv[i];
+        const float lbv = __uint_as_float((uint32_t)(e >> 32));
+        const int bin = min(kSurvBins - 1, (int)(lbv * scale));
+        const bool in_first = bin <= first_bin;
+        take = (pass == 0 ? in_first : !in_first) && lb_keep(lbv, bnd);
+      }
+      const uint64_t m = __ballot(take);
+      const int lane = threadIdx.x & 63;
+      const int leader = __ffsll((long long)m) - 1;
+      uint32_t wb = 0;
+      if (m) {
+        if (lane == leader) wb = atomicAdd(&s_nsel, (uint32_t)__popcll(m));
+        wb = __shfl(wb, leader);
+      }
+      if (take) rows[wb + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)e;
+      __syncthreads();
+      refine_rows(X, L, qs, rows, (int)s_nsel, ids, bnd, tk, rs, nref);
+      __syncthreads();
+      topk_merge(tk, scan, k);
+    }
+  }
+  if ((threadIdx.x & 63) == 0 && nref) atomicAdd(&counters[kCtrRefined], (unsigned long long)nref);
+  if (threadIdx.x == 0 && dropped) {
+    atomicAdd(overflow, 1u);
+    scnt[q] = 0xFFFFFFFFu;  // tells the caller to answer this query again
+  }
+  for (uint32_t r = threadIdx.x; r < kmax; r += kBlock) {
+    uint64_t id = ~0ull;
+    float d = FLT_MAX;
+    if ((int)r < tk.cnt) {
+      uint64_t key = tk.keys[r];
+      d = __uint_as_float((uint32_t)(key >> 32));
+      uint32_t id32 = (uint32_t)key;
+      id = idmap64 ? idmap64[id32] : (uint64_t)id32;
+    }
+    out_ids[(uint64_t)q * kmax + r] = id;
+    out_d2[(uint64_t)q * kmax + r] = d;
+  }
+}
+
+void launch_bf16_refine(hipStream_t st, uint32_t nq, const float *X, uint32_t L, const float *Q, const uint32_t *kq,
+                        uint32_t kmax, const uint32_t *ids, uint32_t cap, const uint64_t *cand,
+                        const uint32_t *cand_cnt, uint32_t scap, const uint64_t *surv, uint32_t *scnt,
+                        const float *thr0, const uint64_t *idmap64, uint64_t *out_ids, float *out_d2,
+                        uint32_t *overflow, unsigned long long *counters)
+{
+  hipLaunchKernelGGL(k_bf16_refine, dim3(nq), dim3(kBlock), L * sizeof(float), st, X, L, Q, kq, kmax, ids, cap, cand,
+                     cand_cnt, scap, surv, scnt, thr0, idmap64, out_ids, out_d2, overflow, counters);
+}
+
+// reset per-query state: bound = the seed (an upper bound on the k-th
+// distance known from an earlier pass) or +inf, no candidates
+__global__ void k_reset_queries(uint32_t nq, uint32_t *__restrict__ bound, uint32_t *__restrict__ cand_cnt,
+                                const uint32_t *__restrict__ seed)
 {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nq) {
-    bound[i] = kInfBits;
+    bound[i] = seed ? seed[i] : kInfBits;
     cand_cnt[i] = 0;
   }
 }
@@ -647,9 +828,9 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t n, uint32_t v)
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-void launch_reset(hipStream_t st, uint32_t nq, uint32_t *bound, uint32_t *cand_cnt)
+void launch_reset(hipStream_t st, uint32_t nq, uint32_t *bound, uint32_t *cand_cnt, const uint32_t *seed)
 {
-  hipLaunchKernelGGL(k_reset_queries, dim3((nq + 255) / 256), dim3(256), 0, st, nq, bound, cand_cnt);
+  hipLaunchKernelGGL(k_reset_queries, dim3((nq + 255) / 256), dim3(256), 0, st, nq, bound, cand_cnt, seed);
 }
 
 void launch_fill(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v)
@@ -682,28 +863,40 @@ void launch_scan_best(hipStream_t st, uint32_t nq, const float *X, uint32_t L, c
                       const uint32_t *leaf_size, const uint32_t *ids, uint32_t cap, uint64_t *cand,
                       uint32_t *cand_cnt, uint32_t *bound, unsigned long long *counters)
 {
-  hipLaunchKernelGGL(k_scan_best, dim3(nq), dim3(kBlock), L * sizeof(float), st, X, L, Q, kq, best,
+  hipLaunchKernelGGL(k_scan_best, dim3(nq, kBestLeaves), dim3(kBlock), L * sizeof(float), st, X, L, Q, kq, best,
                      leaf_pos, leaf_size, ids, cap, cand, cand_cnt, bound, counters);
 }
 
 void launch_collect(hipStream_t st, uint32_t nq, uint32_t nleaf, const float *lb, const uint32_t *best,
-                    const uint32_t *bound, const uint32_t *leaf_size, uint64_t *work, uint32_t work_cap,
-                    uint32_t *nwork, unsigned long long *counters)
+                    const uint32_t *bound, uint32_t *keys, uint32_t *vals, uint32_t *nwork)
 {
-  hipLaunchKernelGGL(k_collect, dim3((nleaf + kBlock - 1) / kBlock, nq), dim3(kBlock), 0, st, nq, nleaf,
-                     lb, best, bound, leaf_size, work, work_cap, nwork, counters);
+  hipLaunchKernelGGL(k_collect, dim3((nleaf + kBlock - 1) / kBlock, nq), dim3(kBlock), 0, st, nq, nleaf, lb, best,
+                     bound, keys, vals, nwork);
+}
+
+size_t sort_work_temp_bytes(uint32_t n)
+{
+  size_t bytes = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                           (const uint32_t *)nullptr, (uint32_t *)nullptr, (int)n, 0, 32);
+  return bytes;
+}
+
+hipError_t launch_sort_work(hipStream_t st, void *tmp, size_t tmp_bytes, const uint32_t *keys, uint32_t *keys_out,
+                            const uint32_t *vals, uint32_t *vals_out, uint32_t n)
+{
+  return hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys_out, vals, vals_out, (int)n, 0, 32, st);
 }
 
 void launch_filter_refine(hipStream_t st, uint32_t grid, const float *X, const uint8_t *sax, uint32_t L,
                           const float *Q, const float *paa, const uint32_t *kq, const uint64_t *leaf_pos,
-                          const uint32_t *leaf_size, const uint32_t *ids, const uint64_t *work,
-                          const uint32_t *nwork, uint32_t work_cap, uint32_t *work_next, uint32_t cap,
-                          uint64_t *cand, uint32_t *cand_cnt, uint32_t *bound,
-                          unsigned long long *counters)
+                          const uint32_t *leaf_size, const uint32_t *ids, uint32_t nleaf, const float *lb,
+                          const uint32_t *work, const uint32_t *nwork, uint32_t *work_next, uint32_t cap,
+                          uint64_t *cand, uint32_t *cand_cnt, uint32_t *bound, unsigned long long *counters)
 {
-  hipLaunchKernelGGL(k_filter_refine, dim3(grid), dim3(kBlock), L * sizeof(float), st, X, sax, L, Q, paa,
-                     kq, leaf_pos, leaf_size, ids, work, nwork, work_cap, work_next, cap, cand, cand_cnt,
-                     bound, counters);
+  hipLaunchKernelGGL(k_filter_refine, dim3(grid), dim3(kBlock), L * sizeof(float), st, X, sax, L, Q, paa, kq,
+                     leaf_pos, leaf_size, ids, nleaf, lb, work, nwork, work_next, cap, cand, cand_cnt, bound,
+                     counters);
 }
 
 void launch_scan(hipStream_t st, uint32_t grid_x, const float *X, uint64_t n, uint32_t L, const float *Q,

@@ -3,19 +3,7 @@
 
 #include "hkg_engine.h"
 
-namespace hkg {
-hkg_status Index::make_bf16()
-{
-  set_error("BF16 filter not available in this build");
-  return HKG_EINVAL;
-}
-hkg_status Index::run_bf16_batch(const float *, const uint32_t *, uint32_t, uint32_t, hipStream_t, QueryBuffers **,
-                                 Counters *, Prof *)
-{
-  set_error("BF16 filter not available in this build");
-  return HKG_EINVAL;
-}
-}  // namespace hkg
+
 
 extern "C" {
 void hkg_default_server_opts(hkg_server_opts *so)

@@ -36,7 +36,8 @@ class Options(C.Structure):
 
 
 class QueryOpts(C.Structure):
-    _fields_ = [("path", C.c_int), ("batch", C.c_uint32), ("profile", C.c_uint32), ("reserved", C.c_uint32 * 5)]
+    _fields_ = [("path", C.c_int), ("batch", C.c_uint32), ("profile", C.c_uint32), ("cand_cap", C.c_uint32),
+                ("reserved", C.c_uint32 * 4)]
 
 
 STAGES = ["prep", "eapca", "select_leaf", "approx", "collect", "refine", "topk", "scan"]
@@ -46,7 +47,8 @@ class QueryStats(C.Structure):
     _fields_ = [("queries", C.c_uint64), ("leaves_total", C.c_uint64), ("leaves_after_eapca", C.c_uint64),
                 ("series_total", C.c_uint64), ("series_lb_checked", C.c_uint64),
                 ("series_refined", C.c_uint64), ("bytes_algorithmic", C.c_uint64), ("device_ms", C.c_double),
-                ("overflow_retries", C.c_uint32), ("launches", C.c_uint32), ("stage_ms", C.c_double * 8),
+                ("overflow_retries", C.c_uint32), ("launches", C.c_uint32), ("bf16_pairs", C.c_uint64),
+                ("stage_ms", C.c_double * 8),
                 ("stage_bytes", C.c_uint64 * 8), ("stage_launches", C.c_uint64 * 8)]
 
     def as_dict(self) -> dict:
@@ -270,7 +272,8 @@ class HerculesIndex:
             pass
 
     # ---- queries -------------------------------------------------------------
-    def knn(self, Q: np.ndarray, k, path: str = "auto", batch: int = 0, profile: bool = False) -> KnnResult:
+    def knn(self, Q: np.ndarray, k, path: str = "auto", batch: int = 0, profile: bool = False,
+            cand_cap: int = 0) -> KnnResult:
         """Exact k-NN of each row of Q.  ``k`` is an int or a per-query array."""
         Q = np.ascontiguousarray(Q, np.float32)
         if Q.ndim == 1:
@@ -284,7 +287,7 @@ class HerculesIndex:
             kmax = int(kv.max()) if nq else 1
         ids = np.empty((nq, kmax), np.uint64)
         d2 = np.empty((nq, kmax), np.float32)
-        qo = QueryOpts(PATHS[path], batch, int(profile))
+        qo = QueryOpts(PATHS[path], batch, int(profile), cand_cap)
         st = QueryStats()
         _check(lib().hkg_knn(self._h, _ptr(Q), nq, _ptr(kv) if kv is not None else None, kmax, C.byref(qo),
                              _ptr(ids), _ptr(d2), C.byref(st)))

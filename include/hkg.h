@@ -79,7 +79,9 @@ typedef struct hkg_query_opts {
   hkg_path path;
   uint32_t batch;      /* queries per batch (0 = library default)             */
   uint32_t profile;    /* 1: time every stage with events (no graph replay)   */
-  uint32_t reserved[5];
+  uint32_t cand_cap;   /* candidate buffer entries per query (0 = default);
+                          when a query overflows it, it is answered again    */
+  uint32_t reserved[4];
 } hkg_query_opts;
 
 /* stages reported in hkg_query_stats.stage_ms / stage_bytes */
@@ -106,6 +108,7 @@ typedef struct hkg_query_stats {
   double device_ms;               /* device time of the query kernels          */
   uint32_t overflow_retries;      /* candidate-buffer overflows re-run         */
   uint32_t launches;              /* kernel launch sets (batches)              */
+  uint64_t bf16_pairs;            /* (query, series) pairs the BF16 bound saw  */
   double stage_ms[8];             /* with profile=1: summed stage times        */
   uint64_t stage_bytes[8];        /* algorithmic HBM bytes of each stage       */
   uint64_t stage_launches[8];     /* kernel launches of each stage             */

@@ -154,3 +154,77 @@ def test_mixed_k_and_determinism(hkg, oracle, small_data, ref_index):
         assert np.array_equal(r1.d2[i, :k].view(np.uint32), d2[i, :k].view(np.uint32)), i
         assert np.array_equal(r1.ids[i, :k], ids[i, :k]), i
         assert (r1.d2[i, k:] == np.finfo(np.float32).max).all()
+
+
+def test_overflow_rerun_stays_exact(hkg, oracle, small_data, ref_index):
+    """A tiny candidate buffer forces overflows; the seeded re-runs (and the
+    full-scan last resort) must still give the exact answers."""
+    X, Q = small_data
+    idx, _ = ref_index
+    _say("overflow re-run")
+    ids, d2 = oracle.knn(X, Q, 50)
+    with hkg.HerculesIndex.open(idx) as ix:
+        ix.set_ids(row_map(idx, X))
+        r = ix.knn(Q, 50, path="batched", cand_cap=64)
+    assert r.stats["overflow_retries"] > 0
+    assert_same(r.ids, r.d2, ids, d2, "overflow")
+
+
+def test_profile_stages(hkg, small_data, ref_index):
+    X, Q = small_data
+    idx, _ = ref_index
+    with hkg.HerculesIndex.open(idx) as ix:
+        r = ix.knn(Q, 10, path="batched", profile=True)
+        s = ix.knn(Q, 10, path="scan", profile=True)
+    assert r.stats["stage_ms"]["refine"] > 0 and r.stats["stage_launches"]["refine"] == 1
+    assert r.stats["stage_bytes"]["refine"] > 0
+    assert s.stats["stage_bytes"]["scan"] == ((Q.shape[0] + 3) // 4) * X.nbytes
+
+
+# ---- BF16 certified-bound path ----------------------------------------------------
+@pytest.mark.parametrize("k", [1, 10, 100])
+def test_bf16_path_matches_oracle(hkg, oracle, small_data, ref_index, k):
+    X, Q = small_data
+    idx, _ = ref_index
+    _say(f"bf16 k={k}")
+    ids, d2 = oracle.knn(X, Q, k)
+    with hkg.HerculesIndex.open(idx, keep_bf16=True) as ix:
+        ix.set_ids(row_map(idx, X))
+        r = ix.knn(Q, k, path="bf16")
+        s = ix.knn(Q, k, path="batched")
+    assert_same(r.ids, r.d2, ids, d2, f"bf16 k={k}")
+    # pairs of the row tiles the filter visited (128-row tiles; skipped tiles excluded)
+    assert 0 < r.stats["bf16_pairs"] <= Q.shape[0] * ((X.shape[0] + 127) // 128) * 128
+    # the certified bound prunes far more than the iSAX bound
+    assert r.stats["series_refined"] <= s.stats["series_refined"] + Q.shape[0] * k
+
+
+def test_bf16_embeddings_l96(hkg, oracle, tmp_path):
+    """L = 96 embedding-like series (N(0,1) -> L2 -> z-normalised)."""
+    if not rr.available():
+        pytest.skip("reference binaries not built")
+    _say("bf16 L=96")
+    X = oracle.series(1, 21, 0, 20000, 96)
+    Qn, _ = oracle.noisy_queries(1, 21, 20000, 23, np.array([0.05, 0.1, 0.2, 0.4] * 5, np.float32), 96)
+    Q = np.ascontiguousarray(np.concatenate([oracle.series(1, 22, 0, 20, 96), Qn]))
+    rr.write_f32(str(tmp_path / "d.bin"), X)
+    rr.build_index(str(tmp_path / "d.bin"), X.shape[0], 96, 400, str(tmp_path / "idx"), threads=8)
+    ids, d2 = oracle.knn(X, Q, 10)
+    with hkg.HerculesIndex.open(str(tmp_path / "idx")) as ix:
+        ix.set_ids(row_map(str(tmp_path / "idx"), X))
+        for path in ("bf16", "batched", "index"):
+            r = ix.knn(Q, 10, path=path)
+            assert_same(r.ids, r.d2, ids, d2, f"L96 {path}")
+
+
+def test_bf16_overflow_rerun_stays_exact(hkg, oracle, small_data, ref_index):
+    """A tiny survivor list forces the seeded BF16 re-runs."""
+    X, Q = small_data
+    idx, _ = ref_index
+    _say("bf16 overflow re-run")
+    ids, d2 = oracle.knn(X, Q, 100)
+    with hkg.HerculesIndex.open(idx) as ix:
+        ix.set_ids(row_map(idx, X))
+        r = ix.knn(Q, 100, path="bf16", cand_cap=128)
+    assert r.stats["overflow_retries"] > 0
+    assert_same(r.ids, r.d2, ids, d2, "bf16 overflow")
