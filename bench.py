@@ -48,6 +48,8 @@ def parse() -> argparse.Namespace:
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample", type=int, default=1_000_000, help="rows of the CPU baseline sample")
     p.add_argument("--parity-queries", type=int, default=8)
+    p.add_argument("--row-probes", type=int, default=32, help="dataset rows looked up exactly after the build")
+    p.add_argument("--bf16-bound-pass", type=int, default=0, choices=[0, 1, 2])
     return p.parse_args()
 
 
@@ -123,7 +125,8 @@ def main() -> int:
         kv = hkg.DeviceArray.__new__(hkg.DeviceArray)
         kv.ptr, kv.nbytes = dk.ptr + step * B * 4, B * 4
         try:
-            return ix.knn_device(qv, B, kv, kmax, dids, dd2, path=args.path, batch=B, profile=profile)
+            return ix.knn_device(qv, B, kv, kmax, dids, dd2, path=args.path, batch=B, profile=profile,
+                                 bf16_bound_pass=args.bf16_bound_pass)
         finally:
             qv.ptr = None
             kv.ptr = None
@@ -162,6 +165,9 @@ def main() -> int:
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     kernel_names = {"refine": "k_filter_refine", "approx": "k_scan_best", "eapca": "k_eapca", "scan": "k_scan",
                     "topk": "k_select", "collect": "k_collect", "prep": "k_prep", "select_leaf": "k_best_leaf"}
+    if args.path == "bf16":
+        kernel_names.update({"refine": "k_bf16_filter", "bound": "k_bf16_filter (bound pass)",
+                             "topk": "k_bf16_refine"})
 
     # parity at full size: index path == full scan on a sample of the queries
     npar = min(args.parity_queries, B)
@@ -175,6 +181,15 @@ def main() -> int:
         zero = [int(i) for i in range(npar) if sig[i] == 0.0]
         par = {"index_vs_full_scan_identical": same, "queries_checked": npar}
         log(f"parity index vs scan on {npar} queries: {same}")
+    if args.row_probes:
+        # every probed dataset row is in the index: found at distance 0 under its own id
+        rows = np.unique(np.concatenate([np.linspace(0, N - 1, args.row_probes).astype(np.int64),
+                                         np.random.default_rng(3).integers(0, N, args.row_probes)]))
+        P = np.concatenate([hkg.gen_series(0, 1, int(r), 1, L) for r in rows])
+        rr = ix.knn(P, 1, path=args.path)
+        found = int(np.sum((rr.d2[:, 0] == 0.0) & (rr.ids[:, 0] == rows.astype(np.uint64))))
+        par["rows_found_exactly"] = f"{found}/{len(rows)}"
+        log(f"dataset rows found at distance 0: {found}/{len(rows)}")
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args, Qh, kq)
     if cpu:

@@ -38,6 +38,7 @@ constexpr int kQPerBlock = 4 * kQPerWave;  // 256 queries per block
 constexpr int kRowTile = 128;              // series per LDS tile
 constexpr float kEvalSlack = 1.0e-3f;      // FP32 evaluation of the bound
 constexpr uint32_t kBf16SurvivorCap = 524288;  // survivors kept per query (8 B each)
+constexpr float kHistLo = 0.6f;            // histogram range: [0.6 bound, threshold]
 
 __device__ __forceinline__ uint16_t to_bf16_bits(float f)
 {
@@ -59,6 +60,20 @@ __device__ __forceinline__ float ru(double v)
   float f = (float)v;
   if ((double)f < v) f = nextafterf(f, FLT_MAX);
   return f;
+}
+
+// Threshold histogram: bin 0 counts upper bounds below lo, bins 1.. split
+// [lo, hi] (hi = the filter threshold) into equal widths.
+struct HistRange {
+  float lo, w, inv;
+};
+__device__ __forceinline__ HistRange hist_range(uint32_t bound_bits, float hi)
+{
+  HistRange r;
+  r.lo = kHistLo * __uint_as_float(bound_bits);
+  r.w = (hi - r.lo) / (float)(kBf16Bins - 1);
+  r.inv = r.w > 0.0f ? 1.0f / r.w : 0.0f;
+  return r;
 }
 
 // rows -> BF16 copy + {|x|^2 (down), |x| (up), |x - x~| (up), |x~| (up)}
@@ -113,7 +128,7 @@ __global__ void k_query_bf16(uint32_t nq, uint32_t L, const float *__restrict__ 
 // batch sorted by first bound, so similar queries share a block) and visits
 // only the row tiles listed for it: tiles that overlap a leaf whose EAPCA
 // bound beats the k-th distance bound of at least one of its queries.
-template <int KS>
+template <int KS, bool HIST>
 __global__ void __launch_bounds__(256) k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb,
                                                      const float4 *__restrict__ xst,
                                                      const uint16_t *__restrict__ Qb,
@@ -123,7 +138,8 @@ __global__ void __launch_bounds__(256) k_bf16_filter(uint64_t n, uint32_t nq, co
                                                      const uint32_t *__restrict__ tiles, uint32_t tile_stride,
                                                      const uint32_t *__restrict__ ntiles_g, uint32_t scap,
                                                      uint64_t *__restrict__ surv, uint32_t *__restrict__ scnt,
-                                                     const float *__restrict__ thr0, float gamma)
+                                                     const float *__restrict__ thr0, float gamma,
+                                                     uint32_t *__restrict__ hist)
 {
   constexpr int L = KS * 16;
   constexpr int kStride = KS * 2 + 1;  // 16-byte vectors per LDS row (one of padding)
@@ -131,6 +147,8 @@ __global__ void __launch_bounds__(256) k_bf16_filter(uint64_t n, uint32_t nq, co
   __shared__ float s_thr[kQPerBlock], s_qn2[kQPerBlock], s_qe[kQPerBlock], s_qb[kQPerBlock];
   __shared__ uint8_t s_full[kQPerBlock];  // the query's survivor list is full
   __shared__ uint32_t s_q[kQPerBlock];     // batch index of every slot
+  __shared__ float s_lo[HIST ? kQPerBlock : 1], s_inv[HIST ? kQPerBlock : 1];
+  __shared__ uint32_t s_hist[HIST ? kQPerBlock * kBf16Bins : 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t qblk = blockIdx.y * kQPerBlock;
   // per-query constants
@@ -145,6 +163,16 @@ __global__ void __launch_bounds__(256) k_bf16_filter(uint64_t n, uint32_t nq, co
       s_qn2[i] = s.x;
       s_qe[i] = s.y;
       s_qb[i] = s.z;
+      if (HIST) {
+        const uint32_t bb = bound[q];
+        if (bb >= kInfBits) {
+          s_thr[i] = -FLT_MAX;  // no finite bound: nothing to histogram
+        } else {
+          const HistRange r = hist_range(bb, thr0[q]);
+          s_lo[i] = r.lo;
+          s_inv[i] = r.inv;
+        }
+      }
     } else {
       s_thr[i] = -FLT_MAX;  // padding slot: never keeps (and is also excluded below)
       s_full[i] = 1;
@@ -153,6 +181,8 @@ __global__ void __launch_bounds__(256) k_bf16_filter(uint64_t n, uint32_t nq, co
       s_qb[i] = 0.0f;
     }
   }
+  if (HIST)
+    for (int i = threadIdx.x; i < kQPerBlock * kBf16Bins; i += 256) s_hist[i] = 0;
   __syncthreads();
   // query operands in registers: 2 tiles x KS steps
   bf16x8 qa[2][KS];
@@ -199,6 +229,16 @@ __global__ void __launch_bounds__(256) k_bf16_filter(uint64_t n, uint32_t nq, co
           const int ql = wave * kQPerWave + t * 32 + qi;
           const float s = t == 0 ? acc0[r] : acc1[r];
           const float slack = 2.0f * (s_qe[ql] * xsv.y + s_qb[ql] * xsv.z + gamma * s_qb[ql] * xsv.w) + kEvalSlack;
+          if (HIST) {
+            // upper bound of d2: count it in the query's histogram
+            const float ub = (s_qn2[ql] + xsv.x) - 2.0f * s + slack;
+            if (valid && ub <= s_thr[ql]) {
+              const float lo = s_lo[ql];
+              const int j = ub < lo ? 0 : min(kBf16Bins - 1, 1 + (int)fminf((ub - lo) * s_inv[ql], (float)kBf16Bins));
+              atomicAdd(&s_hist[ql * kBf16Bins + j], 1u);
+            }
+            continue;
+          }
           const float lb = (s_qn2[ql] + xsv.x) - 2.0f * s - slack;
           const uint32_t q = s_q[ql];
           const bool keep = valid && q < nq && lb <= s_thr[ql] && !s_full[ql];
@@ -224,6 +264,39 @@ __global__ void __launch_bounds__(256) k_bf16_filter(uint64_t n, uint32_t nq, co
     }
     __syncthreads();
   }
+  if (HIST) {
+    for (int i = threadIdx.x; i < kQPerBlock * kBf16Bins; i += 256) {
+      const uint32_t v = s_hist[i];
+      const uint32_t q = s_q[i / kBf16Bins];
+      if (v && q < nq) atomicAdd(&hist[(uint64_t)q * kBf16Bins + i % kBf16Bins], v);
+    }
+  }
+}
+
+// New k-th distance bound of every query from its histogram: the first bin
+// where the count reaches k.  Every series counted up to bin j has an upper
+// bound below lo + (j + 1) w (one bin of margin for the rounding of the bin
+// index), so at least k series lie within that distance.  The bound is
+// counted over series, not distinct distances: the refine stage certifies
+// the answer against it and flags the query otherwise.
+__global__ void k_bf16_tau(uint32_t nq, const uint32_t *__restrict__ kq, const uint32_t *__restrict__ hist,
+                           const float *__restrict__ thr0, uint32_t *__restrict__ bound)
+{
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const uint32_t bb = bound[q];
+  if (bb >= kInfBits) return;
+  const HistRange r = hist_range(bb, thr0[q]);
+  const uint32_t k = kq[q];
+  uint32_t cum = 0;
+  for (int j = 0; j < kBf16Bins; ++j) {
+    cum += hist[(uint64_t)q * kBf16Bins + j];
+    if (cum >= k) {
+      const float t = ru((double)r.lo + (double)(j + 1) * (double)r.w);
+      if (t < __uint_as_float(bb)) bound[q] = __float_as_uint(t);
+      return;
+    }
+  }
 }
 
 }  // namespace
@@ -240,14 +313,18 @@ static uint32_t bf16_grid_x(uint32_t groups)
 void launch_bf16_filter(hipStream_t st, uint32_t L, uint64_t n, uint32_t nq, const uint16_t *Xb, const float4 *xst,
                         const uint16_t *Qb, const float4 *qst, const uint32_t *bound, const uint32_t *qperm,
                         const uint32_t *tiles, uint32_t tile_stride, const uint32_t *ntiles_g, uint32_t scap,
-                        uint64_t *surv, uint32_t *scnt, const float *thr0)
+                        uint64_t *surv, uint32_t *scnt, const float *thr0, uint32_t *hist)
 {
   const uint32_t groups = (nq + kQPerBlock - 1) / kQPerBlock;
   const float gamma = 2.0f * (float)L * 5.9604645e-08f;
   dim3 grid(bf16_grid_x(groups), groups);
-#define HKG_BF16_LAUNCH(KS)                                                                                   \
-  hipLaunchKernelGGL(k_bf16_filter<KS>, grid, dim3(256), 0, st, n, nq, Xb, xst, Qb, qst, bound, qperm, tiles, \
-                     tile_stride, ntiles_g, scap, surv, scnt, thr0, gamma)
+#define HKG_BF16_LAUNCH(KS)                                                                                  \
+  if (hist)                                                                                                  \
+    hipLaunchKernelGGL((k_bf16_filter<KS, true>), grid, dim3(256), 0, st, n, nq, Xb, xst, Qb, qst, bound,   \
+                       qperm, tiles, tile_stride, ntiles_g, scap, surv, scnt, thr0, gamma, hist);            \
+  else                                                                                                       \
+    hipLaunchKernelGGL((k_bf16_filter<KS, false>), grid, dim3(256), 0, st, n, nq, Xb, xst, Qb, qst, bound,  \
+                       qperm, tiles, tile_stride, ntiles_g, scap, surv, scnt, thr0, gamma, hist)
   switch (L) {
   case 96: HKG_BF16_LAUNCH(6); break;
   case 128: HKG_BF16_LAUNCH(8); break;
@@ -377,7 +454,8 @@ hkg_status Index::make_bf16()
 }
 
 hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax, hipStream_t st,
-                                 QueryBuffers **used, Counters *ctr, Prof *pf, const uint32_t *seed_host)
+                                 QueryBuffers **used, Counters *ctr, Prof *pf, const uint32_t *seed_host,
+                                 bool bound_pass)
 {
   if (!dev.has_tree) {
     set_error("the BF16 path needs an index (its first bound comes from the best leaf)");
@@ -405,7 +483,8 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
         hipMalloc((void **)&b->need, (size_t)groups_max * nleaf) != hipSuccess ||
         hipMalloc((void **)&b->tiles, (size_t)groups_max * dev.ntiles * 4) != hipSuccess ||
         hipMalloc((void **)&b->ntiles_g, (size_t)groups_max * 4) != hipSuccess ||
-        hipMalloc((void **)&b->thr, (size_t)b->B * 4) != hipSuccess) {
+        hipMalloc((void **)&b->thr, (size_t)b->B * 4) != hipSuccess ||
+        hipMalloc((void **)&b->hist, (size_t)b->B * kBf16Bins * 4) != hipSuccess) {
       (void)hipGetLastError();
       set_error("no device memory for the BF16 path scratch");
       return HKG_ENOMEM;
@@ -436,19 +515,40 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
   hipLaunchKernelGGL(k_iota_bound, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->bound, skeys, sidx);
   HKG_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(b->bf_sort_tmp, b->bf_sort_bytes, skeys, b->qperm + 3 * b->B, sidx,
                                                  b->qperm, (int)nq, 0, 32, st));
-  launch_fill(st, b->ntiles_g, groups, 0);
-  hipLaunchKernelGGL(k_group_leaf_need, dim3((nleaf + 255) / 256, groups), dim3(256), 0, st, nq, nleaf, b->lb,
-                     b->bound, b->qperm, b->need);
-  hipLaunchKernelGGL(k_group_tiles, dim3((dev.ntiles + 255) / 256, groups), dim3(256), 0, st, dev.ntiles, nleaf,
-                     dev.tile_leaf0, dev.tile_leaf1, b->need, b->tiles, b->ntiles_g);
-  if (pf) pf->mark(st, HKG_STAGE_COLLECT);
+  // row tiles each block group needs under the current bounds, and the
+  // filter threshold of every query
   float *thr0 = reinterpret_cast<float *>(b->thr);
-  hipLaunchKernelGGL(k_bf16_thr_init, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->bound, thr0);
+  auto plan_tiles = [&]() {
+    launch_fill(st, b->ntiles_g, groups, 0);
+    hipLaunchKernelGGL(k_group_leaf_need, dim3((nleaf + 255) / 256, groups), dim3(256), 0, st, nq, nleaf, b->lb,
+                       b->bound, b->qperm, b->need);
+    hipLaunchKernelGGL(k_group_tiles, dim3((dev.ntiles + 255) / 256, groups), dim3(256), 0, st, dev.ntiles, nleaf,
+                       dev.tile_leaf0, dev.tile_leaf1, b->need, b->tiles, b->ntiles_g);
+    hipLaunchKernelGGL(k_bf16_thr_init, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->bound, thr0);
+  };
+  plan_tiles();
+  if (pf) pf->mark(st, HKG_STAGE_COLLECT);
+  std::vector<uint32_t> nt_pass1;
+  if (bound_pass) {
+    // pass 1: histogram of the BF16 upper bounds -> a k-th distance bound
+    // within about one bin of the k-th smallest upper bound
+    launch_fill(st, b->hist, nq * kBf16Bins, 0);
+    launch_bf16_filter(st, L, dev.n, nq, dev.Xb, dev.xst, b->Qb, b->qst, b->bound, b->qperm, b->tiles, dev.ntiles,
+                       b->ntiles_g, b->bf_cap, b->bf_surv, b->bf_cnt, thr0, b->hist);
+    if (ctr) {
+      nt_pass1.resize(groups);
+      HKG_HIP_TRY(hipMemcpyAsync(nt_pass1.data(), b->ntiles_g, groups * 4, hipMemcpyDeviceToHost, st));
+    }
+    hipLaunchKernelGGL(k_bf16_tau, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->kq, b->hist, thr0, b->bound);
+    plan_tiles();
+    if (pf) pf->mark(st, HKG_STAGE_BOUND);
+  }
+  // pass 2: collect the survivors of the certified lower bound
   launch_bf16_filter(st, L, dev.n, nq, dev.Xb, dev.xst, b->Qb, b->qst, b->bound, b->qperm, b->tiles, dev.ntiles,
-                     b->ntiles_g, b->bf_cap, b->bf_surv, b->bf_cnt, thr0);
+                     b->ntiles_g, b->bf_cap, b->bf_surv, b->bf_cnt, thr0, nullptr);
   if (pf) pf->mark(st, HKG_STAGE_REFINE);
   launch_bf16_refine(st, nq, dev.X, L, b->Q, b->kq, b->kmax, dev.ids, b->cap, b->cand, b->cand_cnt, b->bf_cap,
-                     b->bf_surv, b->bf_cnt, thr0, idmap64, b->out_ids, b->out_d2, overflow, b->counters);
+                     b->bf_surv, b->bf_cnt, thr0, b->bound, idmap64, b->out_ids, b->out_d2, overflow, b->counters);
   if (pf) pf->mark(st, HKG_STAGE_TOPK);
   HKG_HIP_TRY(hipGetLastError());
   if (ctr) {
@@ -457,11 +557,13 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
     std::vector<uint32_t> nt(groups);
     HKG_HIP_TRY(hipMemcpyAsync(nt.data(), b->ntiles_g, groups * 4, hipMemcpyDeviceToHost, st));
     HKG_HIP_TRY(hipStreamSynchronize(st));
-    // (query, series) pairs and row tiles the filter went through
+    // (query, series) pairs and row tiles the filter passes went through
     for (uint32_t g = 0; g < groups; ++g) {
       uint64_t qn = std::min<uint32_t>(kQPerBlock, nq - g * kQPerBlock);
-      ctr->v[kCtrBf16] += (unsigned long long)nt[g] * kRowTile * qn;
+      const uint64_t t1 = nt_pass1.empty() ? 0 : nt_pass1[g];
+      ctr->v[kCtrBf16] += (unsigned long long)(nt[g] + t1) * kRowTile * qn;
       ctr->bf16_rows += (uint64_t)nt[g] * kRowTile;
+      ctr->bf16_rows_bound += t1 * kRowTile;
     }
   }
   *used = b;
@@ -485,6 +587,12 @@ hkg_status Index::fix_bf16_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax,
   for (uint32_t q = 0; q < nq; ++q)
     if (cnt[q] > b.bf_cap) todo.push_back(q);
   if (todo.empty()) return HKG_OK;
+  const bool dbg = getenv("HKG_DEBUG") != nullptr;
+  std::vector<uint32_t> dbg_bound;
+  if (dbg) {
+    dbg_bound.resize(nq);
+    HKG_HIP_TRY(hipMemcpy(dbg_bound.data(), b.bound, nq * 4, hipMemcpyDeviceToHost));
+  }
   uint32_t m = (uint32_t)todo.size();
   float *tq = nullptr;
   uint32_t *tk = nullptr;
@@ -504,7 +612,7 @@ hkg_status Index::fix_bf16_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax,
     seed[i] = kth < FLT_MAX ? bits : kInfBits;
   }
   hkg_status rc = HKG_OK;
-  const bool dbg = getenv("HKG_DEBUG") != nullptr;
+  const std::vector<uint32_t> todo0 = todo;
   if (dbg) {
     uint32_t with_seed = 0, overfull = 0;
     for (uint32_t i = 0; i < m; ++i) {
@@ -522,7 +630,9 @@ hkg_status Index::fix_bf16_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax,
     std::vector<uint32_t> c2(m);
     std::vector<float> o2((size_t)m * kmax);
     if (round < 3) {
-      rc = run_bf16_batch(tq, tk, m, kmax, st, &sb, &c, nullptr, seed.data());
+      // the first re-run adds the upper-bound histogram pass (unless the
+      // caller turned it off); later ones hold exact seeds
+      rc = run_bf16_batch(tq, tk, m, kmax, st, &sb, &c, nullptr, seed.data(), round == 0 && bound_pass_mode != 2);
       if (rc != HKG_OK) break;
       if (hipMemcpyAsync(c2.data(), sb->bf_cnt, m * 4, hipMemcpyDeviceToHost, st) != hipSuccess) rc = HKG_EDEVICE;
     } else {
@@ -592,6 +702,20 @@ hkg_status Index::fix_bf16_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax,
   }
   (void)hipFree(tq);
   (void)hipFree(tk);
+  if (dbg && rc == HKG_OK) {
+    // first flagged queries: bound, k-th found by the flagged pass, final k-th
+    std::vector<float> fin((size_t)nq * kmax);
+    if (hipMemcpyAsync(fin.data(), b.out_d2, fin.size() * 4, hipMemcpyDeviceToHost, st) == hipSuccess &&
+        hipStreamSynchronize(st) == hipSuccess) {
+      for (size_t i = 0; i < todo0.size() && i < 12; ++i) {
+        const uint32_t q = todo0[i];
+        float bnd;
+        memcpy(&bnd, &dbg_bound[q], 4);
+        fprintf(stderr, "hkg:   q %u k %u bound %.6g found %.6g final %.6g\n", q, kv[q], bnd,
+                d2[(size_t)q * kmax + kv[q] - 1], fin[(size_t)q * kmax + kv[q] - 1]);
+      }
+    }
+  }
   return rc;
 }
 

@@ -316,14 +316,17 @@ __global__ void k_count_left(uint32_t nf, const uint64_t *__restrict__ nbeg, con
   nleft[i] = ex[e - 1] + flag[e - 1] - ex[b];
 }
 
+// grid-stride: n * L / 4 vectors can exceed the 2^32 work-items one
+// dispatch can address
 __global__ void k_gather_rows(uint64_t n, uint32_t L, const float *__restrict__ X, const uint32_t *__restrict__ perm,
                               float *__restrict__ out)
 {
   const uint32_t q = L / 4;
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * q) return;
-  uint64_t p = i / q, c = i - p * q;
-  reinterpret_cast<float4 *>(out)[i] = reinterpret_cast<const float4 *>(X)[(uint64_t)perm[p] * q + c];
+  const uint64_t nv = n * q, stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride) {
+    const uint64_t p = i / q, c = i - p * q;
+    reinterpret_cast<float4 *>(out)[i] = reinterpret_cast<const float4 *>(X)[(uint64_t)perm[p] * q + c];
+  }
 }
 
 // iSAX word of every row: PAA (sax.c:46-56, sequential FP32 sum then
@@ -732,7 +735,8 @@ hkg_status build_device(float *dX, uint64_t n, uint32_t L, const hkg_build_opts 
     }
   }
   const uint64_t nv = n * (L / 4);
-  hipLaunchKernelGGL(k_gather_rows, dim3((uint32_t)((nv + 255) / 256)), dim3(256), 0, st, n, L, dX, perm.p, Xl);
+  hipLaunchKernelGGL(k_gather_rows, dim3((uint32_t)std::min<uint64_t>((nv + 255) / 256, 1u << 20)), dim3(256), 0, st,
+                     n, L, dX, perm.p, Xl);
   BTRY(hipDeviceSynchronize());
   (void)hipFree(dX);
   dv.X = Xl;

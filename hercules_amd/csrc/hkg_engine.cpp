@@ -310,6 +310,7 @@ void QueryBuffers::free_bf16()
   dfree(tiles);
   dfree(ntiles_g);
   dfree(thr);
+  dfree(hist);
 }
 
 QueryBuffers::~QueryBuffers()
@@ -340,10 +341,14 @@ hkg_status Index::get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuff
 {
   const uint32_t L = dev.L, nleaf = dev.nleaf;
   const uint64_t key = ((uint64_t)B << 40) ^ ((uint64_t)kmax << 20) ^ cap;
-  auto it = bufs.find(key);
-  if (it != bufs.end()) {
-    *out = it->second.get();
-    return HKG_OK;
+  // a cached set of the same shape that no caller up the stack still holds
+  // (an overflow re-run of a whole batch asks for the shape of its caller)
+  auto range = bufs.equal_range(key);
+  for (auto it = range.first; it != range.second; ++it) {
+    if (std::find(in_use.begin(), in_use.end(), it->second.get()) == in_use.end()) {
+      *out = it->second.get();
+      return HKG_OK;
+    }
   }
   // keep at most a few buffer sets alive; never the one a caller still uses
   if (bufs.size() >= 4) {
@@ -395,7 +400,7 @@ hkg_status Index::get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuff
 #undef A
   b->bytes = acct;
   *out = b.get();
-  bufs[key] = std::move(b);
+  bufs.emplace(key, std::move(b));
   return HKG_OK;
 }
 
@@ -628,6 +633,43 @@ hkg_status Index::fix_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax, hipS
   return rc;
 }
 
+static bool debug_enabled() { return getenv("HKG_DEBUG") != nullptr; }
+
+// HKG_DEBUG: how far the filter's first threshold was from the final k-th
+// distance, and how many survivors each query had
+static void print_bf16_summary(const QueryBuffers *b, uint32_t n, uint32_t kmax, const std::vector<uint32_t> &cnt,
+                               const std::vector<uint32_t> &thr_bits)
+{
+  std::vector<uint32_t> kv(n);
+  std::vector<float> d2((size_t)n * kmax);
+  if (hipMemcpy(kv.data(), b->kq, n * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(d2.data(), b->out_d2, d2.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    return;
+  std::vector<double> gap, ratio, surv;
+  uint32_t dropped = 0;
+  for (uint32_t q = 0; q < n; ++q) {
+    float thr;
+    memcpy(&thr, &thr_bits[q], 4);
+    const float kth = d2[(size_t)q * kmax + kv[q] - 1];
+    if (thr < FLT_MAX && kth < FLT_MAX) {
+      gap.push_back((double)thr - kth);
+      ratio.push_back(kth > 0 ? (double)thr / kth : 1.0);
+    }
+    if (cnt[q] == 0xFFFFFFFFu) ++dropped;
+    else surv.push_back(cnt[q]);
+  }
+  auto qs = [](std::vector<double> v, const char *name) {
+    if (v.empty()) return;
+    std::sort(v.begin(), v.end());
+    auto at = [&](double f) { return v[std::min<size_t>(v.size() - 1, (size_t)(f * v.size()))]; };
+    fprintf(stderr, "hkg:   %-22s p10 %.4g p50 %.4g p90 %.4g max %.4g\n", name, at(0.1), at(0.5), at(0.9), v.back());
+  };
+  fprintf(stderr, "hkg: bf16 batch of %u: %u queries dropped survivors\n", n, dropped);
+  qs(gap, "threshold - kth");
+  qs(ratio, "threshold / kth");
+  qs(surv, "survivors (kept lists)");
+}
+
 hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, uint32_t kmax,
                              const hkg_query_opts *qo, uint64_t *dids, float *dd2, hipStream_t st,
                              hkg_query_stats *stats)
@@ -651,7 +693,19 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
   if (batch == 0) batch = path == HKG_PATH_INDEX ? 1 : (path == HKG_PATH_SCAN ? 64 : 1024);
   if (path == HKG_PATH_INDEX) batch = 1;
   if (batch > 16384) batch = 16384;  // grid.y of the per-query launches
+  // per-call settings; a nested call (the last-resort re-run of the BF16
+  // path) restores its caller's on return
+  struct Restore {
+    Index *ix;
+    uint32_t cap, mode;
+    ~Restore()
+    {
+      ix->cap_override = cap;
+      ix->bound_pass_mode = mode;
+    }
+  } restore{this, cap_override, bound_pass_mode};
   cap_override = o.cand_cap;
+  bound_pass_mode = o.bf16_bound_pass;
   HKG_HIP_TRY(hipSetDevice(dev.device));
   if (!st) st = stream;
   hipEvent_t e0, e1;
@@ -677,7 +731,8 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     hkg_status rc;
     if (pf) pf->n = 0;
     if (path == HKG_PATH_BF16)
-      rc = run_bf16_batch(dQ + (uint64_t)q0 * dev.L, dk + q0, n, kmax, st, &b, &c, pf.get());
+      rc = run_bf16_batch(dQ + (uint64_t)q0 * dev.L, dk + q0, n, kmax, st, &b, &c, pf.get(), nullptr,
+                          o.bf16_bound_pass == 1);
     else
       rc = run_batch(path, dQ + (uint64_t)q0 * dev.L, dk + q0, n, kmax, path == HKG_PATH_BATCHED && n == batch, st,
                      &b, &c, pf.get());
@@ -685,12 +740,23 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     HKG_HIP_TRY(hipStreamSynchronize(st));
     ++launches;
     if (pf) pf->accumulate(stage_ms, stage_launches);
+    std::vector<uint32_t> dbg_cnt, dbg_thr;
+    if (path == HKG_PATH_BF16 && debug_enabled()) {
+      dbg_cnt.resize(n);
+      dbg_thr.resize(n);
+      HKG_HIP_TRY(hipMemcpy(dbg_cnt.data(), b->bf_cnt, n * 4, hipMemcpyDeviceToHost));
+      HKG_HIP_TRY(hipMemcpy(dbg_thr.data(), b->thr, n * 4, hipMemcpyDeviceToHost));
+    }
     if (c.overflow) {
       in_use.push_back(b);
       rc = path == HKG_PATH_BF16 ? fix_bf16_overflow(*b, n, kmax, st, &retries)
                                  : fix_overflow(*b, n, kmax, st, &retries);
       in_use.pop_back();
       if (rc != HKG_OK) return rc;
+    }
+    if (!dbg_cnt.empty()) {
+      HKG_HIP_TRY(hipStreamSynchronize(st));
+      print_bf16_summary(b, n, kmax, dbg_cnt, dbg_thr);
     }
     HKG_HIP_TRY(hipMemcpyAsync(dids + (uint64_t)q0 * kmax, b->out_ids, (size_t)n * kmax * 8,
                                hipMemcpyDeviceToDevice, st));
@@ -699,6 +765,7 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     for (int i = 0; i < kNumCounters; ++i) total.v[i] += c.v[i];
     total.nwork += c.nwork;
     total.bf16_rows += c.bf16_rows;
+    total.bf16_rows_bound += c.bf16_rows_bound;
   }
   HKG_HIP_TRY(hipEventRecord(e1, st));
   HKG_HIP_TRY(hipStreamSynchronize(st));
@@ -728,7 +795,10 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     stats->stage_bytes[HKG_STAGE_REFINE] =
         path == HKG_PATH_BF16 ? total.bf16_rows * ((uint64_t)dev.L * 2 + 16)
                               : total.v[kCtrSaxChecked] * 16 + total.v[kCtrRefined] * L4;
-    if (path == HKG_PATH_BF16) stats->stage_bytes[HKG_STAGE_TOPK] = total.v[kCtrRefined] * L4;
+    if (path == HKG_PATH_BF16) {
+      stats->stage_bytes[HKG_STAGE_TOPK] = total.v[kCtrRefined] * L4;
+      stats->stage_bytes[HKG_STAGE_BOUND] = total.bf16_rows_bound * ((uint64_t)dev.L * 2 + 16);
+    }
     stats->stage_bytes[HKG_STAGE_SCAN] = scan ? (uint64_t)launches * 0 + total_scan_bytes(nq, batch) : 0;
     stats->stage_bytes[HKG_STAGE_EAPCA] = scan ? 0 : (uint64_t)nq * dev.nleaf * 8 + total_segs() * 18ull * nq;
     stats->bytes_algorithmic = 0;

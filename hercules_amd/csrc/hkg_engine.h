@@ -16,7 +16,8 @@ struct Counters {
   unsigned long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t overflow = 0;
   uint32_t nwork = 0;
-  uint64_t bf16_rows = 0;    // rows the BF16 filter read (summed over block groups)
+  uint64_t bf16_rows = 0;    // rows the BF16 collecting pass read (summed over block groups)
+  uint64_t bf16_rows_bound = 0;  // rows the BF16 histogram pass read
 };
 
 // per-stage event timing of one batch (profile mode)
@@ -92,6 +93,7 @@ struct QueryBuffers {
   uint32_t *tiles = nullptr;    // groups x ntiles: row tiles of each block group
   uint32_t *ntiles_g = nullptr; // groups
   uint32_t *thr = nullptr;      // B: filter threshold (float bits)
+  uint32_t *hist = nullptr;     // B x kBf16Bins: counts of series by BF16 upper bound
   void free_bf16();
   // captured stage sequence of the batched path
   hipGraphExec_t graph_exec = nullptr;
@@ -104,8 +106,9 @@ struct Index {
   hipStream_t stream = nullptr;
   uint64_t *idmap64 = nullptr;  // rank -> public id (when an id map is attached)
   uint32_t cap_override = 0;    // candidate buffer entries per query requested by the caller
+  uint32_t bound_pass_mode = 0; // hkg_query_opts.bf16_bound_pass of the current call
   std::vector<QueryBuffers *> in_use;  // buffer sets the buffer cache must not evict
-  std::map<uint64_t, std::unique_ptr<QueryBuffers>> bufs;
+  std::multimap<uint64_t, std::unique_ptr<QueryBuffers>> bufs;
 
   ~Index();
   hkg_status init_device(int device);
@@ -120,7 +123,8 @@ struct Index {
                        bool use_graph, hipStream_t st, QueryBuffers **used, Counters *ctr, Prof *pf,
                        const uint32_t *seed_host = nullptr);
   hkg_status run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax, hipStream_t st,
-                            QueryBuffers **used, Counters *ctr, Prof *pf, const uint32_t *seed_host = nullptr);
+                            QueryBuffers **used, Counters *ctr, Prof *pf, const uint32_t *seed_host,
+                            bool bound_pass);
   uint64_t total_scan_bytes(uint32_t nq, uint32_t batch) const;
   uint64_t total_segs() const;
   hkg_status fix_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax, hipStream_t st, uint32_t *retries);

@@ -11,6 +11,7 @@ constexpr int kScanTile = 2048;          // floats of chunk sums per wave (scan)
 constexpr size_t kEapcaLdsBytes = 48 * 1024;
 constexpr int kWorkChunk = 8;            // work items taken per atomic
 constexpr int kBestLeaves = 4;           // leaves per query scanned in the approximate stage
+constexpr int kBf16Bins = 32;            // threshold histogram bins per query (BF16 path)
 
 // device counters (unsigned long long each)
 enum Counter : int {
@@ -51,8 +52,8 @@ void launch_scan(hipStream_t st, uint32_t grid_x, const float *X, uint64_t n, ui
 void launch_bf16_refine(hipStream_t st, uint32_t nq, const float *X, uint32_t L, const float *Q, const uint32_t *kq,
                         uint32_t kmax, const uint32_t *ids, uint32_t cap, const uint64_t *cand,
                         const uint32_t *cand_cnt, uint32_t scap, const uint64_t *surv, uint32_t *scnt,
-                        const float *thr0, const uint64_t *idmap64, uint64_t *out_ids, float *out_d2,
-                        uint32_t *overflow, unsigned long long *counters);
+                        const float *thr0, const uint32_t *bound, const uint64_t *idmap64, uint64_t *out_ids,
+                        float *out_d2, uint32_t *overflow, unsigned long long *counters);
 void launch_query_bf16(hipStream_t st, uint32_t nq, uint32_t L, const float *Q, uint16_t *Qb, float4 *qst);
 void launch_select(hipStream_t st, uint32_t nq, uint32_t kmax, const uint32_t *kq, uint32_t cap,
                    const uint64_t *cand, const uint32_t *cand_cnt, const uint64_t *idmap64, uint64_t *out_ids,

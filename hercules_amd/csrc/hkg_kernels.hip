@@ -683,9 +683,12 @@ __global__ void __launch_bounds__(kBlock) k_select(uint32_t kmax, const uint32_t
 // bounds: first the ~kRefineFirst smallest bounds (which fixes a k-th
 // distance close to the final one), then only the survivors whose bound is
 // within the current k-th distance.  Every survivor whose bound is within the
-// final k-th distance is refined, so the answer is exact when no survivor was
-// dropped for lack of room; otherwise the query is flagged (scnt = ~0) and
-// the caller answers it again, seeded with this k-th distance.
+// final k-th distance is refined.  The survivors are every series whose
+// bound is within `bound` (an upper bound of the k-th distance counted over
+// series), so the answer is exact when no survivor was dropped for lack of
+// room and the final k-th distinct distance is within `bound`; otherwise the
+// query is flagged (scnt = ~0) and the caller answers it again, seeded with
+// this k-th distance.
 constexpr int kSurvBins = 256;
 constexpr uint32_t kRefineFirst = 2048;
 
@@ -693,9 +696,9 @@ __global__ void __launch_bounds__(kBlock) k_bf16_refine(
     const float *__restrict__ X, uint32_t L, const float *__restrict__ Q, const uint32_t *__restrict__ kq,
     uint32_t kmax, const uint32_t *__restrict__ ids, uint32_t cap, const uint64_t *__restrict__ cand,
     const uint32_t *__restrict__ cand_cnt, uint32_t scap, const uint64_t *__restrict__ surv,
-    uint32_t *__restrict__ scnt, const float *__restrict__ thr0, const uint64_t *__restrict__ idmap64,
-    uint64_t *__restrict__ out_ids, float *__restrict__ out_d2, uint32_t *__restrict__ overflow,
-    unsigned long long *__restrict__ counters)
+    uint32_t *__restrict__ scnt, const float *__restrict__ thr0, const uint32_t *__restrict__ bound,
+    const uint64_t *__restrict__ idmap64, uint64_t *__restrict__ out_ids, float *__restrict__ out_d2,
+    uint32_t *__restrict__ overflow, unsigned long long *__restrict__ counters)
 {
   extern __shared__ float qs[];
   __shared__ TopK tk;
@@ -777,7 +780,9 @@ __global__ void __launch_bounds__(kBlock) k_bf16_refine(
     }
   }
   if ((threadIdx.x & 63) == 0 && nref) atomicAdd(&counters[kCtrRefined], (unsigned long long)nref);
-  if (threadIdx.x == 0 && dropped) {
+  const uint32_t bq = bound[q];
+  const bool uncertified = bq < kInfBits && (tk.cnt < k || tk.kth_bits > bq);
+  if (threadIdx.x == 0 && (dropped || uncertified)) {
     atomicAdd(overflow, 1u);
     scnt[q] = 0xFFFFFFFFu;  // tells the caller to answer this query again
   }
@@ -798,11 +803,11 @@ __global__ void __launch_bounds__(kBlock) k_bf16_refine(
 void launch_bf16_refine(hipStream_t st, uint32_t nq, const float *X, uint32_t L, const float *Q, const uint32_t *kq,
                         uint32_t kmax, const uint32_t *ids, uint32_t cap, const uint64_t *cand,
                         const uint32_t *cand_cnt, uint32_t scap, const uint64_t *surv, uint32_t *scnt,
-                        const float *thr0, const uint64_t *idmap64, uint64_t *out_ids, float *out_d2,
-                        uint32_t *overflow, unsigned long long *counters)
+                        const float *thr0, const uint32_t *bound, const uint64_t *idmap64, uint64_t *out_ids,
+                        float *out_d2, uint32_t *overflow, unsigned long long *counters)
 {
   hipLaunchKernelGGL(k_bf16_refine, dim3(nq), dim3(kBlock), L * sizeof(float), st, X, L, Q, kq, kmax, ids, cap, cand,
-                     cand_cnt, scap, surv, scnt, thr0, idmap64, out_ids, out_d2, overflow, counters);
+                     cand_cnt, scap, surv, scnt, thr0, bound, idmap64, out_ids, out_d2, overflow, counters);
 }
 
 // reset per-query state: bound = the seed (an upper bound on the k-th

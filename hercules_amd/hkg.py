@@ -37,10 +37,10 @@ class Options(C.Structure):
 
 class QueryOpts(C.Structure):
     _fields_ = [("path", C.c_int), ("batch", C.c_uint32), ("profile", C.c_uint32), ("cand_cap", C.c_uint32),
-                ("reserved", C.c_uint32 * 4)]
+                ("bf16_bound_pass", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
-STAGES = ["prep", "eapca", "select_leaf", "approx", "collect", "refine", "topk", "scan"]
+STAGES = ["prep", "eapca", "select_leaf", "approx", "collect", "refine", "topk", "scan", "bound"]
 
 
 class QueryStats(C.Structure):
@@ -48,13 +48,13 @@ class QueryStats(C.Structure):
                 ("series_total", C.c_uint64), ("series_lb_checked", C.c_uint64),
                 ("series_refined", C.c_uint64), ("bytes_algorithmic", C.c_uint64), ("device_ms", C.c_double),
                 ("overflow_retries", C.c_uint32), ("launches", C.c_uint32), ("bf16_pairs", C.c_uint64),
-                ("stage_ms", C.c_double * 8),
-                ("stage_bytes", C.c_uint64 * 8), ("stage_launches", C.c_uint64 * 8)]
+                ("stage_ms", C.c_double * len(STAGES)),
+                ("stage_bytes", C.c_uint64 * len(STAGES)), ("stage_launches", C.c_uint64 * len(STAGES))]
 
     def as_dict(self) -> dict:
         d = {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("stage_")}
         for f in ("stage_ms", "stage_bytes", "stage_launches"):
-            d[f] = {STAGES[i]: getattr(self, f)[i] for i in range(8)}
+            d[f] = {STAGES[i]: getattr(self, f)[i] for i in range(len(STAGES))}
         return d
 
 
@@ -273,7 +273,7 @@ class HerculesIndex:
 
     # ---- queries -------------------------------------------------------------
     def knn(self, Q: np.ndarray, k, path: str = "auto", batch: int = 0, profile: bool = False,
-            cand_cap: int = 0) -> KnnResult:
+            cand_cap: int = 0, bf16_bound_pass: int = 0) -> KnnResult:
         """Exact k-NN of each row of Q.  ``k`` is an int or a per-query array."""
         Q = np.ascontiguousarray(Q, np.float32)
         if Q.ndim == 1:
@@ -287,15 +287,16 @@ class HerculesIndex:
             kmax = int(kv.max()) if nq else 1
         ids = np.empty((nq, kmax), np.uint64)
         d2 = np.empty((nq, kmax), np.float32)
-        qo = QueryOpts(PATHS[path], batch, int(profile), cand_cap)
+        qo = QueryOpts(PATHS[path], batch, int(profile), cand_cap, bf16_bound_pass)
         st = QueryStats()
         _check(lib().hkg_knn(self._h, _ptr(Q), nq, _ptr(kv) if kv is not None else None, kmax, C.byref(qo),
                              _ptr(ids), _ptr(d2), C.byref(st)))
         return KnnResult(ids, d2, st.as_dict())
 
     def knn_device(self, dQ: DeviceArray, nq: int, dk: DeviceArray, kmax: int, dids: DeviceArray,
-                   dd2: DeviceArray, path: str = "auto", batch: int = 0, profile: bool = False) -> dict:
-        qo = QueryOpts(PATHS[path], batch, int(profile))
+                   dd2: DeviceArray, path: str = "auto", batch: int = 0, profile: bool = False,
+                   bf16_bound_pass: int = 0) -> dict:
+        qo = QueryOpts(PATHS[path], batch, int(profile), 0, bf16_bound_pass)
         st = QueryStats()
         _check(lib().hkg_knn_device(self._h, dQ.ptr, nq, dk.ptr, kmax, C.byref(qo), dids.ptr, dd2.ptr, None,
                                     C.byref(st)))

@@ -81,7 +81,10 @@ typedef struct hkg_query_opts {
   uint32_t profile;    /* 1: time every stage with events (no graph replay)   */
   uint32_t cand_cap;   /* candidate buffer entries per query (0 = default);
                           when a query overflows it, it is answered again    */
-  uint32_t reserved[4];
+  uint32_t bf16_bound_pass; /* BF16 path: 0 = only when re-running queries
+                               whose survivor list overflowed, 1 = always,
+                               2 = never (seeded re-runs only)              */
+  uint32_t reserved[3];
 } hkg_query_opts;
 
 /* stages reported in hkg_query_stats.stage_ms / stage_bytes */
@@ -94,7 +97,9 @@ enum {
   HKG_STAGE_REFINE = 5,   /* iSAX or BF16 filter + exact refinement          */
   HKG_STAGE_TOPK = 6,     /* final per-query top-k                           */
   HKG_STAGE_SCAN = 7,     /* PSCAN full scan                                 */
-  HKG_NUM_STAGES = 8
+  HKG_STAGE_BOUND = 8,    /* BF16 path: upper-bound histogram pass (k-th
+                             distance bound before survivors are collected)  */
+  HKG_NUM_STAGES = 9
 };
 
 typedef struct hkg_query_stats {
@@ -109,9 +114,9 @@ typedef struct hkg_query_stats {
   uint32_t overflow_retries;      /* candidate-buffer overflows re-run         */
   uint32_t launches;              /* kernel launch sets (batches)              */
   uint64_t bf16_pairs;            /* (query, series) pairs the BF16 bound saw  */
-  double stage_ms[8];             /* with profile=1: summed stage times        */
-  uint64_t stage_bytes[8];        /* algorithmic HBM bytes of each stage       */
-  uint64_t stage_launches[8];     /* kernel launches of each stage             */
+  double stage_ms[HKG_NUM_STAGES];         /* with profile=1: summed stage times */
+  uint64_t stage_bytes[HKG_NUM_STAGES];    /* algorithmic HBM bytes per stage    */
+  uint64_t stage_launches[HKG_NUM_STAGES]; /* kernel launches of each stage      */
 } hkg_query_stats;
 
 typedef struct hkg_index_info {

@@ -182,19 +182,24 @@ def test_profile_stages(hkg, small_data, ref_index):
 
 
 # ---- BF16 certified-bound path ----------------------------------------------------
+@pytest.mark.parametrize("bound_pass", [0, 1])
 @pytest.mark.parametrize("k", [1, 10, 100])
-def test_bf16_path_matches_oracle(hkg, oracle, small_data, ref_index, k):
+def test_bf16_path_matches_oracle(hkg, oracle, small_data, ref_index, k, bound_pass):
     X, Q = small_data
     idx, _ = ref_index
-    _say(f"bf16 k={k}")
+    _say(f"bf16 k={k} bound_pass={bound_pass}")
     ids, d2 = oracle.knn(X, Q, k)
     with hkg.HerculesIndex.open(idx, keep_bf16=True) as ix:
         ix.set_ids(row_map(idx, X))
-        r = ix.knn(Q, k, path="bf16")
+        r = ix.knn(Q, k, path="bf16", bf16_bound_pass=bound_pass, profile=True)
         s = ix.knn(Q, k, path="batched")
     assert_same(r.ids, r.d2, ids, d2, f"bf16 k={k}")
-    # pairs of the row tiles the filter visited (128-row tiles; skipped tiles excluded)
-    assert 0 < r.stats["bf16_pairs"] <= Q.shape[0] * ((X.shape[0] + 127) // 128) * 128
+    # pairs of the row tiles the two filter passes visited (128-row tiles;
+    # skipped tiles excluded)
+    assert 0 < r.stats["bf16_pairs"] <= 2 * Q.shape[0] * ((X.shape[0] + 127) // 128) * 128
+    if bound_pass == 1:
+        assert r.stats["stage_launches"]["bound"] == 1 and r.stats["stage_bytes"]["bound"] > 0
+    assert r.stats["overflow_retries"] == 0
     # the certified bound prunes far more than the iSAX bound
     assert r.stats["series_refined"] <= s.stats["series_refined"] + Q.shape[0] * k
 
