@@ -301,6 +301,12 @@ __global__ void k_bf16_tau(uint32_t nq, const uint32_t *__restrict__ kq, const u
 
 }  // namespace
 
+uint64_t bf16_scratch_per_query(uint32_t L, uint64_t nleaf, uint64_t ntiles)
+{
+  return (uint64_t)kBf16SurvivorCap * 8 + 4 + (uint64_t)L * 2 + 16 + 16 + 4 + 4ull * kBf16Bins +
+         (nleaf + 4 * ntiles) / kQPerBlock + 16;
+}
+
 static uint32_t bf16_grid_x(uint32_t groups)
 {
   int d = 0, cus = 0;
@@ -400,7 +406,14 @@ __global__ void k_iota_bound(uint32_t nq, const uint32_t *__restrict__ bound, ui
 
 hkg_status Index::make_bf16()
 {
-  if (dev.Xb) return HKG_OK;
+  std::lock_guard<std::mutex> lock(mu_bf16);
+  if (bf16_ready) return HKG_OK;
+  // an earlier attempt that failed part-way
+  for (void *p : {(void *)dev.Xb, (void *)dev.xst, (void *)dev.tile_leaf0, (void *)dev.tile_leaf1})
+    if (p) (void)hipFree(p);
+  dev.Xb = nullptr;
+  dev.xst = nullptr;
+  dev.tile_leaf0 = dev.tile_leaf1 = nullptr;
   if (dev.L % 16 != 0 || (dev.L != 96 && dev.L != 128 && dev.L != 256)) {
     set_error("the BF16 filter supports series of length 96, 128 and 256");
     return HKG_EINVAL;
@@ -450,6 +463,7 @@ hkg_status Index::make_bf16()
     HKG_HIP_TRY(hipMemcpy(dev.tile_leaf0, l0.data(), ntiles * 4, hipMemcpyHostToDevice));
     HKG_HIP_TRY(hipMemcpy(dev.tile_leaf1, l1.data(), ntiles * 4, hipMemcpyHostToDevice));
   }
+  bf16_ready = true;
   return HKG_OK;
 }
 
@@ -466,6 +480,8 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
   QueryBuffers *b = nullptr;
   hkg_status rc = get_buffers(nq, kmax, cap, &b);
   if (rc != HKG_OK) return rc;
+  BufLease lease{this, b};  // handed to the caller on success
+  const uint32_t cap_override = call_settings().cap_override;
   const uint32_t want_cap = cap_override ? std::max<uint32_t>(cap_override, 16) : kBf16SurvivorCap;
   const uint32_t groups_max = (b->B + kQPerBlock - 1) / kQPerBlock;
   if (b->bf_surv && b->bf_cap != want_cap) b->free_bf16();
@@ -567,6 +583,7 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
     }
   }
   *used = b;
+  lease.b = nullptr;
   return HKG_OK;
 }
 
@@ -614,26 +631,25 @@ hkg_status Index::fix_bf16_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax,
   hkg_status rc = HKG_OK;
   const std::vector<uint32_t> todo0 = todo;
   if (dbg) {
-    uint32_t with_seed = 0, overfull = 0;
-    for (uint32_t i = 0; i < m; ++i) {
-      with_seed += seed[i] < kInfBits;
-      overfull += cnt[todo[i]] != 0xFFFFFFFFu;
-    }
-    fprintf(stderr, "hkg: bf16 re-run: %u queries (%u with an exact seed, %u list overflows)\n", m, with_seed,
-            overfull);
+    uint32_t with_seed = 0;
+    for (uint32_t i = 0; i < m; ++i) with_seed += seed[i] < kInfBits;
+    fprintf(stderr, "hkg: bf16 re-run: %u flagged queries (%u with a k-th distance seed)\n", m, with_seed);
   }
   for (int round = 0; round < 4 && m > 0 && rc == HKG_OK; ++round) {
     ++*retries;
     const auto t_round = std::chrono::steady_clock::now();
     QueryBuffers *sb = nullptr;
+    BufLease lease{this, nullptr};
     Counters c;
     std::vector<uint32_t> c2(m);
     std::vector<float> o2((size_t)m * kmax);
     if (round < 3) {
       // the first re-run adds the upper-bound histogram pass (unless the
       // caller turned it off); later ones hold exact seeds
-      rc = run_bf16_batch(tq, tk, m, kmax, st, &sb, &c, nullptr, seed.data(), round == 0 && bound_pass_mode != 2);
+      rc = run_bf16_batch(tq, tk, m, kmax, st, &sb, &c, nullptr, seed.data(),
+                          round == 0 && call_settings().bound_pass != 2);
       if (rc != HKG_OK) break;
+      lease.b = sb;
       if (hipMemcpyAsync(c2.data(), sb->bf_cnt, m * 4, hipMemcpyDeviceToHost, st) != hipSuccess) rc = HKG_EDEVICE;
     } else {
       // last resort: the FP32 index path (its own overflow handling included)

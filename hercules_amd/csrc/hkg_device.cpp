@@ -1,4 +1,4 @@
-// Entry points whose device implementations are not in this build yet.
+// Device memory helpers of the C ABI (for callers without their own HIP binding).
 #include <string.h>
 
 #include "hkg_engine.h"
@@ -6,16 +6,6 @@
 
 
 extern "C" {
-void hkg_default_server_opts(hkg_server_opts *so)
-{
-  if (so) memset(so, 0, sizeof(*so));
-}
-hkg_status hkg_server_start(hkg_index *, const hkg_server_opts *, hkg_server **) { return HKG_EINVAL; }
-hkg_status hkg_submit(hkg_server *, const float *, uint32_t, uint64_t *) { return HKG_EINVAL; }
-hkg_status hkg_poll(hkg_server *, uint64_t, uint64_t *, float *, double *) { return HKG_EINVAL; }
-hkg_status hkg_wait(hkg_server *, uint64_t, uint64_t *, float *, double *) { return HKG_EINVAL; }
-void hkg_server_stop(hkg_server *) {}
-
 void *hkg_device_alloc(uint64_t bytes)
 {
   void *p = nullptr;

@@ -340,20 +340,23 @@ QueryBuffers::~QueryBuffers()
 hkg_status Index::get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuffers **out)
 {
   const uint32_t L = dev.L, nleaf = dev.nleaf;
+  B = std::max(B, call_settings().min_buffer_batch);
   const uint64_t key = ((uint64_t)B << 40) ^ ((uint64_t)kmax << 20) ^ cap;
-  // a cached set of the same shape that no caller up the stack still holds
-  // (an overflow re-run of a whole batch asks for the shape of its caller)
+  std::lock_guard<std::mutex> lock(mu_bufs);
+  // a cached set of the same shape that no call holds (an overflow re-run
+  // of a whole batch asks for the shape of its caller)
   auto range = bufs.equal_range(key);
   for (auto it = range.first; it != range.second; ++it) {
-    if (std::find(in_use.begin(), in_use.end(), it->second.get()) == in_use.end()) {
+    if (!it->second->busy) {
+      it->second->busy = true;
       *out = it->second.get();
       return HKG_OK;
     }
   }
-  // keep at most a few buffer sets alive; never the one a caller still uses
+  // keep at most a few buffer sets alive; never one a call holds
   if (bufs.size() >= 4) {
     for (auto e = bufs.begin(); e != bufs.end();) {
-      if (std::find(in_use.begin(), in_use.end(), e->second.get()) == in_use.end()) e = bufs.erase(e);
+      if (!e->second->busy) e = bufs.erase(e);
       else ++e;
     }
   }
@@ -399,9 +402,33 @@ hkg_status Index::get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuff
   A(out_d2, (uint64_t)B * kmax);
 #undef A
   b->bytes = acct;
+  b->busy = true;
   *out = b.get();
   bufs.emplace(key, std::move(b));
   return HKG_OK;
+}
+
+void Index::release_buffers(QueryBuffers *b)
+{
+  std::lock_guard<std::mutex> lock(mu_bufs);
+  b->busy = false;
+}
+
+uint64_t Index::scratch_per_query(hkg_path path, uint32_t kmax) const
+{
+  const uint64_t L = dev.L, nl = std::max<uint32_t>(dev.nleaf, 1);
+  const uint64_t cap = cand_cap(path, 1024, kmax);
+  uint64_t b = L * 4 + 4 + kPaaSegments * 4 + 2 * (L + 1) * 8 + 8 + 4 * kBestLeaves + 8 + cap * 8 + 4 + 4 +
+               kmax * 12 + 64;
+  if (dev.has_tree) b += nl * 4 /* lb */ + nl * 16 /* work keys, values */ + nl * 8 /* sort scratch */;
+  if (path == HKG_PATH_BF16) b += bf16_scratch_per_query(dev.L, nl, dev.ntiles);
+  return b;
+}
+
+CallSettings &call_settings()
+{
+  static thread_local CallSettings s;
+  return s;
 }
 
 // ---------------------------------------------------------------------------
@@ -488,6 +515,7 @@ void Index::enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Pro
 uint32_t Index::cand_cap(hkg_path path, uint32_t nq, uint32_t kmax) const
 {
   if (path == HKG_PATH_SCAN || !dev.has_tree) return scan_grid_x(nq) * kmax + 256;
+  const uint32_t cap_override = call_settings().cap_override;
   if (cap_override) return std::max<uint32_t>(cap_override, kmax);
   return std::max<uint32_t>(8192, 16 * kmax);
 }
@@ -501,6 +529,7 @@ hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, 
   QueryBuffers *b = nullptr;
   hkg_status rc = get_buffers(nq, kmax, cap, &b);
   if (rc != HKG_OK) return rc;
+  BufLease lease{this, b};  // handed to the caller on success
   HKG_HIP_TRY(hipMemcpyAsync(b->Q, dQ, (size_t)nq * dev.L * 4, hipMemcpyDeviceToDevice, st));
   HKG_HIP_TRY(hipMemcpyAsync(b->kq, dk, (size_t)nq * 4, hipMemcpyDeviceToDevice, st));
   if (seed_host) {
@@ -509,7 +538,7 @@ hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, 
     use_graph = false;
   }
   if (use_graph && !scan && !pf) {
-    if (!b->graph_exec || b->graph_stream != st) {
+    if (!b->graph_exec || b->graph_stream != st || b->graph_nq != nq) {
       if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
       b->graph_exec = nullptr;
       hipGraph_t g;
@@ -520,6 +549,7 @@ hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, 
       (void)hipGraphDestroy(g);
       HKG_HIP_TRY(e);
       b->graph_stream = st;
+      b->graph_nq = nq;
     }
     HKG_HIP_TRY(hipGraphLaunch(b->graph_exec, st));
   } else if (scan) {
@@ -534,6 +564,7 @@ hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, 
     HKG_HIP_TRY(hipMemcpyAsync(&ctr->nwork, b->misc + 0, 4, hipMemcpyDeviceToHost, st));
   }
   *used = b;
+  lease.b = nullptr;
   return HKG_OK;
 }
 
@@ -582,6 +613,7 @@ hkg_status Index::fix_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax, hipS
     rc = run_batch(last ? HKG_PATH_SCAN : HKG_PATH_BATCHED, tq, tk, m, kmax, false, st, &sb, &c, nullptr,
                    last ? nullptr : seed.data());
     if (rc != HKG_OK) break;
+    BufLease lease{this, sb};
     std::vector<uint32_t> c2(m);
     std::vector<float> o2((size_t)m * kmax);
     if (hipMemcpyAsync(c2.data(), sb->cand_cnt, m * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -684,8 +716,8 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
   if (nq == 0) return HKG_OK;
   hkg_path path = o.path;
   if (path == HKG_PATH_AUTO) path = HKG_PATH_BATCHED;
-  if (path == HKG_PATH_BF16 && !dev.Xb) {
-    hkg_status rc = make_bf16();
+  if (path == HKG_PATH_BF16) {
+    hkg_status rc = make_bf16();  // once per index
     if (rc != HKG_OK) return rc;
   }
   if (!dev.has_tree && path != HKG_PATH_SCAN) path = HKG_PATH_SCAN;
@@ -696,16 +728,11 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
   // per-call settings; a nested call (the last-resort re-run of the BF16
   // path) restores its caller's on return
   struct Restore {
-    Index *ix;
-    uint32_t cap, mode;
-    ~Restore()
-    {
-      ix->cap_override = cap;
-      ix->bound_pass_mode = mode;
-    }
-  } restore{this, cap_override, bound_pass_mode};
-  cap_override = o.cand_cap;
-  bound_pass_mode = o.bf16_bound_pass;
+    CallSettings saved;
+    ~Restore() { call_settings() = saved; }
+  } restore{call_settings()};
+  call_settings().cap_override = o.cand_cap;
+  call_settings().bound_pass = o.bf16_bound_pass;
   HKG_HIP_TRY(hipSetDevice(dev.device));
   if (!st) st = stream;
   hipEvent_t e0, e1;
@@ -737,6 +764,7 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
       rc = run_batch(path, dQ + (uint64_t)q0 * dev.L, dk + q0, n, kmax, path == HKG_PATH_BATCHED && n == batch, st,
                      &b, &c, pf.get());
     if (rc != HKG_OK) return rc;
+    BufLease lease{this, b};
     HKG_HIP_TRY(hipStreamSynchronize(st));
     ++launches;
     if (pf) pf->accumulate(stage_ms, stage_launches);
@@ -748,10 +776,8 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
       HKG_HIP_TRY(hipMemcpy(dbg_thr.data(), b->thr, n * 4, hipMemcpyDeviceToHost));
     }
     if (c.overflow) {
-      in_use.push_back(b);
       rc = path == HKG_PATH_BF16 ? fix_bf16_overflow(*b, n, kmax, st, &retries)
                                  : fix_overflow(*b, n, kmax, st, &retries);
-      in_use.pop_back();
       if (rc != HKG_OK) return rc;
     }
     if (!dbg_cnt.empty()) {
@@ -762,6 +788,7 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
                                hipMemcpyDeviceToDevice, st));
     HKG_HIP_TRY(hipMemcpyAsync(dd2 + (uint64_t)q0 * kmax, b->out_d2, (size_t)n * kmax * 4,
                                hipMemcpyDeviceToDevice, st));
+    HKG_HIP_TRY(hipStreamSynchronize(st));  // before the set is released
     for (int i = 0; i < kNumCounters; ++i) total.v[i] += c.v[i];
     total.nwork += c.nwork;
     total.bf16_rows += c.bf16_rows;

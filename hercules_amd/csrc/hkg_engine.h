@@ -3,6 +3,7 @@
 
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -95,9 +96,11 @@ struct QueryBuffers {
   uint32_t *thr = nullptr;      // B: filter threshold (float bits)
   uint32_t *hist = nullptr;     // B x kBf16Bins: counts of series by BF16 upper bound
   void free_bf16();
+  bool busy = false;            // held by a query call (guarded by Index::mu_bufs)
   // captured stage sequence of the batched path
   hipGraphExec_t graph_exec = nullptr;
   hipStream_t graph_stream = nullptr;
+  uint32_t graph_nq = 0;
   ~QueryBuffers();
 };
 
@@ -105,9 +108,11 @@ struct Index {
   DevIndex dev;
   hipStream_t stream = nullptr;
   uint64_t *idmap64 = nullptr;  // rank -> public id (when an id map is attached)
-  uint32_t cap_override = 0;    // candidate buffer entries per query requested by the caller
-  uint32_t bound_pass_mode = 0; // hkg_query_opts.bf16_bound_pass of the current call
-  std::vector<QueryBuffers *> in_use;  // buffer sets the buffer cache must not evict
+  // Buffer sets, keyed by shape.  A query call holds each set it uses
+  // (busy) until its results are copied out, so concurrent calls on other
+  // streams and nested re-runs never share one.
+  std::mutex mu_bufs, mu_bf16;
+  bool bf16_ready = false;      // BF16 copy, norms and tile tables built (guarded by mu_bf16)
   std::multimap<uint64_t, std::unique_ptr<QueryBuffers>> bufs;
 
   ~Index();
@@ -116,6 +121,9 @@ struct Index {
   hkg_status set_ids(const uint64_t *ids, uint64_t n);
   hkg_status make_bf16();
   hkg_status get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuffers **out);
+  void release_buffers(QueryBuffers *b);
+  // device scratch one query of a batch needs on a path (buffer sets)
+  uint64_t scratch_per_query(hkg_path path, uint32_t kmax) const;
   uint32_t cand_cap(hkg_path path, uint32_t nq, uint32_t kmax) const;
   void enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf, bool seeded);
   void enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf);
@@ -133,6 +141,27 @@ struct Index {
                         uint64_t *dids, float *dd2, hipStream_t st, hkg_query_stats *stats);
   hkg_status knn_host(const float *Q, uint32_t nq, const uint32_t *k, uint32_t kmax, const hkg_query_opts *qo,
                       uint64_t *ids, float *d2, hkg_query_stats *stats);
+};
+
+// Settings of the query call running on this thread (from hkg_query_opts);
+// nested calls save and restore them.
+struct CallSettings {
+  uint32_t cap_override = 0;  // candidate (or survivor) entries per query requested by the caller
+  uint32_t bound_pass = 0;    // hkg_query_opts.bf16_bound_pass
+  uint32_t min_buffer_batch = 0;  // size buffer sets for at least this many queries (server workers)
+};
+CallSettings &call_settings();
+// BF16 path scratch of one query beyond the FP32 buffer set
+uint64_t bf16_scratch_per_query(uint32_t L, uint64_t nleaf, uint64_t ntiles);
+
+// releases a held buffer set when it goes out of scope
+struct BufLease {
+  Index *ix;
+  QueryBuffers *b;
+  ~BufLease()
+  {
+    if (b) ix->release_buffers(b);
+  }
 };
 
 hkg_status open_dir(const char *dir, const hkg_options *opt, Index **out);

@@ -1,0 +1,312 @@
+// Serving loop over a resident index: continuous batching.
+//
+// Requests (one query each) wait in a FIFO.  max_inflight worker threads,
+// each with its own HIP stream, pinned staging and device buffers, loop:
+// take every pending request that fits (max_batch queries, and the worker's
+// share of the scratch budget at the path's per-query scratch), upload the
+// batch, answer it with Index::knn_device on the worker's stream, copy the
+// results back and complete the requests.  A request that arrives while
+// batches are on the device is admitted by the next worker that frees up,
+// so with two workers one batch is uploaded and started while the other is
+// still running.  Each request is answered exactly as it would be alone:
+// the result of a query does not depend on the batch it ran in.
+//
+// The reference answers a query file in a loop, one query at a time
+// (hercules_file_buffer_manager / query answering in hercules.c); this
+// loop has no single counterpart there.
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "hkg_engine.h"
+
+namespace hkg {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  std::vector<float> q;
+  uint32_t k = 0;
+  Clock::time_point t0;
+  int state = 0;  // 0 pending, 1 on the device, 2 done, 3 failed
+  hkg_status rc = HKG_OK;
+  std::vector<uint64_t> ids;
+  std::vector<float> d2;
+  double lat_us = 0.0;
+};
+
+}  // namespace
+
+struct Server {
+  Index *ix = nullptr;
+  hkg_server_opts opt{};
+  uint64_t batch_scratch = 0;  // scratch share of one batch
+  uint64_t per_query = 0;      // scratch one query needs on the path
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::unordered_map<uint64_t, Request> reqs;
+  std::deque<uint64_t> pending;
+  uint64_t next_ticket = 1;
+  bool stopping = false;
+  std::vector<std::thread> workers;
+  hkg_status start_rc = HKG_OK;
+
+  void worker(int wi);
+};
+
+namespace {
+
+struct Slot {
+  hipStream_t st = nullptr;
+  float *hQ = nullptr, *dQ = nullptr, *hd2 = nullptr, *dd2 = nullptr;
+  uint32_t *hk = nullptr, *dk = nullptr;
+  uint64_t *hids = nullptr, *dids = nullptr;
+  ~Slot()
+  {
+    for (void *p : {(void *)hQ, (void *)hd2, (void *)hk, (void *)hids})
+      if (p) (void)hipHostFree(p);
+    for (void *p : {(void *)dQ, (void *)dd2, (void *)dk, (void *)dids})
+      if (p) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+  }
+  bool alloc(uint32_t B, uint32_t L, uint32_t K)
+  {
+    return hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
+           hipHostMalloc((void **)&hQ, (size_t)B * L * 4, hipHostMallocDefault) == hipSuccess &&
+           hipHostMalloc((void **)&hk, (size_t)B * 4, hipHostMallocDefault) == hipSuccess &&
+           hipHostMalloc((void **)&hids, (size_t)B * K * 8, hipHostMallocDefault) == hipSuccess &&
+           hipHostMalloc((void **)&hd2, (size_t)B * K * 4, hipHostMallocDefault) == hipSuccess &&
+           hipMalloc((void **)&dQ, (size_t)B * L * 4) == hipSuccess &&
+           hipMalloc((void **)&dk, (size_t)B * 4) == hipSuccess &&
+           hipMalloc((void **)&dids, (size_t)B * K * 8) == hipSuccess &&
+           hipMalloc((void **)&dd2, (size_t)B * K * 4) == hipSuccess;
+  }
+};
+
+}  // namespace
+
+void Server::worker(int wi)
+{
+  (void)wi;
+  const uint32_t L = ix->dev.L, B = opt.max_batch, K = opt.max_k;
+  if (hipSetDevice(ix->dev.device) != hipSuccess) return;
+  Slot s;
+  const bool ok = s.alloc(B, L, K);
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ok) {
+      (void)hipGetLastError();
+      start_rc = HKG_ENOMEM;
+      stopping = true;
+      cv_work.notify_all();
+      cv_done.notify_all();
+      return;
+    }
+  }
+  // buffer sets sized for a full batch: batches of any size reuse them
+  call_settings().min_buffer_batch = B;
+  hkg_query_opts qo;
+  hkg_default_query_opts(&qo);
+  qo.path = (hkg_path)opt.path;
+  qo.batch = B;
+  qo.bf16_bound_pass = opt.bf16_bound_pass;
+  std::vector<Request *> batch;
+  batch.reserve(B);
+  for (;;) {
+    batch.clear();
+    {
+      std::unique_lock<std::mutex> lock(mu);
+      cv_work.wait(lock, [&] { return stopping || !pending.empty(); });
+      if (pending.empty()) break;  // stopping and drained
+      // admit what fits: max_batch queries and this batch's scratch share
+      uint64_t used = 0;
+      while (!pending.empty() && batch.size() < B && (batch.empty() || used + per_query <= batch_scratch)) {
+        const uint64_t t = pending.front();
+        pending.pop_front();
+        Request &r = reqs[t];
+        r.state = 1;
+        batch.push_back(&r);
+        used += per_query;
+      }
+    }
+    // requests on the device are not touched by the client (their state
+    // is 1), and unordered_map never moves its elements
+    const uint32_t n = (uint32_t)batch.size();
+    for (uint32_t i = 0; i < n; ++i) {
+      const Request &r = *batch[i];
+      memcpy(s.hQ + (size_t)i * L, r.q.data(), (size_t)L * 4);
+      s.hk[i] = r.k;
+    }
+    hkg_status rc = HKG_OK;
+    if (hipMemcpyAsync(s.dQ, s.hQ, (size_t)n * L * 4, hipMemcpyHostToDevice, s.st) != hipSuccess ||
+        hipMemcpyAsync(s.dk, s.hk, (size_t)n * 4, hipMemcpyHostToDevice, s.st) != hipSuccess) {
+      rc = HKG_EDEVICE;
+    }
+    if (rc == HKG_OK) rc = ix->knn_device(s.dQ, n, s.dk, K, &qo, s.dids, s.dd2, s.st, nullptr);
+    if (rc == HKG_OK &&
+        (hipMemcpyAsync(s.hids, s.dids, (size_t)n * K * 8, hipMemcpyDeviceToHost, s.st) != hipSuccess ||
+         hipMemcpyAsync(s.hd2, s.dd2, (size_t)n * K * 4, hipMemcpyDeviceToHost, s.st) != hipSuccess ||
+         hipStreamSynchronize(s.st) != hipSuccess)) {
+      rc = HKG_EDEVICE;
+    }
+    const Clock::time_point now = Clock::now();
+    std::lock_guard<std::mutex> lock(mu);
+    for (uint32_t i = 0; i < n; ++i) {
+      Request &r = *batch[i];
+      r.rc = rc;
+      if (rc == HKG_OK) {
+        r.ids.assign(s.hids + (size_t)i * K, s.hids + (size_t)i * K + r.k);
+        r.d2.assign(s.hd2 + (size_t)i * K, s.hd2 + (size_t)i * K + r.k);
+      }
+      r.q.clear();
+      r.q.shrink_to_fit();
+      r.lat_us = std::chrono::duration<double, std::micro>(now - r.t0).count();
+      r.state = rc == HKG_OK ? 2 : 3;
+    }
+    cv_done.notify_all();
+  }
+}
+
+}  // namespace hkg
+
+using hkg::Server;
+
+extern "C" {
+
+void hkg_default_server_opts(hkg_server_opts *so)
+{
+  if (!so) return;
+  memset(so, 0, sizeof(*so));
+  so->max_batch = 1024;
+  so->scratch_bytes = 16ull << 30;
+  so->queue_capacity = 65536;
+  so->path = HKG_PATH_BATCHED;
+  so->max_inflight = 2;
+  so->max_k = hkg::kMaxK;
+}
+
+hkg_status hkg_server_start(hkg_index *index, const hkg_server_opts *so, hkg_server **out)
+{
+  if (!index || !out) return HKG_EINVAL;
+  *out = nullptr;
+  hkg_server_opts o;
+  hkg_default_server_opts(&o);
+  if (so) o = *so;
+  if (o.max_batch == 0) o.max_batch = 1024;
+  if (o.scratch_bytes == 0) o.scratch_bytes = 16ull << 30;
+  if (o.queue_capacity == 0) o.queue_capacity = 65536;
+  if (o.max_inflight == 0) o.max_inflight = 2;
+  if (o.max_k == 0) o.max_k = hkg::kMaxK;
+  if (o.max_batch > 16384 || o.max_inflight > 8 || o.max_k > (uint32_t)hkg::kMaxK ||
+      (o.path != HKG_PATH_AUTO && o.path != HKG_PATH_BATCHED && o.path != HKG_PATH_INDEX &&
+       o.path != HKG_PATH_SCAN && o.path != HKG_PATH_BF16)) {
+    hkg::set_error("server: max_batch <= 16384, max_inflight <= 8, max_k <= 256 and a valid path");
+    return HKG_EINVAL;
+  }
+  if (o.path == HKG_PATH_AUTO) o.path = HKG_PATH_BATCHED;
+  std::unique_ptr<Server> s(new Server());
+  s->ix = reinterpret_cast<hkg::Index *>(index);
+  s->opt = o;
+  s->per_query = s->ix->scratch_per_query((hkg_path)o.path, o.max_k);
+  s->batch_scratch = o.scratch_bytes / o.max_inflight;
+  if (s->per_query > s->batch_scratch) {
+    hkg::set_error("server: the scratch budget does not hold one query per batch");
+    return HKG_EINVAL;
+  }
+  if (o.path == HKG_PATH_BF16) {
+    hkg_status rc = s->ix->make_bf16();
+    if (rc != HKG_OK) return rc;
+  }
+  for (uint32_t i = 0; i < o.max_inflight; ++i) s->workers.emplace_back(&Server::worker, s.get(), (int)i);
+  *out = reinterpret_cast<hkg_server *>(s.release());
+  return HKG_OK;
+}
+
+hkg_status hkg_submit(hkg_server *sv, const float *q, uint32_t k, uint64_t *ticket)
+{
+  Server *s = reinterpret_cast<Server *>(sv);
+  if (!s || !q || !ticket) return HKG_EINVAL;
+  if (k == 0 || k > s->opt.max_k) {
+    hkg::set_error("submit: k must be in [1, max_k]");
+    return HKG_EINVAL;
+  }
+  const uint32_t L = s->ix->dev.L;
+  std::lock_guard<std::mutex> lock(s->mu);
+  if (s->stopping) {
+    hkg::set_error(s->start_rc != HKG_OK ? "server failed to start its workers" : "server is stopping");
+    return s->start_rc != HKG_OK ? s->start_rc : HKG_EINVAL;
+  }
+  if (s->pending.size() >= s->opt.queue_capacity) {
+    hkg::set_error("submit: queue full");
+    return HKG_EBUSY;
+  }
+  const uint64_t t = s->next_ticket++;
+  hkg::Request &r = s->reqs[t];
+  r.q.assign(q, q + L);
+  r.k = k;
+  r.t0 = hkg::Clock::now();
+  s->pending.push_back(t);
+  *ticket = t;
+  s->cv_work.notify_one();
+  return HKG_OK;
+}
+
+// copy a finished request out and forget it
+static hkg_status take(Server *s, uint64_t ticket, uint64_t *ids, float *d2, double *lat, bool block)
+{
+  std::unique_lock<std::mutex> lock(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) {
+    hkg::set_error("unknown ticket");
+    return HKG_EINVAL;
+  }
+  if (block)
+    s->cv_done.wait(lock, [&] { return it->second.state >= 2 || (s->stopping && s->start_rc != HKG_OK); });
+  hkg::Request &r = it->second;
+  if (r.state < 2) return s->start_rc != HKG_OK ? s->start_rc : HKG_ENOTREADY;
+  const hkg_status rc = r.rc;
+  if (rc == HKG_OK) {
+    if (ids) memcpy(ids, r.ids.data(), r.ids.size() * 8);
+    if (d2) memcpy(d2, r.d2.data(), r.d2.size() * 4);
+  }
+  if (lat) *lat = r.lat_us;
+  s->reqs.erase(it);
+  return rc;
+}
+
+hkg_status hkg_poll(hkg_server *sv, uint64_t ticket, uint64_t *ids, float *d2, double *latency_us)
+{
+  Server *s = reinterpret_cast<Server *>(sv);
+  if (!s) return HKG_EINVAL;
+  return take(s, ticket, ids, d2, latency_us, false);
+}
+
+hkg_status hkg_wait(hkg_server *sv, uint64_t ticket, uint64_t *ids, float *d2, double *latency_us)
+{
+  Server *s = reinterpret_cast<Server *>(sv);
+  if (!s) return HKG_EINVAL;
+  return take(s, ticket, ids, d2, latency_us, true);
+}
+
+void hkg_server_stop(hkg_server *sv)
+{
+  Server *s = reinterpret_cast<Server *>(sv);
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lock(s->mu);
+    s->stopping = true;
+  }
+  s->cv_work.notify_all();
+  for (std::thread &t : s->workers) t.join();  // pending requests are answered first
+  delete s;
+}
+
+}  // extern "C"

@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhkg.so")
 
 HKG_PATH_AUTO, HKG_PATH_INDEX, HKG_PATH_BATCHED, HKG_PATH_SCAN, HKG_PATH_BF16 = range(5)
+HKG_OK, HKG_EINVAL, HKG_ENOMEM, HKG_EIO, HKG_EFORMAT, HKG_EDEVICE, HKG_EBUSY, HKG_ENOTREADY = range(8)
 PATHS = {"auto": HKG_PATH_AUTO, "index": HKG_PATH_INDEX, "batched": HKG_PATH_BATCHED,
          "scan": HKG_PATH_SCAN, "bf16": HKG_PATH_BF16}
 STATUS = {0: "ok", 1: "invalid argument", 2: "out of memory", 3: "i/o error", 4: "malformed index",
@@ -72,7 +73,8 @@ class BuildOpts(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("max_batch", C.c_uint32), ("scratch_bytes", C.c_uint64), ("queue_capacity", C.c_uint32),
-                ("reserved", C.c_uint32 * 5)]
+                ("path", C.c_uint32), ("max_inflight", C.c_uint32), ("max_k", C.c_uint32),
+                ("bf16_bound_pass", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 # every entry point of include/hkg.h with its ctypes signature
@@ -333,7 +335,8 @@ def gen_noisy_queries(kind: int, data_seed: int, N: int, query_seed: int, sigmas
 class Server:
     """Continuous-batching request loop over a resident index."""
 
-    def __init__(self, index: HerculesIndex, max_batch: int = 0, scratch_bytes: int = 0, queue_capacity: int = 0):
+    def __init__(self, index: HerculesIndex, max_batch: int = 0, scratch_bytes: int = 0, queue_capacity: int = 0,
+                 path: str = "batched", max_inflight: int = 0, max_k: int = 0, bf16_bound_pass: int = 0):
         so = ServerOpts()
         lib().hkg_default_server_opts(C.byref(so))
         if max_batch:
@@ -342,6 +345,12 @@ class Server:
             so.scratch_bytes = scratch_bytes
         if queue_capacity:
             so.queue_capacity = queue_capacity
+        so.path = PATHS[path]
+        if max_inflight:
+            so.max_inflight = max_inflight
+        if max_k:
+            so.max_k = max_k
+        so.bf16_bound_pass = bf16_bound_pass
         h = C.c_void_p()
         _check(lib().hkg_server_start(index._h, C.byref(so), C.byref(h)))
         self._h = h
@@ -358,7 +367,7 @@ class Server:
         d2 = np.empty(k, np.float32)
         lat = C.c_double()
         rc = lib().hkg_poll(self._h, ticket, _ptr(ids), _ptr(d2), C.byref(lat))
-        if rc == 7:
+        if rc == HKG_ENOTREADY:
             return None
         _check(rc)
         return ids, d2, lat.value

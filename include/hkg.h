@@ -202,10 +202,16 @@ hkg_status hkg_gen_noisy_queries_device(int kind, uint64_t data_seed, uint64_t N
 
 /* ---- serving loop (continuous batching) --------------------------------- */
 typedef struct hkg_server_opts {
-  uint32_t max_batch;        /* queries admitted per device batch            */
-  uint64_t scratch_bytes;    /* device scratch budget for admitted queries   */
-  uint32_t queue_capacity;   /* pending requests before HKG_EBUSY            */
-  uint32_t reserved[5];
+  uint32_t max_batch;        /* queries admitted per device batch (1024)     */
+  uint64_t scratch_bytes;    /* device scratch budget of all batches in
+                                flight; a batch admits queries while its
+                                share fits (0 = 16 GiB)                     */
+  uint32_t queue_capacity;   /* pending requests before HKG_EBUSY (65536)    */
+  uint32_t path;             /* hkg_path of the batches (0 = BATCHED)        */
+  uint32_t max_inflight;     /* batches on the device at once (2)            */
+  uint32_t max_k;            /* largest k a request may ask for (256)        */
+  uint32_t bf16_bound_pass;  /* as in hkg_query_opts                         */
+  uint32_t reserved;
 } hkg_server_opts;
 
 void hkg_default_server_opts(hkg_server_opts *so);
