@@ -1,4 +1,4 @@
-// BF16 MFMA certified lower bound: a batched Q.X^T on the matrix cores
+// BF16 MFMA certified lower bound: a batched X.Q^T on the matrix cores
 // prunes (query, series) pairs; every survivor is refined exactly in FP32.
 //
 // For a query q and a series x (FP32 values), with q~, x~ their BF16
@@ -7,11 +7,17 @@
 //   |q.x - q~.x~| <= |q - q~| |x| + |q~| |x - x~|          (Cauchy-Schwarz)
 //   |q~.x~ - s|   <= g_L |q~| |x~|,  g_L = 2 L 2^-24        (products of two
 //       BF16 values are exact in FP32; any summation order of L terms)
-// so  LB = |q|^2 + |x|^2 - 2 s - 2 (|q - q~||x| + |q~||x - x~| + g_L |q~||x~|)
-// is a lower bound of d2.  The norms are computed in FP64 and rounded in the
-// safe direction; the FP32 evaluation of LB itself is covered by a further
-// absolute slack, and the comparison with the k-th distance uses the same
-// certified margin as the other bounds.  LB is used only to prune.
+// so  LB = |q|^2 + |x|^2 - 2 s - (|q - q~| A_x + |q~| B_x)
+// with A_x = 2|x| and B_x = 2 (|x - x~| + g_L |x~|) is a lower bound of d2
+// (and the same expression with + is an upper bound).  The norms are
+// computed in FP64 and rounded in the safe direction; the FP32 evaluation
+// is covered by a further absolute slack, and the comparison with the k-th
+// distance uses the same certified margin as the other bounds.  LB is used
+// only to prune.
+//
+// The filter tests every pair first against the bound with the largest
+// A_x, B_x of its 128-row tile (two instructions per pair); only pairs that
+// pass are re-tested with their own A_x, B_x.
 #include <float.h>
 #include <math.h>
 #include <string.h>
@@ -34,7 +40,10 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 namespace {
 
 constexpr int kQPerWave = 64;              // two 32-query MFMA tiles
-constexpr int kQPerBlock = 4 * kQPerWave;  // 256 queries per block
+// Waves per filter block: the query operands of a wave take 8 KS registers,
+// so L = 256 runs 4 waves (one per SIMD, up to 512 registers with the
+// accumulation registers) and shorter series 8 (two per SIMD).
+__host__ __device__ constexpr int filter_waves(int KS) { return KS >= 16 ? 4 : 8; }
 constexpr int kRowTile = 128;              // series per LDS tile
 constexpr float kEvalSlack = 1.0e-3f;      // FP32 evaluation of the bound
 constexpr uint32_t kBf16SurvivorCap = 524288;  // survivors kept per query (8 B each)
@@ -76,9 +85,9 @@ __device__ __forceinline__ HistRange hist_range(uint32_t bound_bits, float hi)
   return r;
 }
 
-// rows -> BF16 copy + {|x|^2 (down), |x| (up), |x - x~| (up), |x~| (up)}
+// rows -> BF16 copy, {|x|^2 (down), A_x (up), B_x (up), 0} and |x|^2 alone
 __global__ void k_make_bf16(uint64_t n, uint32_t L, const float *__restrict__ X, uint16_t *__restrict__ Xb,
-                            float4 *__restrict__ st)
+                            float4 *__restrict__ st, float *__restrict__ xn2)
 {
   uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
@@ -94,8 +103,26 @@ __global__ void k_make_bf16(uint64_t n, uint32_t L, const float *__restrict__ X,
     e2 += ((double)a - da) * ((double)a - da) + ((double)c - dc) * ((double)c - dc);
     b2 += da * da + dc * dc;
   }
-  st[r] = make_float4(rd(n2 * (1.0 - 1e-15)), ru(sqrt(n2) * (1.0 + 1e-15)), ru(sqrt(e2) * (1.0 + 1e-15)),
-                      ru(sqrt(b2) * (1.0 + 1e-15)));
+  const double gL = 2.0 * (double)L * 5.9604644775390625e-08;  // 2 L 2^-24
+  const double up = 1.0 + 1e-15;
+  const float n2d = rd(n2 * (1.0 - 1e-15));
+  st[r] = make_float4(n2d, ru(2.0 * sqrt(n2) * up), ru(2.0 * (sqrt(e2) * up + gL * sqrt(b2) * up) * up), 0.0f);
+  xn2[r] = n2d;
+}
+
+// per-tile maxima of A_x and B_x
+__global__ void k_tile_slack(uint64_t n, const float4 *__restrict__ st, float2 *__restrict__ ts)
+{
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t r0 = t * kRowTile;
+  if (r0 >= n) return;
+  const uint64_t r1 = r0 + kRowTile < n ? r0 + kRowTile : n;
+  float a = 0.0f, b = 0.0f;
+  for (uint64_t r = r0; r < r1; ++r) {
+    a = fmaxf(a, st[r].y);
+    b = fmaxf(b, st[r].z);
+  }
+  ts[t] = make_float2(a, b);
 }
 
 // queries -> BF16 + {|q|^2 (down), |q - q~| (up), |q~| (up), 0}
@@ -118,157 +145,199 @@ __global__ void k_query_bf16(uint32_t nq, uint32_t L, const float *__restrict__ 
   st[q] = make_float4(rd(n2 * (1.0 - 1e-15)), ru(sqrt(e2) * (1.0 + 1e-15)), ru(sqrt(b2) * (1.0 + 1e-15)), 0.0f);
 }
 
-// Filter: block = 4 waves x 64 queries; persistent over row tiles of 128
-// series staged in LDS.  MFMA 32x32x16 BF16: A = 32 queries x 16 dims
-// (lane l: query l%32, dims 8(l/32)..+8), B = 16 dims x 32 series (lane l:
-// series l%32, same dims), D[i][j] in lane (j + 32*((i/4)%2)), register
-// 4*(i/8) + i%4.
+typedef __attribute__((address_space(3))) void lds_void;
+
+// asynchronous copy of 16 bytes per lane from global memory into LDS at
+// lds + 16 * lane (lds is wave-uniform)
+__device__ __forceinline__ void load_lds16(const void *g, lds_void *lds)
+{
+  __builtin_amdgcn_global_load_lds(g, lds, 16, 0, 0);
+}
+__device__ __forceinline__ void load_lds4(const void *g, lds_void *lds)
+{
+  __builtin_amdgcn_global_load_lds(g, lds, 4, 0, 0);
+}
+
+// Filter.  Block = 8 waves x 64 queries (two 32-query tiles per wave) of
+// block group g (grid.y), the queries qperm[512 g ..]; it loops over the
+// row tiles listed for the group (tiles overlapping a leaf whose EAPCA bound
+// is within the k-th distance bound of one of its queries), double-buffered
+// in LDS by direct global->LDS loads.
 //
-// Block group g (grid.y) serves the queries qperm[256 g .. 256 g + 255] (the
-// batch sorted by first bound, so similar queries share a block) and visits
-// only the row tiles listed for it: tiles that overlap a leaf whose EAPCA
-// bound beats the k-th distance bound of at least one of its queries.
-template <int KS, bool HIST>
-__global__ void __launch_bounds__(256) k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb,
-                                                     const float4 *__restrict__ xst,
-                                                     const uint16_t *__restrict__ Qb,
-                                                     const float4 *__restrict__ qst,
-                                                     const uint32_t *__restrict__ bound,
-                                                     const uint32_t *__restrict__ qperm,
-                                                     const uint32_t *__restrict__ tiles, uint32_t tile_stride,
-                                                     const uint32_t *__restrict__ ntiles_g, uint32_t scap,
-                                                     uint64_t *__restrict__ surv, uint32_t *__restrict__ scnt,
-                                                     const float *__restrict__ thr0, float gamma,
-                                                     uint32_t *__restrict__ hist)
+// MFMA 32x32x16 BF16, D = A.B: A = 32 series x 16 dims from LDS (lane l:
+// series l%32, dims 8(l/32)..+8), B = 16 dims x 32 queries from registers
+// (lane l: query l%32, same dims); D[i][j] (series i, query j) sits in lane
+// j + 32((i/4)%2), register 4(i/8) + i%4.  So every lane owns one query per
+// query tile and 16 series per 32-row sub-tile.
+//
+// LDS tile layout: row r, 16-byte vector v stored at vector position
+// v ^ (r & SW) of the row (rows of L bf16, no padding): conflict-free MFMA
+// operand reads, and each wave's direct load fills 1 KB of consecutive LDS.
+//
+// HIST = false: pairs whose certified lower bound is within the query's
+// threshold are appended to its survivor list.  HIST = true: pairs whose
+// certified upper bound is within the threshold are counted in the query's
+// histogram (global atomics; only pairs passing the lower-bound pre-test
+// get there, which undercounts nothing that matters: an undercount only
+// loosens the bound).
+template <int KS, bool HIST, int W = filter_waves(KS)>
+__global__ void __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(8 / W, 8 / W)))
+k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const float *__restrict__ xn2g,
+              const float4 *__restrict__ xst, const float2 *__restrict__ tslack, const uint16_t *__restrict__ Qb,
+              const float4 *__restrict__ qst, const uint32_t *__restrict__ bound, const uint32_t *__restrict__ qperm,
+              const uint32_t *__restrict__ tiles, uint32_t tile_stride, const uint32_t *__restrict__ ntiles_g,
+              uint32_t scap, uint64_t *__restrict__ surv, uint32_t *__restrict__ scnt, const float *__restrict__ thr0,
+              uint32_t *__restrict__ hist)
 {
   constexpr int L = KS * 16;
-  constexpr int kStride = KS * 2 + 1;  // 16-byte vectors per LDS row (one of padding)
-  __shared__ bf16x8 xs[kRowTile * kStride];  // row-major tile
-  __shared__ float s_thr[kQPerBlock], s_qn2[kQPerBlock], s_qe[kQPerBlock], s_qb[kQPerBlock];
-  __shared__ uint8_t s_full[kQPerBlock];  // the query's survivor list is full
-  __shared__ uint32_t s_q[kQPerBlock];     // batch index of every slot
-  __shared__ float s_lo[HIST ? kQPerBlock : 1], s_inv[HIST ? kQPerBlock : 1];
-  __shared__ uint32_t s_hist[HIST ? kQPerBlock * kBf16Bins : 1];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t qblk = blockIdx.y * kQPerBlock;
-  // per-query constants
-  for (int i = threadIdx.x; i < kQPerBlock; i += 256) {
-    uint32_t slot = qblk + i;
-    uint32_t q = slot < nq ? qperm[slot] : 0xFFFFFFFFu;
-    s_q[i] = q;
-    if (slot < nq) {
-      float4 s = qst[q];
-      s_thr[i] = thr0[q];
-      s_full[i] = 0;
-      s_qn2[i] = s.x;
-      s_qe[i] = s.y;
-      s_qb[i] = s.z;
-      if (HIST) {
-        const uint32_t bb = bound[q];
-        if (bb >= kInfBits) {
-          s_thr[i] = -FLT_MAX;  // no finite bound: nothing to histogram
-        } else {
-          const HistRange r = hist_range(bb, thr0[q]);
-          s_lo[i] = r.lo;
-          s_inv[i] = r.inv;
-        }
-      }
-    } else {
-      s_thr[i] = -FLT_MAX;  // padding slot: never keeps (and is also excluded below)
-      s_full[i] = 1;
-      s_qn2[i] = 0.0f;
-      s_qe[i] = 0.0f;
-      s_qb[i] = 0.0f;
-    }
-  }
-  if (HIST)
-    for (int i = threadIdx.x; i < kQPerBlock * kBf16Bins; i += 256) s_hist[i] = 0;
-  __syncthreads();
-  // query operands in registers: 2 tiles x KS steps
-  bf16x8 qa[2][KS];
-  for (int t = 0; t < 2; ++t) {
-    uint32_t q = s_q[wave * kQPerWave + t * 32 + (lane & 31)];
-    const bf16x8 *src = reinterpret_cast<const bf16x8 *>(Qb + (uint64_t)(q < nq ? q : 0) * L);
+  constexpr int VPR = L / 8;                       // 16-byte vectors per row
+  constexpr int SW = (VPR % 8 == 0) ? 7 : 3;       // swizzle mask (VPR is a multiple of SW + 1)
+  constexpr int NCH = kRowTile * VPR / 64;         // 1 KB chunks per tile
+  constexpr int kWaves = W, kQPerBlock = W * kQPerWave;
+  static_assert(VPR % (SW + 1) == 0 && (kRowTile * VPR) % 64 == 0, "tile shape");
+  __shared__ __attribute__((aligned(16))) uint16_t xs[2][kRowTile * L];
+  __shared__ __attribute__((aligned(16))) float xn2s[2][kRowTile];
+  __shared__ uint8_t s_full[kQPerBlock];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5, l32 = lane & 31;
+  const uint32_t g = blockIdx.y;
+
+  // this lane's query of each query tile, and its constants
+  uint32_t qid[2];
+  float qn2[2], qe[2], qbn[2], thr[2], hlo[2], hinv[2];
+  bf16x8 qop[2][KS];
 #pragma unroll
-    for (int k = 0; k < KS; ++k) qa[t][k] = src[2 * k + (lane >> 5)];
-  }
-  const uint32_t my_tiles = ntiles_g[blockIdx.y];
-  const uint32_t *tlist = tiles + (uint64_t)blockIdx.y * tile_stride;
-  for (uint32_t ti = blockIdx.x; ti < my_tiles; ti += gridDim.x) {
-    const uint64_t r0 = (uint64_t)tlist[ti] * kRowTile;
-    const int rows = (int)(n - r0 < (uint64_t)kRowTile ? n - r0 : (uint64_t)kRowTile);
-    // stage the tile: rows x (L/8) vectors of 16 B
-    const bf16x8 *gsrc = reinterpret_cast<const bf16x8 *>(Xb + r0 * L);
-    for (int v = threadIdx.x; v < kRowTile * KS * 2; v += 256) {
-      int row = v / (KS * 2), col = v - row * (KS * 2);
-      xs[row * kStride + col] = row < rows ? gsrc[v] : bf16x8{};
+  for (int t = 0; t < 2; ++t) {
+    const uint32_t slot = g * kQPerBlock + wave * kQPerWave + t * 32 + l32;
+    const uint32_t q = slot < nq ? qperm[slot] : 0xFFFFFFFFu;
+    qid[t] = q;
+    const uint32_t qs = q < nq ? q : 0;
+    const float4 s = qst[qs];
+    qn2[t] = s.x;
+    qe[t] = s.y;
+    qbn[t] = s.z;
+    thr[t] = q < nq ? thr0[q] : -FLT_MAX;  // padding never passes
+    hlo[t] = 0.0f;
+    hinv[t] = 0.0f;
+    if (HIST && q < nq) {
+      const uint32_t bb = bound[q];
+      if (bb >= kInfBits) {
+        thr[t] = -FLT_MAX;  // no finite bound: nothing to histogram
+      } else {
+        const HistRange r = hist_range(bb, thr[t]);
+        hlo[t] = r.lo;
+        hinv[t] = r.inv;
+      }
     }
-    __syncthreads();
+    const bf16x8 *src = reinterpret_cast<const bf16x8 *>(Qb + (uint64_t)qs * L);
+#pragma unroll
+    for (int k = 0; k < KS; ++k) qop[t][k] = src[2 * k + half];
+  }
+  for (int i = threadIdx.x; i < kQPerBlock; i += kWaves * 64) s_full[i] = 0;
+
+  const uint32_t nt = ntiles_g[g];
+  const uint32_t my_n = blockIdx.x < nt ? (nt - blockIdx.x + gridDim.x - 1) / gridDim.x : 0;
+  const uint32_t *tlist = tiles + (uint64_t)g * tile_stride;
+
+  // issue the loads of the j-th tile of this block into buffer b
+  auto issue = [&](uint32_t j, int b) {
+    const uint64_t r0 = (uint64_t)tlist[blockIdx.x + j * gridDim.x] * kRowTile;
+    for (int c = wave; c < NCH; c += kWaves) {
+      const int lin = c * 64 + lane;     // LDS vector index of this lane
+      const int r = lin / VPR, p = lin - r * VPR;
+      const uint64_t gr = r0 + r < n ? r0 + r : n - 1;  // rows past the end: a copy of the last row
+      const uint16_t *src = Xb + gr * L + (uint32_t)((p ^ (r & SW)) * 8);
+      load_lds16(src, (lds_void *)(reinterpret_cast<char *>(&xs[b][0]) + c * 1024));
+    }
+    static_assert(kRowTile / 64 <= kWaves, "xn2 loads");
+    if (wave < kRowTile / 64) {
+      const uint64_t r = r0 + wave * 64 + lane;
+      load_lds4(xn2g + (r < n ? r : n - 1), (lds_void *)(reinterpret_cast<char *>(&xn2s[b][0]) + wave * 256));
+    }
+  };
+
+  if (my_n) issue(0, 0);
+  for (uint32_t j = 0; j < my_n; ++j) {
+    const int b = j & 1;
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's loads of tile j have landed
+    __syncthreads();                // ... and every wave's; tile j-1 is no longer read
+    if (j + 1 < my_n) issue(j + 1, b ^ 1);
+    const uint32_t tile = tlist[blockIdx.x + j * gridDim.x];
+    const uint64_t r0 = (uint64_t)tile * kRowTile;
+    const float2 tsl = tslack[tile];
+    // pre-test constant: xn2 - 2 s <= C  <=>  bound with the tile's largest slack <= threshold
+    float C[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) C[t] = (thr[t] - qn2[t]) + (qe[t] * tsl.x + qbn[t] * tsl.y + kEvalSlack);
+    const bf16x8 *xv = reinterpret_cast<const bf16x8 *>(&xs[b][0]);
 #pragma unroll 1
     for (int sub = 0; sub < kRowTile / 32; ++sub) {
       f32x16 acc0 = {}, acc1 = {};
-      const int row = sub * 32 + (lane & 31);
+      const int row = sub * 32 + l32;
 #pragma unroll
       for (int k = 0; k < KS; ++k) {
-        bf16x8 xb = xs[row * kStride + 2 * k + (lane >> 5)];
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qa[0][k], xb, acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qa[1][k], xb, acc1, 0, 0, 0);
+        const bf16x8 a = xv[row * VPR + ((2 * k + half) ^ (row & SW))];
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qop[0][k], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qop[1][k], acc1, 0, 0, 0);
       }
-      // epilogue: this lane holds series row (lane % 32) of the sub-tile;
-      // for a given register both half-waves hold one query each
-      const bool valid = row < rows;
-      const uint64_t grow = r0 + row;
-      const float4 xsv = valid ? xst[grow] : make_float4(0.f, 0.f, 0.f, 0.f);  // {|x|^2, |x|, |x-x~|, |x~|}
-      const int half = lane >> 5;
-      const uint64_t half_mask = half ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
+      // schedule: two operand reads ahead, then (2 MFMA, 1 read) -- keeps
+      // few operand registers live (the query operands take 8 KS)
+      __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
+      for (int k = 0; k < KS - 2; ++k) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+      // series of register r: sub*32 + 8(r/4) + 4 half + r%4
+      float4 x4[4];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int qi = 8 * (r >> 2) + 4 * half + (r & 3);  // query within the 32-tile
-          const int ql = wave * kQPerWave + t * 32 + qi;
-          const float s = t == 0 ? acc0[r] : acc1[r];
-          const float slack = 2.0f * (s_qe[ql] * xsv.y + s_qb[ql] * xsv.z + gamma * s_qb[ql] * xsv.w) + kEvalSlack;
-          if (HIST) {
-            // upper bound of d2: count it in the query's histogram
-            const float ub = (s_qn2[ql] + xsv.x) - 2.0f * s + slack;
-            if (valid && ub <= s_thr[ql]) {
-              const float lo = s_lo[ql];
-              const int j = ub < lo ? 0 : min(kBf16Bins - 1, 1 + (int)fminf((ub - lo) * s_inv[ql], (float)kBf16Bins));
-              atomicAdd(&s_hist[ql * kBf16Bins + j], 1u);
-            }
-            continue;
-          }
-          const float lb = (s_qn2[ql] + xsv.x) - 2.0f * s - slack;
-          const uint32_t q = s_q[ql];
-          const bool keep = valid && q < nq && lb <= s_thr[ql] && !s_full[ql];
-          const uint64_t m = __ballot(keep) & half_mask;
-          if (m) {
-            // one counter atomic per half-wave
-            const int leader = __ffsll((long long)m) - 1;
-            const uint32_t cnt = (uint32_t)__popcll(m);
-            uint32_t base = 0;
-            if (lane == leader) {
-              base = atomicAdd(&scnt[q], cnt);
-              if (base + cnt > scap) s_full[ql] = 1;
-            }
-            base = __shfl(base, leader);
-            if (keep) {
-              const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1) & half_mask);
-              if (slot < scap)
-                surv[(uint64_t)q * scap + slot] = ((uint64_t)__float_as_uint(fmaxf(lb, 0.0f)) << 32) | (uint32_t)grow;
+      for (int gi = 0; gi < 4; ++gi)
+        x4[gi] = *reinterpret_cast<const float4 *>(&xn2s[b][sub * 32 + 8 * gi + 4 * half]);
+      bool hit = false;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const float xr = x4[r >> 2][r & 3];
+        hit |= fmaf(acc0[r], -2.0f, xr) <= C[0];
+        hit |= fmaf(acc1[r], -2.0f, xr) <= C[1];
+      }
+      if (__ballot(hit)) {
+        // rare: pairs that passed the pre-test, with their own slack
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const float s = t == 0 ? acc0[r] : acc1[r];
+            const float xr = x4[r >> 2][r & 3];
+            if (fmaf(s, -2.0f, xr) <= C[t]) {
+              const uint64_t grow = r0 + sub * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+              if (grow < n) {
+                const float4 xsv = xst[grow];
+                const float slack = qe[t] * xsv.y + qbn[t] * xsv.z + kEvalSlack;
+                const float base = (qn2[t] + xsv.x) - 2.0f * s;
+                if (HIST) {
+                  const float ub = base + slack;
+                  if (ub <= thr[t]) {
+                    const float lo = hlo[t];
+                    const int bin = ub < lo ? 0 : min(kBf16Bins - 1, 1 + (int)fminf((ub - lo) * hinv[t], (float)kBf16Bins));
+                    atomicAdd(&hist[(uint64_t)qid[t] * kBf16Bins + bin], 1u);
+                  }
+                } else {
+                  const float lb = base - slack;
+                  const int sl = wave * kQPerWave + t * 32 + l32;
+                  if (lb <= thr[t] && !s_full[sl]) {
+                    const uint32_t pos = atomicAdd(&scnt[qid[t]], 1u);
+                    if (pos < scap)
+                      surv[(uint64_t)qid[t] * scap + pos] =
+                          ((uint64_t)__float_as_uint(fmaxf(lb, 0.0f)) << 32) | (uint32_t)grow;
+                    else
+                      s_full[sl] = 1;
+                  }
+                }
+              }
             }
           }
         }
       }
-    }
-    __syncthreads();
-  }
-  if (HIST) {
-    for (int i = threadIdx.x; i < kQPerBlock * kBf16Bins; i += 256) {
-      const uint32_t v = s_hist[i];
-      const uint32_t q = s_q[i / kBf16Bins];
-      if (v && q < nq) atomicAdd(&hist[(uint64_t)q * kBf16Bins + i % kBf16Bins], v);
     }
   }
 }
@@ -301,37 +370,43 @@ __global__ void k_bf16_tau(uint32_t nq, const uint32_t *__restrict__ kq, const u
 
 }  // namespace
 
+// queries per filter block (block group) for series of length L
+static uint32_t bf16_qpb(uint32_t L) { return (uint32_t)filter_waves((int)L / 16) * kQPerWave; }
+
 uint64_t bf16_scratch_per_query(uint32_t L, uint64_t nleaf, uint64_t ntiles)
 {
   return (uint64_t)kBf16SurvivorCap * 8 + 4 + (uint64_t)L * 2 + 16 + 16 + 4 + 4ull * kBf16Bins +
-         (nleaf + 4 * ntiles) / kQPerBlock + 16;
+         (nleaf + 4 * ntiles) / bf16_qpb(L) + 16;
 }
 
+// one resident block per CU, split over the block groups
 static uint32_t bf16_grid_x(uint32_t groups)
 {
   int d = 0, cus = 0;
   (void)hipGetDevice(&d);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d);
-  uint32_t total = (uint32_t)std::max(cus, 1) * 2;
+  const uint32_t total = (uint32_t)std::max(cus, 1);
   return std::max<uint32_t>(1, total / std::max<uint32_t>(groups, 1));
 }
 
-void launch_bf16_filter(hipStream_t st, uint32_t L, uint64_t n, uint32_t nq, const uint16_t *Xb, const float4 *xst,
-                        const uint16_t *Qb, const float4 *qst, const uint32_t *bound, const uint32_t *qperm,
-                        const uint32_t *tiles, uint32_t tile_stride, const uint32_t *ntiles_g, uint32_t scap,
-                        uint64_t *surv, uint32_t *scnt, const float *thr0, uint32_t *hist)
+void launch_bf16_filter(hipStream_t st, const DevIndex &dv, uint32_t nq, const uint16_t *Qb, const float4 *qst,
+                        const uint32_t *bound, const uint32_t *qperm, const uint32_t *tiles,
+                        const uint32_t *ntiles_g, uint32_t scap, uint64_t *surv, uint32_t *scnt, const float *thr0,
+                        uint32_t *hist)
 {
-  const uint32_t groups = (nq + kQPerBlock - 1) / kQPerBlock;
-  const float gamma = 2.0f * (float)L * 5.9604645e-08f;
+  const uint32_t qpb = bf16_qpb(dv.L);
+  const uint32_t groups = (nq + qpb - 1) / qpb;
   dim3 grid(bf16_grid_x(groups), groups);
-#define HKG_BF16_LAUNCH(KS)                                                                                  \
-  if (hist)                                                                                                  \
-    hipLaunchKernelGGL((k_bf16_filter<KS, true>), grid, dim3(256), 0, st, n, nq, Xb, xst, Qb, qst, bound,   \
-                       qperm, tiles, tile_stride, ntiles_g, scap, surv, scnt, thr0, gamma, hist);            \
-  else                                                                                                       \
-    hipLaunchKernelGGL((k_bf16_filter<KS, false>), grid, dim3(256), 0, st, n, nq, Xb, xst, Qb, qst, bound,  \
-                       qperm, tiles, tile_stride, ntiles_g, scap, surv, scnt, thr0, gamma, hist)
-  switch (L) {
+#define HKG_BF16_LAUNCH(KS)                                                                                       \
+  if (hist)                                                                                                       \
+    hipLaunchKernelGGL((k_bf16_filter<KS, true>), grid, dim3(filter_waves(KS) * 64), 0, st, dv.n, nq, dv.Xb,     \
+                       dv.xn2, dv.xst, dv.tile_slack, Qb, qst, bound, qperm, tiles, dv.ntiles, ntiles_g, scap,   \
+                       surv, scnt, thr0, hist);                                                                   \
+  else                                                                                                            \
+    hipLaunchKernelGGL((k_bf16_filter<KS, false>), grid, dim3(filter_waves(KS) * 64), 0, st, dv.n, nq, dv.Xb,    \
+                       dv.xn2, dv.xst, dv.tile_slack, Qb, qst, bound, qperm, tiles, dv.ntiles, ntiles_g, scap,   \
+                       surv, scnt, thr0, hist)
+  switch (dv.L) {
   case 96: HKG_BF16_LAUNCH(6); break;
   case 128: HKG_BF16_LAUNCH(8); break;
   default: HKG_BF16_LAUNCH(16); break;
@@ -352,7 +427,7 @@ __global__ void k_bf16_thr_init(uint32_t nq, const uint32_t *__restrict__ bound,
 
 // leaves some query of a block group still needs: EAPCA bound within the
 // query's current k-th distance bound
-__global__ void k_group_leaf_need(uint32_t nq, uint32_t nleaf, const float *__restrict__ lb,
+__global__ void k_group_leaf_need(uint32_t nq, uint32_t qpb, uint32_t nleaf, const float *__restrict__ lb,
                                   const uint32_t *__restrict__ bound, const uint32_t *__restrict__ qperm,
                                   uint8_t *__restrict__ need)
 {
@@ -360,8 +435,8 @@ __global__ void k_group_leaf_need(uint32_t nq, uint32_t nleaf, const float *__re
   const uint32_t g = blockIdx.y;
   if (leaf >= nleaf) return;
   uint8_t nd = 0;
-  const uint32_t s0 = g * kQPerBlock;
-  const uint32_t s1 = min(nq, s0 + kQPerBlock);
+  const uint32_t s0 = g * qpb;
+  const uint32_t s1 = min(nq, s0 + qpb);
   for (uint32_t sl = s0; sl < s1 && !nd; ++sl) {
     const uint32_t q = qperm[sl];
     const uint32_t bb = bound[q];
@@ -393,13 +468,15 @@ __global__ void k_group_tiles(uint32_t ntiles, uint32_t nleaf, const uint32_t *_
   }
 }
 
-// batch order by first bound: keys = bound bits, values = batch index
-__global__ void k_iota_bound(uint32_t nq, const uint32_t *__restrict__ bound, uint32_t *__restrict__ keys,
-                             uint32_t *__restrict__ vals)
+// batch order by best leaf (leaves are in tree order, so queries of one
+// block group need overlapping sets of leaves): keys = best leaf, values =
+// batch index
+__global__ void k_iota_best(uint32_t nq, const uint32_t *__restrict__ best, uint32_t *__restrict__ keys,
+                            uint32_t *__restrict__ vals)
 {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nq) {
-    keys[i] = bound[i];
+    keys[i] = best[(uint64_t)i * kBestLeaves];
     vals[i] = i;
   }
 }
@@ -409,10 +486,13 @@ hkg_status Index::make_bf16()
   std::lock_guard<std::mutex> lock(mu_bf16);
   if (bf16_ready) return HKG_OK;
   // an earlier attempt that failed part-way
-  for (void *p : {(void *)dev.Xb, (void *)dev.xst, (void *)dev.tile_leaf0, (void *)dev.tile_leaf1})
+  for (void *p : {(void *)dev.Xb, (void *)dev.xst, (void *)dev.xn2, (void *)dev.tile_slack, (void *)dev.tile_leaf0,
+                  (void *)dev.tile_leaf1})
     if (p) (void)hipFree(p);
   dev.Xb = nullptr;
   dev.xst = nullptr;
+  dev.xn2 = nullptr;
+  dev.tile_slack = nullptr;
   dev.tile_leaf0 = dev.tile_leaf1 = nullptr;
   if (dev.L % 16 != 0 || (dev.L != 96 && dev.L != 128 && dev.L != 256)) {
     set_error("the BF16 filter supports series of length 96, 128 and 256");
@@ -426,16 +506,19 @@ hkg_status Index::make_bf16()
     set_error("no device memory for the BF16 copy of the series");
     return HKG_ENOMEM;
   }
-  if (hipMalloc((void **)&dev.xst, n * sizeof(float4)) != hipSuccess) {
+  const uint64_t ntiles_all = (n + kRowTile - 1) / kRowTile;
+  if (hipMalloc((void **)&dev.xst, n * sizeof(float4)) != hipSuccess ||
+      hipMalloc((void **)&dev.xn2, n * sizeof(float)) != hipSuccess ||
+      hipMalloc((void **)&dev.tile_slack, ntiles_all * sizeof(float2)) != hipSuccess) {
     (void)hipGetLastError();
-    (void)hipFree(dev.Xb);
-    dev.Xb = nullptr;
     set_error("no device memory for the BF16 row norms");
-    return HKG_ENOMEM;
+    return HKG_ENOMEM;  // freed by the next attempt or the index
   }
-  dev.device_bytes += n * dev.L * 2 + n * sizeof(float4);
+  dev.device_bytes += n * dev.L * 2 + n * sizeof(float4) + n * sizeof(float) + ntiles_all * sizeof(float2);
   hipLaunchKernelGGL(k_make_bf16, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, n, dev.L, dev.X, dev.Xb,
-                     dev.xst);
+                     dev.xst, dev.xn2);
+  hipLaunchKernelGGL(k_tile_slack, dim3((uint32_t)((ntiles_all + 255) / 256)), dim3(256), 0, stream, n, dev.xst,
+                     dev.tile_slack);
   HKG_HIP_TRY(hipGetLastError());
   HKG_HIP_TRY(hipStreamSynchronize(stream));
   if (dev.has_tree) {
@@ -483,7 +566,8 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
   BufLease lease{this, b};  // handed to the caller on success
   const uint32_t cap_override = call_settings().cap_override;
   const uint32_t want_cap = cap_override ? std::max<uint32_t>(cap_override, 16) : kBf16SurvivorCap;
-  const uint32_t groups_max = (b->B + kQPerBlock - 1) / kQPerBlock;
+  const uint32_t qpb = bf16_qpb(L);
+  const uint32_t groups_max = (b->B + qpb - 1) / qpb;
   if (b->bf_surv && b->bf_cap != want_cap) b->free_bf16();
   if (!b->bf_surv) {
     b->bf_cap = want_cap;
@@ -526,9 +610,9 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
                    b->cand_cnt, b->bound, b->counters);
   if (pf) pf->mark(st, HKG_STAGE_APPROX);
   // batch order by first bound, leaves and row tiles each block group needs
-  const uint32_t groups = (nq + kQPerBlock - 1) / kQPerBlock;
+  const uint32_t groups = (nq + qpb - 1) / qpb;
   uint32_t *skeys = b->qperm + b->B, *sidx = b->qperm + 2 * b->B;
-  hipLaunchKernelGGL(k_iota_bound, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->bound, skeys, sidx);
+  hipLaunchKernelGGL(k_iota_best, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->best, skeys, sidx);
   HKG_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(b->bf_sort_tmp, b->bf_sort_bytes, skeys, b->qperm + 3 * b->B, sidx,
                                                  b->qperm, (int)nq, 0, 32, st));
   // row tiles each block group needs under the current bounds, and the
@@ -536,7 +620,7 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
   float *thr0 = reinterpret_cast<float *>(b->thr);
   auto plan_tiles = [&]() {
     launch_fill(st, b->ntiles_g, groups, 0);
-    hipLaunchKernelGGL(k_group_leaf_need, dim3((nleaf + 255) / 256, groups), dim3(256), 0, st, nq, nleaf, b->lb,
+    hipLaunchKernelGGL(k_group_leaf_need, dim3((nleaf + 255) / 256, groups), dim3(256), 0, st, nq, qpb, nleaf, b->lb,
                        b->bound, b->qperm, b->need);
     hipLaunchKernelGGL(k_group_tiles, dim3((dev.ntiles + 255) / 256, groups), dim3(256), 0, st, dev.ntiles, nleaf,
                        dev.tile_leaf0, dev.tile_leaf1, b->need, b->tiles, b->ntiles_g);
@@ -549,8 +633,8 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
     // pass 1: histogram of the BF16 upper bounds -> a k-th distance bound
     // within about one bin of the k-th smallest upper bound
     launch_fill(st, b->hist, nq * kBf16Bins, 0);
-    launch_bf16_filter(st, L, dev.n, nq, dev.Xb, dev.xst, b->Qb, b->qst, b->bound, b->qperm, b->tiles, dev.ntiles,
-                       b->ntiles_g, b->bf_cap, b->bf_surv, b->bf_cnt, thr0, b->hist);
+    launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, b->bf_cap, b->bf_surv,
+                       b->bf_cnt, thr0, b->hist);
     if (ctr) {
       nt_pass1.resize(groups);
       HKG_HIP_TRY(hipMemcpyAsync(nt_pass1.data(), b->ntiles_g, groups * 4, hipMemcpyDeviceToHost, st));
@@ -560,8 +644,8 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
     if (pf) pf->mark(st, HKG_STAGE_BOUND);
   }
   // pass 2: collect the survivors of the certified lower bound
-  launch_bf16_filter(st, L, dev.n, nq, dev.Xb, dev.xst, b->Qb, b->qst, b->bound, b->qperm, b->tiles, dev.ntiles,
-                     b->ntiles_g, b->bf_cap, b->bf_surv, b->bf_cnt, thr0, nullptr);
+  launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, b->bf_cap, b->bf_surv,
+                     b->bf_cnt, thr0, nullptr);
   if (pf) pf->mark(st, HKG_STAGE_REFINE);
   launch_bf16_refine(st, nq, dev.X, L, b->Q, b->kq, b->kmax, dev.ids, b->cap, b->cand, b->cand_cnt, b->bf_cap,
                      b->bf_surv, b->bf_cnt, thr0, b->bound, idmap64, b->out_ids, b->out_d2, overflow, b->counters);
@@ -575,7 +659,7 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
     HKG_HIP_TRY(hipStreamSynchronize(st));
     // (query, series) pairs and row tiles the filter passes went through
     for (uint32_t g = 0; g < groups; ++g) {
-      uint64_t qn = std::min<uint32_t>(kQPerBlock, nq - g * kQPerBlock);
+      uint64_t qn = std::min<uint32_t>(qpb, nq - g * qpb);
       const uint64_t t1 = nt_pass1.empty() ? 0 : nt_pass1[g];
       ctr->v[kCtrBf16] += (unsigned long long)(nt[g] + t1) * kRowTile * qn;
       ctr->bf16_rows += (uint64_t)nt[g] * kRowTile;

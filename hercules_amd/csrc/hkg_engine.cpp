@@ -123,6 +123,8 @@ Index::~Index()
   dfree(dev.seg_ind);
   dfree(dev.Xb);
   dfree(dev.xst);
+  dfree(dev.xn2);
+  dfree(dev.tile_slack);
   dfree(dev.tile_leaf0);
   dfree(dev.tile_leaf1);
   dfree(idmap64);
