@@ -27,6 +27,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense
 
 
 def log(msg: str) -> None:
@@ -53,6 +54,8 @@ def parse() -> argparse.Namespace:
     p.add_argument("--serve-qps", type=float, default=0.0,
                    help="offered load of the serving leg (0: half the measured throughput)")
     p.add_argument("--serve-requests", type=int, default=4096, help="requests of the serving leg (0: skip)")
+    p.add_argument("--calibrate-copy-mib", type=int, default=0,
+                   help="one device-to-device copy of this size first (reference dispatch for counter runs)")
     return p.parse_args()
 
 
@@ -87,6 +90,20 @@ def cpu_baseline(args, Q: np.ndarray, kq: np.ndarray) -> dict:
             "sample": f"reference PSCAN ({threads - 1} scan workers + 1 reader) over the first {m} rows of the "
                       f"same dataset, {nq} of the benchmark's queries (k={k}), {wall:.2f} s; queries/s scaled "
                       f"by {m}/{args.n}"}
+
+
+def traffic_bytes(kernel: str):
+    """HBM bytes per launch of a kernel at this workload, from the counter
+    runs summarised in profiles/ (FETCH_SIZE + WRITE_SIZE, scaled by the
+    same counters on a copy of known size); None when not measured."""
+    path = os.path.join(REPO, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 2 and parts[0] == kernel:
+            return float(parts[1])
+    return None
 
 
 def serving_leg(hkg, ix, args, Qh: np.ndarray, kq: np.ndarray, qps: float) -> dict:
@@ -152,6 +169,12 @@ def main() -> int:
     from hercules_amd import hkg
 
     L, N, B = args.length, args.n, args.batch
+    if args.calibrate_copy_mib:
+        nb = args.calibrate_copy_mib << 20
+        ca, cb = hkg.DeviceArray(nb), hkg.DeviceArray(nb)
+        hkg._check(hkg.lib().hkg_memcpy(cb.ptr, ca.ptr, nb, 3))
+        del ca, cb
+        log(f"calibration copy: {args.calibrate_copy_mib} MiB device to device")
     t0 = time.perf_counter()
     dX = hkg.DeviceArray(N * L * 4)
     hkg.gen_series_device(0, 1, 0, N, L, dX)
@@ -220,6 +243,11 @@ def main() -> int:
     avg_ms = stage_ms[dom] / launches
     bytes_per_launch = stage_bytes[dom] / launches
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    mfma = args.path == "bf16" and dom == "refine"
+    if mfma:
+        # the BF16 filter is a contraction: 2 L N B FLOPs per batch pass
+        flops_per_launch = 2.0 * L * N * B
+        achieved_tf = flops_per_launch / (avg_ms * 1e-3) / 1e12
     kernel_names = {"refine": "k_filter_refine", "approx": "k_scan_best", "eapca": "k_eapca", "scan": "k_scan",
                     "topk": "k_select", "collect": "k_collect", "prep": "k_prep", "select_leaf": "k_best_leaf"}
     if args.path == "bf16":
@@ -275,9 +303,16 @@ def main() -> int:
                                f"k ~ U{{1..{args.kmax}}}, path {args.path}",
                    "n_series": N, "length": L, "batch": B, "leaf_size": args.leaf, "n_leaves": info["n_leaves"],
                    "build_s": round(build_s, 2)},
-        "roofline": {"bound": "hbm", "kernel": kernel_names.get(dom, dom), "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch},
+        "roofline": ({"bound": "mfma", "kernel": kernel_names.get(dom, dom), "achieved": achieved_tf,
+                      "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved_tf / MFMA_BF16_PEAK_TFLOPS,
+                      "traffic": traffic_bytes(kernel_names.get(dom, dom)), "avg_launch_ms": avg_ms,
+                      "flops_per_launch": flops_per_launch, "tile_bytes_per_launch": bytes_per_launch,
+                      "tile_bytes_gbs": achieved}
+                     if mfma else
+                     {"bound": "hbm", "kernel": kernel_names.get(dom, dom), "achieved": achieved,
+                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                      "traffic": traffic_bytes(kernel_names.get(dom, dom)), "avg_launch_ms": avg_ms,
+                      "bytes_per_launch": bytes_per_launch}),
         "stages_ms": {k: round(v / steps, 3) for k, v in stage_ms.items() if v > 0},
         "series_refined_per_query": refined / (steps * B),
         "series_lb_checked_per_query": checked / (steps * B),

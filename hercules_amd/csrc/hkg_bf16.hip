@@ -44,7 +44,7 @@ constexpr int kQPerWave = 64;              // two 32-query MFMA tiles
 // so L = 256 runs 4 waves (one per SIMD, up to 512 registers with the
 // accumulation registers) and shorter series 8 (two per SIMD).
 __host__ __device__ constexpr int filter_waves(int KS) { return KS >= 16 ? 4 : 8; }
-constexpr int kRowTile = 128;              // series per LDS tile
+constexpr int kRowTile = 128;              // series per LDS tile (double-buffered)
 constexpr float kEvalSlack = 1.0e-3f;      // FP32 evaluation of the bound
 constexpr uint32_t kBf16SurvivorCap = 524288;  // survivors kept per query (8 B each)
 constexpr float kHistLo = 0.6f;            // histogram range: [0.6 bound, threshold]
@@ -234,9 +234,14 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   }
   for (int i = threadIdx.x; i < kQPerBlock; i += kWaves * 64) s_full[i] = 0;
 
-  const uint32_t nt = ntiles_g[g];
+  // every block group walks the one tile list of the batch (tile_stride =
+  // 0) or its own; with the shared list, block x of every group reads the
+  // same tiles in the same order on the same XCD, so one of them brings a
+  // tile from HBM and the others find it in the caches
+  const uint32_t gl = tile_stride ? g : 0;
+  const uint32_t nt = ntiles_g[gl];
   const uint32_t my_n = blockIdx.x < nt ? (nt - blockIdx.x + gridDim.x - 1) / gridDim.x : 0;
-  const uint32_t *tlist = tiles + (uint64_t)g * tile_stride;
+  const uint32_t *tlist = tiles + (uint64_t)gl * tile_stride;
 
   // issue the loads of the j-th tile of this block into buffer b
   auto issue = [&](uint32_t j, int b) {
@@ -255,6 +260,88 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     }
   };
 
+  // MFMAs of one 32-row sub-tile (rows crow.. of buffer b)
+  auto mfma_sub = [&](int b, int crow, f32x16 &a0, f32x16 &a1) {
+    const bf16x8 *xv = reinterpret_cast<const bf16x8 *>(&xs[b][0]);
+    const int row = crow + l32;
+    a0 = f32x16{};
+    a1 = f32x16{};
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const bf16x8 a = xv[row * VPR + ((2 * k + half) ^ (row & SW))];
+      a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qop[0][k], a0, 0, 0, 0);
+      a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qop[1][k], a1, 0, 0, 0);
+    }
+  };
+  // pre-test of one sub-tile's pairs: series of register r is
+  // crow + 8(r/4) + 4 half + r%4 of the tile
+  auto pretest = [&](int b, int crow, const f32x16 &a0, const f32x16 &a1, float C0, float C1, float4 (&x4)[4]) {
+#pragma unroll
+    for (int gi = 0; gi < 4; ++gi)
+      x4[gi] = *reinterpret_cast<const float4 *>(&xn2s[b][crow + 8 * gi + 4 * half]);
+    bool hit = false;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const float xr = x4[r >> 2][r & 3];
+      hit |= fmaf(a0[r], -2.0f, xr) <= C0;
+      hit |= fmaf(a1[r], -2.0f, xr) <= C1;
+    }
+    return hit;
+  };
+  // rare: pairs that passed the pre-test, with their own slack
+  auto slow = [&](uint64_t r0, const f32x16 &a0, const f32x16 &a1, float C0, float C1, const float4 (&x4)[4]) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const float sv = t == 0 ? a0[r] : a1[r];
+        const float xr = x4[r >> 2][r & 3];
+        if (fmaf(sv, -2.0f, xr) <= (t == 0 ? C0 : C1)) {
+          const uint64_t grow = r0 + 8 * (r >> 2) + 4 * half + (r & 3);
+          if (grow < n) {
+            const float4 xsv = xst[grow];
+            const float slack = qe[t] * xsv.y + qbn[t] * xsv.z + kEvalSlack;
+            const float base = (qn2[t] + xsv.x) - 2.0f * sv;
+            if (HIST) {
+              const float ub = base + slack;
+              if (ub <= thr[t]) {
+                const float lo = hlo[t];
+                const int bin = ub < lo ? 0 : min(kBf16Bins - 1, 1 + (int)fminf((ub - lo) * hinv[t], (float)kBf16Bins));
+                atomicAdd(&hist[(uint64_t)qid[t] * kBf16Bins + bin], 1u);
+              }
+            } else {
+              const float lb = base - slack;
+              const int sl = wave * kQPerWave + t * 32 + l32;
+              if (lb <= thr[t] && !s_full[sl]) {
+                const uint32_t pos = atomicAdd(&scnt[qid[t]], 1u);
+                if (pos < scap)
+                  surv[(uint64_t)qid[t] * scap + pos] =
+                      ((uint64_t)__float_as_uint(fmaxf(lb, 0.0f)) << 32) | (uint32_t)grow;
+                else
+                  s_full[sl] = 1;
+              }
+            }
+          }
+        }
+      }
+    }
+  };
+  // schedule of one MFMA block with the previous sub-tile's pre-test: two
+  // operand reads ahead, then (2 MFMA, 1 read, a few VALU) -- few operand
+  // registers live (the query operands take 8 KS) and the pre-test's VALU
+  // work fills the MFMA shadow
+  auto interleave = [&]() {
+    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+#pragma unroll
+    for (int k = 0; k < KS - 2; ++k) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+  };
+
+  static_assert(kRowTile == 128, "four sub-tiles per tile below");
   if (my_n) issue(0, 0);
   for (uint32_t j = 0; j < my_n; ++j) {
     const int b = j & 1;
@@ -263,82 +350,29 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     if (j + 1 < my_n) issue(j + 1, b ^ 1);
     const uint32_t tile = tlist[blockIdx.x + j * gridDim.x];
     const uint64_t r0 = (uint64_t)tile * kRowTile;
-    const float2 tsl = tslack[tile];
     // pre-test constant: xn2 - 2 s <= C  <=>  bound with the tile's largest slack <= threshold
-    float C[2];
-#pragma unroll
-    for (int t = 0; t < 2; ++t) C[t] = (thr[t] - qn2[t]) + (qe[t] * tsl.x + qbn[t] * tsl.y + kEvalSlack);
-    const bf16x8 *xv = reinterpret_cast<const bf16x8 *>(&xs[b][0]);
-#pragma unroll 1
-    for (int sub = 0; sub < kRowTile / 32; ++sub) {
-      f32x16 acc0 = {}, acc1 = {};
-      const int row = sub * 32 + l32;
-#pragma unroll
-      for (int k = 0; k < KS; ++k) {
-        const bf16x8 a = xv[row * VPR + ((2 * k + half) ^ (row & SW))];
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qop[0][k], acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qop[1][k], acc1, 0, 0, 0);
-      }
-      // schedule: two operand reads ahead, then (2 MFMA, 1 read) -- keeps
-      // few operand registers live (the query operands take 8 KS)
-      __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
-#pragma unroll
-      for (int k = 0; k < KS - 2; ++k) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
-        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-      }
-      __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
-      // series of register r: sub*32 + 8(r/4) + 4 half + r%4
-      float4 x4[4];
-#pragma unroll
-      for (int gi = 0; gi < 4; ++gi)
-        x4[gi] = *reinterpret_cast<const float4 *>(&xn2s[b][sub * 32 + 8 * gi + 4 * half]);
-      bool hit = false;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const float xr = x4[r >> 2][r & 3];
-        hit |= fmaf(acc0[r], -2.0f, xr) <= C[0];
-        hit |= fmaf(acc1[r], -2.0f, xr) <= C[1];
-      }
-      if (__ballot(hit)) {
-        // rare: pairs that passed the pre-test, with their own slack
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-#pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            const float s = t == 0 ? acc0[r] : acc1[r];
-            const float xr = x4[r >> 2][r & 3];
-            if (fmaf(s, -2.0f, xr) <= C[t]) {
-              const uint64_t grow = r0 + sub * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
-              if (grow < n) {
-                const float4 xsv = xst[grow];
-                const float slack = qe[t] * xsv.y + qbn[t] * xsv.z + kEvalSlack;
-                const float base = (qn2[t] + xsv.x) - 2.0f * s;
-                if (HIST) {
-                  const float ub = base + slack;
-                  if (ub <= thr[t]) {
-                    const float lo = hlo[t];
-                    const int bin = ub < lo ? 0 : min(kBf16Bins - 1, 1 + (int)fminf((ub - lo) * hinv[t], (float)kBf16Bins));
-                    atomicAdd(&hist[(uint64_t)qid[t] * kBf16Bins + bin], 1u);
-                  }
-                } else {
-                  const float lb = base - slack;
-                  const int sl = wave * kQPerWave + t * 32 + l32;
-                  if (lb <= thr[t] && !s_full[sl]) {
-                    const uint32_t pos = atomicAdd(&scnt[qid[t]], 1u);
-                    if (pos < scap)
-                      surv[(uint64_t)qid[t] * scap + pos] =
-                          ((uint64_t)__float_as_uint(fmaxf(lb, 0.0f)) << 32) | (uint32_t)grow;
-                    else
-                      s_full[sl] = 1;
-                  }
-                }
-              }
-            }
-          }
-        }
-      }
-    }
+    const float2 tsl = tslack[tile];
+    const float C0 = (thr[0] - qn2[0]) + (qe[0] * tsl.x + qbn[0] * tsl.y + kEvalSlack);
+    const float C1 = (thr[1] - qn2[1]) + (qe[1] * tsl.x + qbn[1] * tsl.y + kEvalSlack);
+    f32x16 A0, A1, B0, B1;
+    float4 xa[4], xb[4];
+    bool ha, hb;
+    // sub-tiles 0..3, the MFMAs of one beside the pre-test of the previous
+    mfma_sub(b, 0, A0, A1);
+    mfma_sub(b, 32, B0, B1);
+    ha = pretest(b, 0, A0, A1, C0, C1, xa);
+    interleave();
+    if (__ballot(ha)) slow(r0, A0, A1, C0, C1, xa);
+    mfma_sub(b, 64, A0, A1);
+    hb = pretest(b, 32, B0, B1, C0, C1, xb);
+    interleave();
+    if (__ballot(hb)) slow(r0 + 32, B0, B1, C0, C1, xb);
+    mfma_sub(b, 96, B0, B1);
+    ha = pretest(b, 64, A0, A1, C0, C1, xa);
+    interleave();
+    if (__ballot(ha)) slow(r0 + 64, A0, A1, C0, C1, xa);
+    hb = pretest(b, 96, B0, B1, C0, C1, xb);
+    if (__ballot(hb)) slow(r0 + 96, B0, B1, C0, C1, xb);
   }
 }
 
@@ -379,32 +413,36 @@ uint64_t bf16_scratch_per_query(uint32_t L, uint64_t nleaf, uint64_t ntiles)
          (nleaf + 4 * ntiles) / bf16_qpb(L) + 16;
 }
 
-// one resident block per CU, split over the block groups
+// one resident block per CU, split over the block groups; a multiple of 8
+// (the XCD count) so that block x of every group lands on the same XCD
+// (blocks go to XCDs round-robin by linear id x + grid.x * y)
 static uint32_t bf16_grid_x(uint32_t groups)
 {
   int d = 0, cus = 0;
   (void)hipGetDevice(&d);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d);
-  const uint32_t total = (uint32_t)std::max(cus, 1);
-  return std::max<uint32_t>(1, total / std::max<uint32_t>(groups, 1));
+  const uint32_t per = (uint32_t)std::max(cus, 1) / std::max<uint32_t>(groups, 1);
+  return per >= 8 ? per / 8 * 8 : std::max<uint32_t>(per, 1);
 }
 
+// shared_list: every block group walks list 0 of tiles / ntiles_g
 void launch_bf16_filter(hipStream_t st, const DevIndex &dv, uint32_t nq, const uint16_t *Qb, const float4 *qst,
                         const uint32_t *bound, const uint32_t *qperm, const uint32_t *tiles,
-                        const uint32_t *ntiles_g, uint32_t scap, uint64_t *surv, uint32_t *scnt, const float *thr0,
-                        uint32_t *hist)
+                        const uint32_t *ntiles_g, bool shared_list, uint32_t scap, uint64_t *surv, uint32_t *scnt,
+                        const float *thr0, uint32_t *hist)
 {
+  const uint32_t stride = shared_list ? 0 : dv.ntiles;
   const uint32_t qpb = bf16_qpb(dv.L);
   const uint32_t groups = (nq + qpb - 1) / qpb;
   dim3 grid(bf16_grid_x(groups), groups);
 #define HKG_BF16_LAUNCH(KS)                                                                                       \
   if (hist)                                                                                                       \
     hipLaunchKernelGGL((k_bf16_filter<KS, true>), grid, dim3(filter_waves(KS) * 64), 0, st, dv.n, nq, dv.Xb,     \
-                       dv.xn2, dv.xst, dv.tile_slack, Qb, qst, bound, qperm, tiles, dv.ntiles, ntiles_g, scap,   \
+                       dv.xn2, dv.xst, dv.tile_slack, Qb, qst, bound, qperm, tiles, stride, ntiles_g, scap,       \
                        surv, scnt, thr0, hist);                                                                   \
   else                                                                                                            \
     hipLaunchKernelGGL((k_bf16_filter<KS, false>), grid, dim3(filter_waves(KS) * 64), 0, st, dv.n, nq, dv.Xb,    \
-                       dv.xn2, dv.xst, dv.tile_slack, Qb, qst, bound, qperm, tiles, dv.ntiles, ntiles_g, scap,   \
+                       dv.xn2, dv.xst, dv.tile_slack, Qb, qst, bound, qperm, tiles, stride, ntiles_g, scap,       \
                        surv, scnt, thr0, hist)
   switch (dv.L) {
   case 96: HKG_BF16_LAUNCH(6); break;
@@ -615,14 +653,16 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
   hipLaunchKernelGGL(k_iota_best, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->best, skeys, sidx);
   HKG_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(b->bf_sort_tmp, b->bf_sort_bytes, skeys, b->qperm + 3 * b->B, sidx,
                                                  b->qperm, (int)nq, 0, 32, st));
-  // row tiles each block group needs under the current bounds, and the
-  // filter threshold of every query
+  // row tiles the batch needs under the current bounds (one list that
+  // every block group walks), and the filter threshold of every query
   float *thr0 = reinterpret_cast<float *>(b->thr);
+  constexpr bool kSharedList = true;
+  const uint32_t lists = kSharedList ? 1 : groups, list_qpb = kSharedList ? nq : qpb;
   auto plan_tiles = [&]() {
-    launch_fill(st, b->ntiles_g, groups, 0);
-    hipLaunchKernelGGL(k_group_leaf_need, dim3((nleaf + 255) / 256, groups), dim3(256), 0, st, nq, qpb, nleaf, b->lb,
-                       b->bound, b->qperm, b->need);
-    hipLaunchKernelGGL(k_group_tiles, dim3((dev.ntiles + 255) / 256, groups), dim3(256), 0, st, dev.ntiles, nleaf,
+    launch_fill(st, b->ntiles_g, lists, 0);
+    hipLaunchKernelGGL(k_group_leaf_need, dim3((nleaf + 255) / 256, lists), dim3(256), 0, st, nq, list_qpb, nleaf,
+                       b->lb, b->bound, b->qperm, b->need);
+    hipLaunchKernelGGL(k_group_tiles, dim3((dev.ntiles + 255) / 256, lists), dim3(256), 0, st, dev.ntiles, nleaf,
                        dev.tile_leaf0, dev.tile_leaf1, b->need, b->tiles, b->ntiles_g);
     hipLaunchKernelGGL(k_bf16_thr_init, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->bound, thr0);
   };
@@ -633,19 +673,19 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
     // pass 1: histogram of the BF16 upper bounds -> a k-th distance bound
     // within about one bin of the k-th smallest upper bound
     launch_fill(st, b->hist, nq * kBf16Bins, 0);
-    launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, b->bf_cap, b->bf_surv,
-                       b->bf_cnt, thr0, b->hist);
+    launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, kSharedList, b->bf_cap,
+                       b->bf_surv, b->bf_cnt, thr0, b->hist);
     if (ctr) {
-      nt_pass1.resize(groups);
-      HKG_HIP_TRY(hipMemcpyAsync(nt_pass1.data(), b->ntiles_g, groups * 4, hipMemcpyDeviceToHost, st));
+      nt_pass1.resize(lists);
+      HKG_HIP_TRY(hipMemcpyAsync(nt_pass1.data(), b->ntiles_g, lists * 4, hipMemcpyDeviceToHost, st));
     }
     hipLaunchKernelGGL(k_bf16_tau, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->kq, b->hist, thr0, b->bound);
     plan_tiles();
     if (pf) pf->mark(st, HKG_STAGE_BOUND);
   }
   // pass 2: collect the survivors of the certified lower bound
-  launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, b->bf_cap, b->bf_surv,
-                     b->bf_cnt, thr0, nullptr);
+  launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, kSharedList, b->bf_cap,
+                     b->bf_surv, b->bf_cnt, thr0, nullptr);
   if (pf) pf->mark(st, HKG_STAGE_REFINE);
   launch_bf16_refine(st, nq, dev.X, L, b->Q, b->kq, b->kmax, dev.ids, b->cap, b->cand, b->cand_cnt, b->bf_cap,
                      b->bf_surv, b->bf_cnt, thr0, b->bound, idmap64, b->out_ids, b->out_d2, overflow, b->counters);
@@ -654,15 +694,17 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
   if (ctr) {
     HKG_HIP_TRY(hipMemcpyAsync(ctr->v, b->counters, sizeof(ctr->v), hipMemcpyDeviceToHost, st));
     HKG_HIP_TRY(hipMemcpyAsync(&ctr->overflow, overflow, 4, hipMemcpyDeviceToHost, st));
-    std::vector<uint32_t> nt(groups);
-    HKG_HIP_TRY(hipMemcpyAsync(nt.data(), b->ntiles_g, groups * 4, hipMemcpyDeviceToHost, st));
+    std::vector<uint32_t> nt(lists);
+    HKG_HIP_TRY(hipMemcpyAsync(nt.data(), b->ntiles_g, lists * 4, hipMemcpyDeviceToHost, st));
     HKG_HIP_TRY(hipStreamSynchronize(st));
     // (query, series) pairs and row tiles the filter passes went through
+    // (rows counted once per block group that streamed them)
     for (uint32_t g = 0; g < groups; ++g) {
+      const uint32_t gl = lists == 1 ? 0 : g;
       uint64_t qn = std::min<uint32_t>(qpb, nq - g * qpb);
-      const uint64_t t1 = nt_pass1.empty() ? 0 : nt_pass1[g];
-      ctr->v[kCtrBf16] += (unsigned long long)(nt[g] + t1) * kRowTile * qn;
-      ctr->bf16_rows += (uint64_t)nt[g] * kRowTile;
+      const uint64_t t1 = nt_pass1.empty() ? 0 : nt_pass1[gl];
+      ctr->v[kCtrBf16] += (unsigned long long)(nt[gl] + t1) * kRowTile * qn;
+      ctr->bf16_rows += (uint64_t)nt[gl] * kRowTile;
       ctr->bf16_rows_bound += t1 * kRowTile;
     }
   }

@@ -87,7 +87,7 @@ struct DevIndex {
   uint16_t *Xb = nullptr;      // n x L bf16 bits (round to nearest even)
   float4 *xst = nullptr;       // n x {|x|^2 down, 2|x| up, 2(|x - x~| + g_L |x~|) up, 0}, x~ = bf16(x)
   float *xn2 = nullptr;        // n x |x|^2 (down), contiguous for the filter's tile loads
-  float2 *tile_slack = nullptr;  // per 128-row tile: max of xst.y and of xst.z over its rows
+  float2 *tile_slack = nullptr;  // per 128-row filter tile: max of xst.y and of xst.z over its rows
   uint32_t ntiles = 0;         // 128-row tiles of the BF16 filter
   uint32_t *tile_leaf0 = nullptr, *tile_leaf1 = nullptr;  // leaves overlapping each tile
   uint64_t device_bytes = 0;
