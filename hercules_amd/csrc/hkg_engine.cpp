@@ -484,7 +484,7 @@ uint64_t Index::total_scan_bytes(uint32_t nq, uint32_t batch) const
   uint64_t passes = 0;
   for (uint32_t q0 = 0; q0 < nq; q0 += batch) {
     uint32_t n = std::min(batch, nq - q0);
-    passes += (n + kScanQ - 1) / kScanQ;
+    passes += (n + scan_group_size(n) - 1) / scan_group_size(n);
   }
   return passes * dev.n * dev.L * 4;
 }
@@ -493,10 +493,12 @@ uint64_t Index::total_segs() const { return dev.tree.seg_end.size(); }
 
 uint32_t scan_grid_x(uint32_t nq)
 {
-  uint32_t groups = (nq + kScanQ - 1) / kScanQ;
+  // a multiple of 8 (the XCD count): block x of every query group runs on
+  // the same XCD and reads the same rows
+  uint32_t groups = (nq + scan_group_size(nq) - 1) / scan_group_size(nq);
   uint32_t total = persistent_grid();
   uint32_t gx = total / std::max<uint32_t>(groups, 1);
-  return std::max<uint32_t>(gx, 64);
+  return std::max<uint32_t>(gx / 8 * 8, 64);
 }
 
 void Index::enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf)

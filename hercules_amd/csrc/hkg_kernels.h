@@ -7,7 +7,8 @@ namespace hkg {
 
 constexpr int kRefineTile = 2048;        // floats of chunk sums per wave (refine)
 constexpr int kScanQ = 4;                // queries per block in the full scan
-constexpr int kScanTile = 2048;          // floats of chunk sums per wave (scan)
+// queries per pass of the full scan for a batch of nq
+inline uint32_t scan_group_size(uint32_t nq) { return nq <= 2 ? (nq ? nq : 1) : (uint32_t)kScanQ; }
 constexpr size_t kEapcaLdsBytes = 48 * 1024;
 constexpr int kWorkChunk = 8;            // work items taken per atomic
 constexpr int kBestLeaves = 4;           // leaves per query scanned in the approximate stage

@@ -566,8 +566,18 @@ __global__ void __launch_bounds__(kBlock) k_filter_refine(
   }
 }
 
-// PSCAN: exact full scan, a group of up to kScanQ queries per block,
-// persistent over row tiles
+// PSCAN: exact full scan (parallel_scan_utils.c:349-429 does one query per
+// worker over the original-order file).  A block serves a group of QN
+// queries (QN <= kScanQ) and sweeps the rows persistently, one row per
+// lane: the lane streams its row in 128-byte steps (four 8-float chunks)
+// and folds the chunk sums in the reference's order (ts.c:83-109: an
+// 8-term tree per chunk, chunks added in sequence), abandoning the row once
+// every query's partial sum exceeds its bound (partial sums only grow, so
+// no row within the bound is lost).  Queries are read from LDS by
+// broadcast.  Block x of every query group reads the same rows (grid.x a
+// multiple of the XCD count), so one group brings them from HBM and the
+// others find them in the caches.
+template <int QN>
 __global__ void __launch_bounds__(kBlock) k_scan(const float *__restrict__ X, uint64_t n, uint32_t L,
                                                  const float *__restrict__ Q, uint32_t nq,
                                                  const uint32_t *__restrict__ kq,
@@ -576,56 +586,65 @@ __global__ void __launch_bounds__(kBlock) k_scan(const float *__restrict__ X, ui
                                                  uint32_t *__restrict__ cand_cnt,
                                                  uint32_t *__restrict__ bound)
 {
-  extern __shared__ float qs[];  // kScanQ * L
-  __shared__ TopK tk[kScanQ];
-  __shared__ float csum[kBlock / 64][kScanTile];
+  extern __shared__ float qs[];  // QN * L
+  __shared__ TopK tk[QN];
   __shared__ uint32_t scan[kBlock];
-  const uint32_t q0 = blockIdx.y * kScanQ;
-  const int nqg = (int)(nq - q0 < (uint32_t)kScanQ ? nq - q0 : (uint32_t)kScanQ);
+  const uint32_t q0 = blockIdx.y * QN;
+  const int nqg = (int)(nq - q0 < (uint32_t)QN ? nq - q0 : (uint32_t)QN);
   for (int j = 0; j < nqg; ++j) load_query(Q, q0 + j, L, qs + j * L);
-  for (int j = 0; j < kScanQ; ++j) topk_reset(tk[j]);
+  for (int j = 0; j < QN; ++j) topk_reset(tk[j]);
   __syncthreads();
-  const int cpr = (int)(L >> 3);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int g = kScanTile / (kScanQ * (cpr + 1));
-  if (g > 64) g = 64;
-  const uint64_t rows_per_iter = (uint64_t)g * (kBlock / 64);
-  float *cs = csum[wave];
-  uint32_t bnd[kScanQ];
-  for (uint64_t base = (uint64_t)blockIdx.x * rows_per_iter; base < n;
-       base += (uint64_t)gridDim.x * rows_per_iter) {
+  const int nch = (int)(L >> 3);  // 8-float chunks per row
+  const float4 *q4 = reinterpret_cast<const float4 *>(qs);
+  for (uint64_t base = (uint64_t)blockIdx.x * kBlock; base < n; base += (uint64_t)gridDim.x * kBlock) {
+    uint32_t bnd[QN];
 #pragma unroll
-    for (int j = 0; j < kScanQ; ++j) {
-      uint32_t gb = j < nqg ? __atomic_load_n(&bound[q0 + j], __ATOMIC_RELAXED) : 0u;
+    for (int j = 0; j < QN; ++j) {
+      const uint32_t gb = j < nqg ? __atomic_load_n(&bound[q0 + j], __ATOMIC_RELAXED) : 0u;
       bnd[j] = gb < tk[j].kth_bits ? gb : tk[j].kth_bits;
     }
-    uint64_t wbeg = base + (uint64_t)wave * g;
-    int cnt = wbeg < n ? (int)((n - wbeg) < (uint64_t)g ? (n - wbeg) : (uint64_t)g) : 0;
-    const int tasks = cnt * cpr;
-    for (int s = lane; s < tasks; s += 64) {
-      int r = s / cpr;
-      int c = s - r * cpr;
-      const float *xp = X + (wbeg + r) * L + (uint64_t)c * 8;
-      float4 xa = ld_stream(xp);
-      float4 xb = ld_stream(xp + 4);
+    const uint64_t row = base + threadIdx.x;
+    if (row < n) {
+      const float4 *xp = reinterpret_cast<const float4 *>(X + row * L);
+      float acc[QN];
 #pragma unroll
-      for (int j = 0; j < kScanQ; ++j) {
-        const float4 *qp = reinterpret_cast<const float4 *>(qs + j * L + c * 8);
-        cs[(j * g + r) * (cpr + 1) + c] = ed_chunk(qp[0], qp[1], xa, xb);
-      }
-    }
-    __syncthreads();
-    if (lane < cnt) {
-      uint64_t row = wbeg + lane;
-      uint32_t id = ids ? ids[row] : (uint32_t)row;
+      for (int j = 0; j < QN; ++j) acc[j] = 0.0f;
+      bool alive = true;
+      int c = 0;
+      // four chunks (128 bytes) per step
+      for (; c + 4 <= nch && alive; c += 4) {
+        float4 xv[8];
 #pragma unroll
-      for (int j = 0; j < kScanQ; ++j) {
-        if (j < nqg) {
-          const float *p = cs + (j * g + lane) * (cpr + 1);
-          float acc = 0.0f;
-          for (int c = 0; c < cpr; ++c) acc = acc + p[c];
-          if (__float_as_uint(acc) <= bnd[j]) topk_stage(tk[j], make_key(acc, id));
+        for (int i = 0; i < 8; ++i) xv[i] = ld_stream(reinterpret_cast<const float *>(xp + 2 * c + i));
+        alive = false;
+#pragma unroll
+        for (int j = 0; j < QN; ++j) {
+          if (j < nqg) {
+#pragma unroll
+            for (int sub = 0; sub < 4; ++sub)
+              acc[j] = acc[j] + ed_chunk(q4[(j * L >> 2) + 2 * (c + sub)], q4[(j * L >> 2) + 2 * (c + sub) + 1],
+                                         xv[2 * sub], xv[2 * sub + 1]);
+            alive |= __float_as_uint(acc[j]) <= bnd[j];
+          }
         }
+      }
+      for (; c < nch && alive; ++c) {
+        const float4 xa = ld_stream(reinterpret_cast<const float *>(xp + 2 * c));
+        const float4 xb = ld_stream(reinterpret_cast<const float *>(xp + 2 * c + 1));
+        alive = false;
+#pragma unroll
+        for (int j = 0; j < QN; ++j) {
+          if (j < nqg) {
+            acc[j] = acc[j] + ed_chunk(q4[(j * L >> 2) + 2 * c], q4[(j * L >> 2) + 2 * c + 1], xa, xb);
+            alive |= __float_as_uint(acc[j]) <= bnd[j];
+          }
+        }
+      }
+      if (alive) {  // every chunk was added: acc is the exact distance
+        const uint32_t id = ids ? ids[row] : (uint32_t)row;
+#pragma unroll
+        for (int j = 0; j < QN; ++j)
+          if (j < nqg && __float_as_uint(acc[j]) <= bnd[j]) topk_stage(tk[j], make_key(acc[j], id));
       }
     }
     __syncthreads();
@@ -908,9 +927,15 @@ void launch_scan(hipStream_t st, uint32_t grid_x, const float *X, uint64_t n, ui
                  uint32_t nq, const uint32_t *kq, const uint32_t *ids, uint32_t cap, uint64_t *cand,
                  uint32_t *cand_cnt, uint32_t *bound)
 {
-  uint32_t groups = (nq + kScanQ - 1) / kScanQ;
-  hipLaunchKernelGGL(k_scan, dim3(grid_x, groups), dim3(kBlock), (size_t)kScanQ * L * sizeof(float), st, X,
-                     n, L, Q, nq, kq, ids, cap, cand, cand_cnt, bound);
+  // queries per pass: up to kScanQ (a lone query scans alone)
+  const uint32_t qn = scan_group_size(nq);
+  const uint32_t groups = (nq + qn - 1) / qn;
+  const size_t lds = (size_t)qn * L * sizeof(float);
+  switch (qn) {
+  case 1: hipLaunchKernelGGL(k_scan<1>, dim3(grid_x, groups), dim3(kBlock), lds, st, X, n, L, Q, nq, kq, ids, cap, cand, cand_cnt, bound); break;
+  case 2: hipLaunchKernelGGL(k_scan<2>, dim3(grid_x, groups), dim3(kBlock), lds, st, X, n, L, Q, nq, kq, ids, cap, cand, cand_cnt, bound); break;
+  default: hipLaunchKernelGGL(k_scan<kScanQ>, dim3(grid_x, groups), dim3(kBlock), lds, st, X, n, L, Q, nq, kq, ids, cap, cand, cand_cnt, bound); break;
+  }
 }
 
 void launch_select(hipStream_t st, uint32_t nq, uint32_t kmax, const uint32_t *kq, uint32_t cap,
