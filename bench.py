@@ -43,7 +43,7 @@ def parse() -> argparse.Namespace:
     p.add_argument("--length", type=int, default=256)
     p.add_argument("--batch", type=int, default=1024, help="queries per step")
     p.add_argument("--leaf", type=int, default=10000)
-    p.add_argument("--path", default="batched", choices=["batched", "index", "scan", "bf16"])
+    p.add_argument("--path", default="bf16", choices=["batched", "index", "scan", "bf16"])
     p.add_argument("--sigma-max", type=float, default=0.2)
     p.add_argument("--kmax", type=int, default=100)
     p.add_argument("--no-cpu-baseline", action="store_true")
