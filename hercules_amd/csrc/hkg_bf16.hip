@@ -27,6 +27,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 #include "hkg_engine.h"
@@ -40,11 +41,13 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 namespace {
 
 constexpr int kQPerWave = 64;              // two 32-query MFMA tiles
-// Waves per filter block: the query operands of a wave take 8 KS registers,
-// so L = 256 runs 4 waves (one per SIMD, up to 512 registers with the
-// accumulation registers) and shorter series 8 (two per SIMD).
-__host__ __device__ constexpr int filter_waves(int KS) { return KS >= 16 ? 4 : 8; }
-constexpr int kRowTile = 128;              // series per LDS tile (double-buffered)
+// Waves per filter block: one per SIMD, so a wave can hold its query
+// operands (8 KS registers) with up to 512 registers including the
+// accumulation registers.
+__host__ __device__ constexpr int filter_waves(int) { return 4; }
+constexpr int kRowTile = 96;               // series per LDS tile
+constexpr int kBuf = 3;                    // LDS tile buffers (tiles loaded two ahead)
+constexpr uint32_t kRegionCap = 65536;     // survivors per filter block before k_bf16_scatter
 constexpr float kEvalSlack = 1.0e-3f;      // FP32 evaluation of the bound
 constexpr uint32_t kBf16SurvivorCap = 524288;  // survivors kept per query (8 B each)
 constexpr float kHistLo = 0.6f;            // histogram range: [0.6 bound, threshold]
@@ -158,11 +161,17 @@ __device__ __forceinline__ void load_lds4(const void *g, lds_void *lds)
   __builtin_amdgcn_global_load_lds(g, lds, 4, 0, 0);
 }
 
-// Filter.  Block = 8 waves x 64 queries (two 32-query tiles per wave) of
-// block group g (grid.y), the queries qperm[512 g ..]; it loops over the
-// row tiles listed for the group (tiles overlapping a leaf whose EAPCA bound
-// is within the k-th distance bound of one of its queries), double-buffered
-// in LDS by direct global->LDS loads.
+// Filter.  Block = W waves x 64 queries (two 32-query tiles per wave) of
+// block group g (grid.y), the queries qperm[64 W g ..]; it loops over the
+// row tiles of the batch's tile list (tiles overlapping a leaf whose EAPCA
+// bound is within the k-th distance bound of one of the batch's queries).
+// Tiles of kRowTile series go through a ring of kBuf LDS buffers, loaded
+// two tiles ahead by direct global->LDS copies (the series in BF16, their
+// |x|^2 and their slack terms) that each wave issues among its MFMAs.  The
+// loop waits on vector memory only for the tile it is about to use: the
+// pre-test and the exact bound read LDS, and survivors go to the wave's
+// slice of a per-block region in global memory, at positions from ballots
+// (k_bf16_scatter files them per query).
 //
 // MFMA 32x32x16 BF16, D = A.B: A = 32 series x 16 dims from LDS (lane l:
 // series l%32, dims 8(l/32)..+8), B = 16 dims x 32 queries from registers
@@ -175,31 +184,56 @@ __device__ __forceinline__ void load_lds4(const void *g, lds_void *lds)
 // operand reads, and each wave's direct load fills 1 KB of consecutive LDS.
 //
 // HIST = false: pairs whose certified lower bound is within the query's
-// threshold are appended to its survivor list.  HIST = true: pairs whose
-// certified upper bound is within the threshold are counted in the query's
-// histogram (global atomics; only pairs passing the lower-bound pre-test
-// get there, which undercounts nothing that matters: an undercount only
-// loosens the bound).
+// threshold become survivors.  HIST = true: pairs whose certified upper
+// bound is within the threshold are counted in the query's histogram
+// (global atomics; only pairs passing the lower-bound pre-test get there,
+// which undercounts nothing that matters: an undercount only loosens the
+// bound).
 template <int KS, bool HIST, int W = filter_waves(KS)>
-__global__ void __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(8 / W, 8 / W)))
+__global__ void __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const float *__restrict__ xn2g,
               const float4 *__restrict__ xst, const float2 *__restrict__ tslack, const uint16_t *__restrict__ Qb,
               const float4 *__restrict__ qst, const uint32_t *__restrict__ bound, const uint32_t *__restrict__ qperm,
               const uint32_t *__restrict__ tiles, uint32_t tile_stride, const uint32_t *__restrict__ ntiles_g,
               uint32_t scap, uint64_t *__restrict__ surv, uint32_t *__restrict__ scnt, const float *__restrict__ thr0,
-              uint32_t *__restrict__ hist)
+              uint32_t *__restrict__ hist, uint4 *__restrict__ region, uint32_t region_cap,
+              uint32_t *__restrict__ region_cnt)
 {
   constexpr int L = KS * 16;
   constexpr int VPR = L / 8;                       // 16-byte vectors per row
-  constexpr int SW = (VPR % 8 == 0) ? 7 : 3;       // swizzle mask (VPR is a multiple of SW + 1)
+  // swizzle mask (VPR is a multiple of SW + 1): the 16 lanes that one LDS
+  // cycle serves for a 16-byte read are 16 consecutive rows at the same
+  // vector position, so 16 distinct positions keep them conflict-free
+  constexpr int SW = (VPR % 16 == 0) ? 15 : (VPR % 8 == 0) ? 7 : 3;
   constexpr int NCH = kRowTile * VPR / 64;         // 1 KB chunks per tile
   constexpr int kWaves = W, kQPerBlock = W * kQPerWave;
-  static_assert(VPR % (SW + 1) == 0 && (kRowTile * VPR) % 64 == 0, "tile shape");
-  __shared__ __attribute__((aligned(16))) uint16_t xs[2][kRowTile * L];
-  __shared__ __attribute__((aligned(16))) float xn2s[2][kRowTile];
-  __shared__ uint8_t s_full[kQPerBlock];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5, l32 = lane & 31;
+  // vector memory ops every wave issues per tile, at least: its share of
+  // the series chunks, and with four waves one |x|^2 or slack-term load each
+  constexpr int kWaitOps = NCH / W + (W == 4 ? 1 : 0);
+  static_assert(VPR % (SW + 1) == 0 && (kRowTile * VPR) % 64 == 0 && kRowTile == 96 && W >= 4 && kWaitOps <= 15,
+                "tile shape");
+  // three distinct buffers (not one array): the compiler then sees that the
+  // copies in flight into one buffer do not alias reads of another and
+  // does not wait for them
+  struct TileBuf {
+    uint16_t x[kRowTile * L];  // series, BF16, swizzled rows
+    float n2[kRowTile];        // |x|^2
+    float4 st[kRowTile];       // {|x|^2, A_x, B_x, 0}
+  };
+  __shared__ __attribute__((aligned(16))) TileBuf tb0, tb1, tb2;
+  auto buf = [&](auto Bc) -> TileBuf & {
+    constexpr int B = decltype(Bc)::value;
+    if constexpr (B == 0)
+      return tb0;
+    else if constexpr (B == 1)
+      return tb1;
+    else
+      return tb2;
+  };
+  const int lane = threadIdx.x & 63, half = lane >> 5, l32 = lane & 31;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar loops and copy bases
   const uint32_t g = blockIdx.y;
+  const uint32_t blk = blockIdx.y * gridDim.x + blockIdx.x;
 
   // this lane's query of each query tile, and its constants
   uint32_t qid[2];
@@ -232,93 +266,196 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
 #pragma unroll
     for (int k = 0; k < KS; ++k) qop[t][k] = src[2 * k + half];
   }
-  for (int i = threadIdx.x; i < kQPerBlock; i += kWaves * 64) s_full[i] = 0;
+  // survivors of this wave: its slice of the block's region, counted in a
+  // wave-uniform register (positions from ballots, no atomics)
+  const uint32_t wcap = region_cap / kWaves;
+  const uint64_t wbase = ((uint64_t)blk * kWaves + wave) * wcap;
+  uint32_t wcnt = 0;
 
   // every block group walks the one tile list of the batch (tile_stride =
-  // 0) or its own; with the shared list, block x of every group reads the
-  // same tiles in the same order on the same XCD, so one of them brings a
-  // tile from HBM and the others find it in the caches
+  // 0) or its own; block x takes list entries x, x + grid.x, ..., so block
+  // x of every group reads the same tiles at about the same time on the
+  // same XCD
   const uint32_t gl = tile_stride ? g : 0;
   const uint32_t nt = ntiles_g[gl];
   const uint32_t my_n = blockIdx.x < nt ? (nt - blockIdx.x + gridDim.x - 1) / gridDim.x : 0;
   const uint32_t *tlist = tiles + (uint64_t)gl * tile_stride;
-
-  // issue the loads of the j-th tile of this block into buffer b
-  auto issue = [&](uint32_t j, int b) {
-    const uint64_t r0 = (uint64_t)tlist[blockIdx.x + j * gridDim.x] * kRowTile;
-    for (int c = wave; c < NCH; c += kWaves) {
-      const int lin = c * 64 + lane;     // LDS vector index of this lane
+  // the block's tile ids and slack maxima in registers, 64 tiles per
+  // chunk, one per lane: the chunk of the current step (cid, cts) and the
+  // next (nid, nts), read with readlane.  No memory access in the steps:
+  // scalar loads would share the LDS wait counter and return out of
+  // order, and an LDS read after the tile copies are issued makes the
+  // compiler wait for all of them.  A chunk is fetched and waited for at
+  // once (the empty asm uses its registers), so that no load is pending
+  // across steps: the compiler would otherwise drain every copy in flight
+  // before touching those registers.
+  auto id_of = [&](uint32_t c) -> uint32_t {
+    const uint32_t j = c * 64 + lane;
+    return j < my_n ? tlist[blockIdx.x + (uint64_t)j * gridDim.x] : 0u;
+  };
+  auto fetch_chunk = [&](uint32_t c, uint32_t &id, float2 &ts) {
+    id = id_of(c);
+    ts = tslack[id];
+    asm volatile("" ::"v"(id), "v"(ts.x), "v"(ts.y));
+  };
+  uint32_t cid, nid;
+  float2 cts, nts;
+  fetch_chunk(0, cid, cts);
+  fetch_chunk(1, nid, nts);
+  // id of the block's j-th tile, from the chunk of step jc or the next
+  auto tile_at = [&](uint32_t j, uint32_t jc) -> uint32_t {
+    return (j >> 6) == (jc >> 6) ? __builtin_amdgcn_readlane(cid, j & 63) : __builtin_amdgcn_readlane(nid, j & 63);
+  };
+  auto slack_at = [&](uint32_t j) -> float2 {
+    return make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(cts.x), j & 63)),
+                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cts.y), j & 63)));
+  };
+  // Copies of the tile at rows r0.. into buffer Bc: 1 KB series chunks c =
+  // wave + i W (this wave's i-th), and one load of |x|^2 (rows 0..63: wave
+  // 0, 64..95: wave 1) or of the slack terms (waves 2, 3)
+  constexpr int NCI = (NCH + kWaves - 1) / kWaves;  // series chunks per wave, at most
+  auto issue_chunk = [&](uint64_t r0, auto Bc, int i) {
+    TileBuf &tbuf = buf(Bc);
+    const int c = wave + i * kWaves;
+    if (NCH % kWaves == 0 || c < NCH) {
+      const int lin = c * 64 + lane;  // LDS vector index of this lane
       const int r = lin / VPR, p = lin - r * VPR;
-      const uint64_t gr = r0 + r < n ? r0 + r : n - 1;  // rows past the end: a copy of the last row
-      const uint16_t *src = Xb + gr * L + (uint32_t)((p ^ (r & SW)) * 8);
-      load_lds16(src, (lds_void *)(reinterpret_cast<char *>(&xs[b][0]) + c * 1024));
+      // a uniform base and a 32-bit lane offset (the rows are padded to
+      // whole tiles, so the last tile needs no clamp)
+      const uint16_t *src = Xb + r0 * L + (uint32_t)(r * L + (p ^ (r & SW)) * 8);
+      load_lds16(src, (lds_void *)(reinterpret_cast<char *>(&tbuf.x[0]) + c * 1024));
     }
-    static_assert(kRowTile / 64 <= kWaves, "xn2 loads");
-    if (wave < kRowTile / 64) {
-      const uint64_t r = r0 + wave * 64 + lane;
-      load_lds4(xn2g + (r < n ? r : n - 1), (lds_void *)(reinterpret_cast<char *>(&xn2s[b][0]) + wave * 256));
+  };
+  auto issue_stats = [&](uint64_t r0, auto Bc) {
+    TileBuf &tbuf = buf(Bc);
+    const int part = wave & 1, rbase = part * 64;
+    if (wave < 4 && (part == 0 || lane < kRowTile - 64)) {
+      if (wave < 2)
+        load_lds4(xn2g + r0 + rbase + lane, (lds_void *)(reinterpret_cast<char *>(&tbuf.n2[0]) + rbase * 4));
+      else
+        load_lds16(xst + r0 + rbase + lane, (lds_void *)(reinterpret_cast<char *>(&tbuf.st[0]) + rbase * 16));
     }
+  };
+  auto issue = [&](uint32_t tile, auto Bc) {
+    const uint64_t r0 = (uint64_t)tile * kRowTile;
+#pragma unroll
+    for (int i = 0; i < NCI; ++i) issue_chunk(r0, Bc, i);
+    issue_stats(r0, Bc);
   };
 
-  // MFMAs of one 32-row sub-tile (rows crow.. of buffer b)
-  auto mfma_sub = [&](int b, int crow, f32x16 &a0, f32x16 &a1) {
-    const bf16x8 *xv = reinterpret_cast<const bf16x8 *>(&xs[b][0]);
-    const int row = crow + l32;
-    a0 = f32x16{};
-    a1 = f32x16{};
-#pragma unroll
-    for (int k = 0; k < KS; ++k) {
-      const bf16x8 a = xv[row * VPR + ((2 * k + half) ^ (row & SW))];
-      a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qop[0][k], a0, 0, 0, 0);
-      a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qop[1][k], a1, 0, 0, 0);
-    }
-  };
-  // pre-test of one sub-tile's pairs: series of register r is
-  // crow + 8(r/4) + 4 half + r%4 of the tile
-  auto pretest = [&](int b, int crow, const f32x16 &a0, const f32x16 &a1, float C0, float C1, float4 (&x4)[4]) {
+  // Pre-test of one sub-tile's pairs (series of register r is crow +
+  // 8(r/4) + 4 half + r%4 of the tile): m_t = min over the lane's 16 series
+  // of xn2 - 2 s for query tile t; some pair passes (xn2 - 2 s <= C_t) iff
+  // m_t <= C_t, as fminf skips NaN the way <= rejects it.
+  auto n2_of = [&](auto Bc, int crow, float4 (&x4)[4]) {
+    const TileBuf &tbuf = buf(Bc);
 #pragma unroll
     for (int gi = 0; gi < 4; ++gi)
-      x4[gi] = *reinterpret_cast<const float4 *>(&xn2s[b][crow + 8 * gi + 4 * half]);
-    bool hit = false;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const float xr = x4[r >> 2][r & 3];
-      hit |= fmaf(a0[r], -2.0f, xr) <= C0;
-      hit |= fmaf(a1[r], -2.0f, xr) <= C1;
-    }
-    return hit;
+      x4[gi] = *reinterpret_cast<const float4 *>(&tbuf.n2[crow + 8 * gi + 4 * half]);
   };
-  // rare: pairs that passed the pre-test, with their own slack
-  auto slow = [&](uint64_t r0, const f32x16 &a0, const f32x16 &a1, float C0, float C1, const float4 (&x4)[4]) {
+  auto pt_elem = [&](int r, const f32x16 &p0, const f32x16 &p1, const float4 (&x4)[4], float &m0, float &m1) {
+    const float xr = x4[r >> 2][r & 3];
+    const float v0 = fmaf(p0[r], -2.0f, xr), v1 = fmaf(p1[r], -2.0f, xr);
+    m0 = r == 0 ? v0 : fminf(m0, v0);
+    m1 = r == 0 ? v1 : fminf(m1, v1);
+  };
+  // MFMAs of 32-row sub-tile SUB (rows crow = 32 SUB.. of buffer Bc) into
+  // a0/a1; the series operands are read two k-steps ahead of their MFMAs.
+  // Between the MFMAs this wave issues its copies of the tile at rows r0n..
+  // into buffer BNc, a third per sub-tile: all waves issuing a whole tile
+  // at once would queue on the CU's copy path and stall, while spread
+  // over the MFMAs the copies proceed in the background.  With PT,
+  // the pre-test of the previous sub-tile (rows pcrow.., accumulators p0/p1)
+  // runs between the MFMAs, from the third k-step on (its accumulators are
+  // complete by then): the matrix cores stay busy while the wave, alone on
+  // its SIMD, does the vector work.
+  auto mfma_sub = [&](auto Bc, auto SUBc, f32x16 &a0, f32x16 &a1, auto PTc, int pcrow, const f32x16 &p0,
+                      const f32x16 &p1, float4 (&x4)[4], float &m0, float &m1, uint64_t r0n, auto BNc) {
+    constexpr bool PT = decltype(PTc)::value;
+    constexpr int SUB = decltype(SUBc)::value;
+    constexpr int c_lo = SUB * NCI / 3, c_n = (SUB + 1) * NCI / 3 - c_lo;  // this sub-tile's copy chunks
+    const int crow = 32 * SUB;
+    const bf16x8 *xv = reinterpret_cast<const bf16x8 *>(&buf(Bc).x[0]);
+    const int row = crow + l32;
+    auto op = [&](int k) { return xv[row * VPR + ((2 * k + half) ^ (row & SW))]; };
+    // pre-test elements done after k-step k: 0 for k < 2, then spread evenly
+    auto done = [](int k) { return k < 2 ? 0 : (k - 2) * 16 / (KS - 2); };
+    if constexpr (PT) {
+      __builtin_amdgcn_sched_barrier(0);
+      n2_of(Bc, pcrow, x4);
+    }
+    a0 = f32x16{};
+    a1 = f32x16{};
+    bf16x8 cur = op(0), nxt = op(1);
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const bf16x8 nn = k + 2 < KS ? op(k + 2) : nxt;
+      a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur, qop[0][k], a0, 0, 0, 0);
+      a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur, qop[1][k], a1, 0, 0, 0);
+      if constexpr (PT) {
+#pragma unroll
+        for (int r = done(k); r < done(k + 1); ++r) pt_elem(r, p0, p1, x4, m0, m1);
+      }
+      // chunk c_lo + q at k-step 1 + q KS / c_n; |x|^2 or slack terms at
+      // k-step 0 of sub-tile 0
+#pragma unroll
+      for (int q = 0; q < c_n; ++q)
+        if (k == 1 + q * KS / c_n) issue_chunk(r0n, BNc, c_lo + q);
+      if (SUB == 0 && k == 0) issue_stats(r0n, BNc);
+      cur = nxt;
+      nxt = nn;
+    }
+  };
+  auto pretest = [&](auto Bc, int crow, const f32x16 &a0, const f32x16 &a1, float4 (&x4)[4], float &m0, float &m1) {
+    n2_of(Bc, crow, x4);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) pt_elem(r, a0, a1, x4, m0, m1);
+  };
+  // Pairs of a sub-tile whose pre-test passed (rare).  HIST: the pairs
+  // whose upper bound with their own slack is within the threshold go to
+  // the query's histogram.  Otherwise every pair that passes the pre-test
+  // goes, as {row, xn2 - 2 s, query}, to the wave's slice of the region:
+  // k_bf16_scatter tests it with its own slack.  Per pair: one FMA and a
+  // ballot, and a uniform skip when no lane passes.
+  auto slow = [&](auto Bc, int crow, uint64_t r0, const f32x16 &a0, const f32x16 &a1, float C0, float C1,
+                  const float4 (&x4)[4]) {
+    const TileBuf &tbuf = buf(Bc);
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const float sv = t == 0 ? a0[r] : a1[r];
-        const float xr = x4[r >> 2][r & 3];
-        if (fmaf(sv, -2.0f, xr) <= (t == 0 ? C0 : C1)) {
-          const uint64_t grow = r0 + 8 * (r >> 2) + 4 * half + (r & 3);
-          if (grow < n) {
-            const float4 xsv = xst[grow];
-            const float slack = qe[t] * xsv.y + qbn[t] * xsv.z + kEvalSlack;
-            const float base = (qn2[t] + xsv.x) - 2.0f * sv;
-            if (HIST) {
-              const float ub = base + slack;
-              if (ub <= thr[t]) {
-                const float lo = hlo[t];
-                const int bin = ub < lo ? 0 : min(kBf16Bins - 1, 1 + (int)fminf((ub - lo) * hinv[t], (float)kBf16Bins));
-                atomicAdd(&hist[(uint64_t)qid[t] * kBf16Bins + bin], 1u);
-              }
-            } else {
-              const float lb = base - slack;
-              const int sl = wave * kQPerWave + t * 32 + l32;
-              if (lb <= thr[t] && !s_full[sl]) {
-                const uint32_t pos = atomicAdd(&scnt[qid[t]], 1u);
-                if (pos < scap)
-                  surv[(uint64_t)qid[t] * scap + pos] =
-                      ((uint64_t)__float_as_uint(fmaxf(lb, 0.0f)) << 32) | (uint32_t)grow;
-                else
-                  s_full[sl] = 1;
+        const float v = fmaf(sv, -2.0f, x4[r >> 2][r & 3]);  // xn2 - 2 s
+        const bool pass = v <= (t == 0 ? C0 : C1);
+        const uint64_t m = __ballot(pass);
+        if (m == 0) continue;
+        const int tr = crow + 8 * (r >> 2) + 4 * half + (r & 3);
+        const uint64_t grow = r0 + tr;
+        if constexpr (HIST) {
+          if (pass && grow < n) {
+            const float4 xsv = tbuf.st[tr];
+            const float ub = (v + qn2[t]) + (qe[t] * xsv.y + qbn[t] * xsv.z + kEvalSlack);
+            if (ub <= thr[t]) {
+              const float lo = hlo[t];
+              const int bin = ub < lo ? 0 : min(kBf16Bins - 1, 1 + (int)fminf((ub - lo) * hinv[t], (float)kBf16Bins));
+              atomicAdd(&hist[(uint64_t)qid[t] * kBf16Bins + bin], 1u);
+            }
+          }
+        } else {
+          const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+          const uint32_t pos = wcnt + below;
+          wcnt += (uint32_t)__popcll(m);
+          if (pass) {
+            if (pos < wcap) {
+              region[wbase + pos] = make_uint4((uint32_t)grow, __float_as_uint(v), qid[t], 0u);
+            } else if (grow < n) {
+              // the wave's slice is full: tested here, straight to the query's list
+              const float4 xsv = xst[grow];
+              const float lb = (v + qn2[t]) - (qe[t] * xsv.y + qbn[t] * xsv.z + kEvalSlack);
+              if (lb <= thr[t]) {
+                const uint32_t qp = atomicAdd(&scnt[qid[t]], 1u);
+                if (qp < scap)
+                  surv[(uint64_t)qid[t] * scap + qp] = ((uint64_t)__float_as_uint(fmaxf(lb, 0.0f)) << 32) | (uint32_t)grow;
               }
             }
           }
@@ -326,53 +463,90 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
       }
     }
   };
-  // schedule of one MFMA block with the previous sub-tile's pre-test: two
-  // operand reads ahead, then (2 MFMA, 1 read, a few VALU) -- few operand
-  // registers live (the query operands take 8 KS) and the pre-test's VALU
-  // work fills the MFMA shadow
-  auto interleave = [&]() {
-    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
-#pragma unroll
-    for (int k = 0; k < KS - 2; ++k) {
-      __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
-      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-      __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);
+  using B0 = std::integral_constant<int, 0>;
+  using B1 = std::integral_constant<int, 1>;
+  using B2 = std::integral_constant<int, 2>;
+  if (my_n > 0) issue(tile_at(0, 0), B0{});
+  if (my_n > 1) issue(tile_at(1, 0), B1{});
+  // one tile: j in buffer Bc (= j % 3); issues tile j + 2 into the buffer
+  // that held tile j - 1
+  auto step = [&](auto Bc, uint32_t j) {
+    constexpr int B = decltype(Bc)::value;
+    using BN = std::integral_constant<int, (B + 2) % kBuf>;
+    // this wave's loads of tile j have landed (those of tile j + 1 may still
+    // be in flight), then every wave's; tile j - 1 is no longer read
+    if (j + 1 < my_n)
+      __builtin_amdgcn_s_waitcnt(0xF70 | kWaitOps);
+    else
+      __builtin_amdgcn_s_waitcnt(0);
+    // a plain barrier (and a compiler memory barrier): the fences of
+    // __syncthreads would also wait for tile j + 1's copies, as those
+    // write LDS.  Loads complete in order, so the wait above covers every
+    // copy of tile j even with survivor stores in flight.
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    // a new chunk: the next one becomes current and the one after is
+    // fetched (only tile j + 1's copies are in flight here)
+    if ((j & 63) == 0 && j > 0) {
+      cid = nid;
+      cts = nts;
+      fetch_chunk((j >> 6) + 1, nid, nts);
     }
-    __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
-  };
-
-  static_assert(kRowTile == 128, "four sub-tiles per tile below");
-  if (my_n) issue(0, 0);
-  for (uint32_t j = 0; j < my_n; ++j) {
-    const int b = j & 1;
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's loads of tile j have landed
-    __syncthreads();                // ... and every wave's; tile j-1 is no longer read
-    if (j + 1 < my_n) issue(j + 1, b ^ 1);
-    const uint32_t tile = tlist[blockIdx.x + j * gridDim.x];
-    const uint64_t r0 = (uint64_t)tile * kRowTile;
+    // the tile two ahead, copied during this step's MFMAs (past the end
+    // of the list, tile 0 again into the free buffer: the same count of
+    // copies every step, and no branch among the MFMAs)
+    const uint64_t r0n = (uint64_t)tile_at(j + 2, j) * kRowTile;
+    const uint64_t r0 = (uint64_t)tile_at(j, j) * kRowTile;
     // pre-test constant: xn2 - 2 s <= C  <=>  bound with the tile's largest slack <= threshold
-    const float2 tsl = tslack[tile];
+    const float2 tsl = slack_at(j);
     const float C0 = (thr[0] - qn2[0]) + (qe[0] * tsl.x + qbn[0] * tsl.y + kEvalSlack);
     const float C1 = (thr[1] - qn2[1]) + (qe[1] * tsl.x + qbn[1] * tsl.y + kEvalSlack);
-    f32x16 A0, A1, B0, B1;
+    using NoPT = std::false_type;
+    using PT = std::true_type;
+    f32x16 A0, A1, X0, X1;
     float4 xa[4], xb[4];
-    bool ha, hb;
-    // sub-tiles 0..3, the MFMAs of one beside the pre-test of the previous
-    mfma_sub(b, 0, A0, A1);
-    mfma_sub(b, 32, B0, B1);
-    ha = pretest(b, 0, A0, A1, C0, C1, xa);
-    interleave();
-    if (__ballot(ha)) slow(r0, A0, A1, C0, C1, xa);
-    mfma_sub(b, 64, A0, A1);
-    hb = pretest(b, 32, B0, B1, C0, C1, xb);
-    interleave();
-    if (__ballot(hb)) slow(r0 + 32, B0, B1, C0, C1, xb);
-    mfma_sub(b, 96, B0, B1);
-    ha = pretest(b, 64, A0, A1, C0, C1, xa);
-    interleave();
-    if (__ballot(ha)) slow(r0 + 64, A0, A1, C0, C1, xa);
-    hb = pretest(b, 96, B0, B1, C0, C1, xb);
-    if (__ballot(hb)) slow(r0 + 96, B0, B1, C0, C1, xb);
+    float ma0, ma1, mb0, mb1;
+    // sub-tiles 0..2, the MFMAs of one beside the pre-test of the previous
+    using S0 = std::integral_constant<int, 0>;
+    using S1 = std::integral_constant<int, 1>;
+    using S2 = std::integral_constant<int, 2>;
+    mfma_sub(Bc, S0{}, A0, A1, NoPT{}, 0, A0, A1, xa, ma0, ma1, r0n, BN{});
+    mfma_sub(Bc, S1{}, X0, X1, PT{}, 0, A0, A1, xa, ma0, ma1, r0n, BN{});
+    if (__ballot(ma0 <= C0 || ma1 <= C1)) slow(Bc, 0, r0, A0, A1, C0, C1, xa);
+    mfma_sub(Bc, S2{}, A0, A1, PT{}, 32, X0, X1, xb, mb0, mb1, r0n, BN{});
+    if (__ballot(mb0 <= C0 || mb1 <= C1)) slow(Bc, 32, r0, X0, X1, C0, C1, xb);
+    pretest(Bc, 64, A0, A1, xa, ma0, ma1);
+    if (__ballot(ma0 <= C0 || ma1 <= C1)) slow(Bc, 64, r0, A0, A1, C0, C1, xa);
+  };
+  static_assert(kBuf == 3, "the loop below cycles three buffers");
+  for (uint32_t j = 0; j < my_n; j += 3) {
+    step(B0{}, j);
+    if (j + 1 < my_n) step(B1{}, j + 1);
+    if (j + 2 < my_n) step(B2{}, j + 2);
+  }
+  // the last steps' copies (re-reads into free buffers) land before the
+  // block's LDS is released
+  __builtin_amdgcn_s_waitcnt(0);
+  if (!HIST && lane == 0) region_cnt[(uint64_t)blk * kWaves + wave] = wcnt < wcap ? wcnt : wcap;
+}
+
+// The pairs every filter wave found (one region slice each, {row, xn2 -
+// 2 s, query}): the lower bound with the row's own slack, and the pairs
+// within the query's threshold into its list, keyed by that bound.
+__global__ void k_bf16_scatter(const uint4 *__restrict__ region, uint32_t slice_cap,
+                               const uint32_t *__restrict__ region_cnt, uint64_t n, const float4 *__restrict__ xst,
+                               const float4 *__restrict__ qst, const float *__restrict__ thr0, uint32_t scap,
+                               uint64_t *__restrict__ surv, uint32_t *__restrict__ scnt)
+{
+  const uint32_t sl = blockIdx.x;
+  const uint32_t cnt = region_cnt[sl];
+  for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+    const uint4 e = region[(uint64_t)sl * slice_cap + i];
+    if (e.x >= n) continue;  // padding rows of the last tile
+    const float4 xsv = xst[e.x], qs = qst[e.z];
+    const float lb = (__uint_as_float(e.y) + qs.x) - (qs.y * xsv.y + qs.z * xsv.z + kEvalSlack);
+    if (!(lb <= thr0[e.z])) continue;
+    const uint32_t pos = atomicAdd(&scnt[e.z], 1u);
+    if (pos < scap) surv[(uint64_t)e.z * scap + pos] = ((uint64_t)__float_as_uint(fmaxf(lb, 0.0f)) << 32) | e.x;
   }
 }
 
@@ -429,7 +603,7 @@ static uint32_t bf16_grid_x(uint32_t groups)
 void launch_bf16_filter(hipStream_t st, const DevIndex &dv, uint32_t nq, const uint16_t *Qb, const float4 *qst,
                         const uint32_t *bound, const uint32_t *qperm, const uint32_t *tiles,
                         const uint32_t *ntiles_g, bool shared_list, uint32_t scap, uint64_t *surv, uint32_t *scnt,
-                        const float *thr0, uint32_t *hist)
+                        const float *thr0, uint32_t *hist, uint4 *region, uint32_t *region_cnt)
 {
   const uint32_t stride = shared_list ? 0 : dv.ntiles;
   const uint32_t qpb = bf16_qpb(dv.L);
@@ -439,17 +613,31 @@ void launch_bf16_filter(hipStream_t st, const DevIndex &dv, uint32_t nq, const u
   if (hist)                                                                                                       \
     hipLaunchKernelGGL((k_bf16_filter<KS, true>), grid, dim3(filter_waves(KS) * 64), 0, st, dv.n, nq, dv.Xb,     \
                        dv.xn2, dv.xst, dv.tile_slack, Qb, qst, bound, qperm, tiles, stride, ntiles_g, scap,       \
-                       surv, scnt, thr0, hist);                                                                   \
+                       surv, scnt, thr0, hist, region, kRegionCap, region_cnt);                                   \
   else                                                                                                            \
     hipLaunchKernelGGL((k_bf16_filter<KS, false>), grid, dim3(filter_waves(KS) * 64), 0, st, dv.n, nq, dv.Xb,    \
                        dv.xn2, dv.xst, dv.tile_slack, Qb, qst, bound, qperm, tiles, stride, ntiles_g, scap,       \
-                       surv, scnt, thr0, hist)
+                       surv, scnt, thr0, hist, region, kRegionCap, region_cnt)
   switch (dv.L) {
   case 96: HKG_BF16_LAUNCH(6); break;
   case 128: HKG_BF16_LAUNCH(8); break;
   default: HKG_BF16_LAUNCH(16); break;
   }
 #undef HKG_BF16_LAUNCH
+  if (!hist) {
+    const uint32_t w = (uint32_t)filter_waves((int)dv.L / 16);
+    hipLaunchKernelGGL(k_bf16_scatter, dim3(grid.x * grid.y * w), dim3(256), 0, st, region, kRegionCap / w,
+                       region_cnt, dv.n, dv.xst, qst, thr0, scap, surv, scnt);
+  }
+}
+
+// filter blocks of a launch at most (one per CU)
+static uint32_t bf16_max_blocks()
+{
+  int d = 0, cus = 0;
+  (void)hipGetDevice(&d);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d);
+  return (uint32_t)std::max(cus, 1);
 }
 
 // filter threshold of every query: its k-th distance bound with the
@@ -538,21 +726,28 @@ hkg_status Index::make_bf16()
   }
   HKG_HIP_TRY(hipSetDevice(dev.device));
   const uint64_t n = dev.n;
-  if (hipMalloc((void **)&dev.Xb, n * dev.L * 2) != hipSuccess) {
+  // rows padded to whole filter tiles (zeros), so that tile copies need no
+  // clamp at the end
+  const uint64_t ntiles_all = (n + kRowTile - 1) / kRowTile, npad = ntiles_all * kRowTile;
+  if (hipMalloc((void **)&dev.Xb, npad * dev.L * 2) != hipSuccess) {
     (void)hipGetLastError();
     dev.Xb = nullptr;
     set_error("no device memory for the BF16 copy of the series");
     return HKG_ENOMEM;
   }
-  const uint64_t ntiles_all = (n + kRowTile - 1) / kRowTile;
-  if (hipMalloc((void **)&dev.xst, n * sizeof(float4)) != hipSuccess ||
-      hipMalloc((void **)&dev.xn2, n * sizeof(float)) != hipSuccess ||
+  if (hipMalloc((void **)&dev.xst, npad * sizeof(float4)) != hipSuccess ||
+      hipMalloc((void **)&dev.xn2, npad * sizeof(float)) != hipSuccess ||
       hipMalloc((void **)&dev.tile_slack, ntiles_all * sizeof(float2)) != hipSuccess) {
     (void)hipGetLastError();
     set_error("no device memory for the BF16 row norms");
     return HKG_ENOMEM;  // freed by the next attempt or the index
   }
-  dev.device_bytes += n * dev.L * 2 + n * sizeof(float4) + n * sizeof(float) + ntiles_all * sizeof(float2);
+  dev.device_bytes += npad * dev.L * 2 + npad * sizeof(float4) + npad * sizeof(float) + ntiles_all * sizeof(float2);
+  if (npad > n) {
+    HKG_HIP_TRY(hipMemsetAsync(dev.Xb + n * dev.L, 0, (npad - n) * dev.L * 2, stream));
+    HKG_HIP_TRY(hipMemsetAsync(dev.xst + n, 0, (npad - n) * sizeof(float4), stream));
+    HKG_HIP_TRY(hipMemsetAsync(dev.xn2 + n, 0, (npad - n) * sizeof(float), stream));
+  }
   hipLaunchKernelGGL(k_make_bf16, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, n, dev.L, dev.X, dev.Xb,
                      dev.xst, dev.xn2);
   hipLaunchKernelGGL(k_tile_slack, dim3((uint32_t)((ntiles_all + 255) / 256)), dim3(256), 0, stream, n, dev.xst,
@@ -622,7 +817,9 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
         hipMalloc((void **)&b->tiles, (size_t)groups_max * dev.ntiles * 4) != hipSuccess ||
         hipMalloc((void **)&b->ntiles_g, (size_t)groups_max * 4) != hipSuccess ||
         hipMalloc((void **)&b->thr, (size_t)b->B * 4) != hipSuccess ||
-        hipMalloc((void **)&b->hist, (size_t)b->B * kBf16Bins * 4) != hipSuccess) {
+        hipMalloc((void **)&b->hist, (size_t)b->B * kBf16Bins * 4) != hipSuccess ||
+        hipMalloc((void **)&b->bf_region, (size_t)bf16_max_blocks() * kRegionCap * sizeof(uint4)) != hipSuccess ||
+        hipMalloc((void **)&b->bf_region_cnt, (size_t)bf16_max_blocks() * 4 * sizeof(uint32_t)) != hipSuccess) {
       (void)hipGetLastError();
       set_error("no device memory for the BF16 path scratch");
       return HKG_ENOMEM;
@@ -674,7 +871,7 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
     // within about one bin of the k-th smallest upper bound
     launch_fill(st, b->hist, nq * kBf16Bins, 0);
     launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, kSharedList, b->bf_cap,
-                       b->bf_surv, b->bf_cnt, thr0, b->hist);
+                       b->bf_surv, b->bf_cnt, thr0, b->hist, b->bf_region, b->bf_region_cnt);
     if (ctr) {
       nt_pass1.resize(lists);
       HKG_HIP_TRY(hipMemcpyAsync(nt_pass1.data(), b->ntiles_g, lists * 4, hipMemcpyDeviceToHost, st));
@@ -685,7 +882,7 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
   }
   // pass 2: collect the survivors of the certified lower bound
   launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, kSharedList, b->bf_cap,
-                     b->bf_surv, b->bf_cnt, thr0, nullptr);
+                     b->bf_surv, b->bf_cnt, thr0, nullptr, b->bf_region, b->bf_region_cnt);
   if (pf) pf->mark(st, HKG_STAGE_REFINE);
   launch_bf16_refine(st, nq, dev.X, L, b->Q, b->kq, b->kmax, dev.ids, b->cap, b->cand, b->cand_cnt, b->bf_cap,
                      b->bf_surv, b->bf_cnt, thr0, b->bound, idmap64, b->out_ids, b->out_d2, overflow, b->counters);

@@ -313,6 +313,8 @@ void QueryBuffers::free_bf16()
   dfree(ntiles_g);
   dfree(thr);
   dfree(hist);
+  dfree(bf_region);
+  dfree(bf_region_cnt);
 }
 
 QueryBuffers::~QueryBuffers()
