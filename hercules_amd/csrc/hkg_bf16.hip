@@ -264,7 +264,14 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     }
     const bf16x8 *src = reinterpret_cast<const bf16x8 *>(Qb + (uint64_t)qs * L);
 #pragma unroll
-    for (int k = 0; k < KS; ++k) qop[t][k] = src[2 * k + half];
+    for (int k = 0; k < KS; ++k) {
+      qop[t][k] = src[2 * k + half];
+      // the query operands live in accumulation registers (the MFMAs read
+      // their B operand from there); the accumulators, built with
+      // -amdgpu-mfma-vgpr-form, stay in vector registers where the
+      // pre-test reads them without copies
+      asm volatile("" : "+a"(qop[t][k]));
+    }
   }
   // survivors of this wave: its slice of the block's region, counted in a
   // wave-uniform register (positions from ballots, no atomics)
@@ -360,15 +367,21 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     m1 = r == 0 ? v1 : fminf(m1, v1);
   };
   // MFMAs of 32-row sub-tile SUB (rows crow = 32 SUB.. of buffer Bc) into
-  // a0/a1; the series operands are read two k-steps ahead of their MFMAs.
-  // Between the MFMAs this wave issues its copies of the tile at rows r0n..
-  // into buffer BNc, a third per sub-tile: all waves issuing a whole tile
-  // at once would queue on the CU's copy path and stall, while spread
+  // a0/a1.  The series operands are read in batches of G k-steps, each
+  // batch issued right after the first MFMA of the batch before: with the
+  // tile copies in flight the compiler can only wait for all LDS reads at
+  // once, so every read gets most of a batch of MFMAs to land.  Between the
+  // MFMAs this wave issues its copies of the tile at rows r0n.. into buffer
+  // BNc, a third per sub-tile, at batch starts: all waves issuing a whole
+  // tile at once would queue on the CU's copy path and stall, while spread
   // over the MFMAs the copies proceed in the background.  With PT,
   // the pre-test of the previous sub-tile (rows pcrow.., accumulators p0/p1)
   // runs between the MFMAs, from the third k-step on (its accumulators are
   // complete by then): the matrix cores stay busy while the wave, alone on
   // its SIMD, does the vector work.
+  constexpr int G = KS % 4 == 0 ? 4 : (KS % 3 == 0 ? 3 : 2);  // k-steps per operand batch
+  constexpr int NB = KS / G;
+  static_assert(KS % G == 0, "operand batches");
   auto mfma_sub = [&](auto Bc, auto SUBc, f32x16 &a0, f32x16 &a1, auto PTc, int pcrow, const f32x16 &p0,
                       const f32x16 &p1, float4 (&x4)[4], float &m0, float &m1, uint64_t r0n, auto BNc) {
     constexpr bool PT = decltype(PTc)::value;
@@ -380,30 +393,42 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     auto op = [&](int k) { return xv[row * VPR + ((2 * k + half) ^ (row & SW))]; };
     // pre-test elements done after k-step k: 0 for k < 2, then spread evenly
     auto done = [](int k) { return k < 2 ? 0 : (k - 2) * 16 / (KS - 2); };
-    if constexpr (PT) {
-      __builtin_amdgcn_sched_barrier(0);
-      n2_of(Bc, pcrow, x4);
-    }
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (PT) n2_of(Bc, pcrow, x4);
     a0 = f32x16{};
     a1 = f32x16{};
-    bf16x8 cur = op(0), nxt = op(1);
+    bf16x8 opc[G], opn[G];
 #pragma unroll
-    for (int k = 0; k < KS; ++k) {
-      const bf16x8 nn = k + 2 < KS ? op(k + 2) : nxt;
-      a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur, qop[0][k], a0, 0, 0, 0);
-      a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur, qop[1][k], a1, 0, 0, 0);
-      if constexpr (PT) {
+    for (int g = 0; g < G; ++g) opc[g] = op(g);
 #pragma unroll
-        for (int r = done(k); r < done(k + 1); ++r) pt_elem(r, p0, p1, x4, m0, m1);
+    for (int b = 0; b < NB; ++b) {
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int k = b * G + g;
+        a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(opc[g], qop[0][k], a0, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(opc[g], qop[1][k], a1, 0, 0, 0);
+        if (g == 0) {
+          // copies: chunk c_lo + q at batch q NB / c_n; |x|^2 or slack
+          // terms at batch 0 of sub-tile 0
+#pragma unroll
+          for (int q = 0; q < c_n; ++q)
+            if (b == q * NB / c_n) issue_chunk(r0n, BNc, c_lo + q);
+          if (SUB == 0 && b == 0) issue_stats(r0n, BNc);
+          if (b + 1 < NB) {
+#pragma unroll
+            for (int h = 0; h < G; ++h) opn[h] = op((b + 1) * G + h);
+          }
+        }
+        if constexpr (PT) {
+#pragma unroll
+          for (int r = done(k); r < done(k + 1); ++r) pt_elem(r, p0, p1, x4, m0, m1);
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
-      // chunk c_lo + q at k-step 1 + q KS / c_n; |x|^2 or slack terms at
-      // k-step 0 of sub-tile 0
+      if (b + 1 < NB) {
 #pragma unroll
-      for (int q = 0; q < c_n; ++q)
-        if (k == 1 + q * KS / c_n) issue_chunk(r0n, BNc, c_lo + q);
-      if (SUB == 0 && k == 0) issue_stats(r0n, BNc);
-      cur = nxt;
-      nxt = nn;
+        for (int g = 0; g < G; ++g) opc[g] = opn[g];
+      }
     }
   };
   auto pretest = [&](auto Bc, int crow, const f32x16 &a0, const f32x16 &a1, float4 (&x4)[4], float &m0, float &m1) {
@@ -416,19 +441,22 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   // the query's histogram.  Otherwise every pair that passes the pre-test
   // goes, as {row, xn2 - 2 s, query}, to the wave's slice of the region:
   // k_bf16_scatter tests it with its own slack.  Per pair: one FMA and a
-  // ballot, and a uniform skip when no lane passes.
+  // ballot, and a uniform skip when no lane passes (the passing case is laid
+  // out of line, so the checks stay contiguous in the instruction cache);
+  // only the query tiles t with a pass (h0, h1) are looked at.
   auto slow = [&](auto Bc, int crow, uint64_t r0, const f32x16 &a0, const f32x16 &a1, float C0, float C1,
-                  const float4 (&x4)[4]) {
+                  const float4 (&x4)[4], bool h0, bool h1) {
     const TileBuf &tbuf = buf(Bc);
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
+      if (!(t == 0 ? h0 : h1)) continue;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const float sv = t == 0 ? a0[r] : a1[r];
         const float v = fmaf(sv, -2.0f, x4[r >> 2][r & 3]);  // xn2 - 2 s
         const bool pass = v <= (t == 0 ? C0 : C1);
         const uint64_t m = __ballot(pass);
-        if (m == 0) continue;
+        if (__builtin_expect(m == 0, 1)) continue;
         const int tr = crow + 8 * (r >> 2) + 4 * half + (r & 3);
         const uint64_t grow = r0 + tr;
         if constexpr (HIST) {
@@ -511,11 +539,16 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     using S2 = std::integral_constant<int, 2>;
     mfma_sub(Bc, S0{}, A0, A1, NoPT{}, 0, A0, A1, xa, ma0, ma1, r0n, BN{});
     mfma_sub(Bc, S1{}, X0, X1, PT{}, 0, A0, A1, xa, ma0, ma1, r0n, BN{});
-    if (__ballot(ma0 <= C0 || ma1 <= C1)) slow(Bc, 0, r0, A0, A1, C0, C1, xa);
+    bool h0 = __ballot(ma0 <= C0) != 0, h1 = __ballot(ma1 <= C1) != 0;
+    if (__builtin_expect(h0 || h1, 0)) slow(Bc, 0, r0, A0, A1, C0, C1, xa, h0, h1);
     mfma_sub(Bc, S2{}, A0, A1, PT{}, 32, X0, X1, xb, mb0, mb1, r0n, BN{});
-    if (__ballot(mb0 <= C0 || mb1 <= C1)) slow(Bc, 32, r0, X0, X1, C0, C1, xb);
+    h0 = __ballot(mb0 <= C0) != 0;
+    h1 = __ballot(mb1 <= C1) != 0;
+    if (__builtin_expect(h0 || h1, 0)) slow(Bc, 32, r0, X0, X1, C0, C1, xb, h0, h1);
     pretest(Bc, 64, A0, A1, xa, ma0, ma1);
-    if (__ballot(ma0 <= C0 || ma1 <= C1)) slow(Bc, 64, r0, A0, A1, C0, C1, xa);
+    h0 = __ballot(ma0 <= C0) != 0;
+    h1 = __ballot(ma1 <= C1) != 0;
+    if (__builtin_expect(h0 || h1, 0)) slow(Bc, 64, r0, A0, A1, C0, C1, xa, h0, h1);
   };
   static_assert(kBuf == 3, "the loop below cycles three buffers");
   for (uint32_t j = 0; j < my_n; j += 3) {
