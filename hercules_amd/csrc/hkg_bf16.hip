@@ -321,7 +321,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   // wave + i W (this wave's i-th), and one load of |x|^2 (rows 0..63: wave
   // 0, 64..95: wave 1) or of the slack terms (waves 2, 3)
   constexpr int NCI = (NCH + kWaves - 1) / kWaves;  // series chunks per wave, at most
-  auto issue_chunk = [&](uint64_t r0, auto Bc, int i) {
+  auto issue_chunk = [&](uint64_t r0, auto Bc, int i) __attribute__((always_inline)) {
     TileBuf &tbuf = buf(Bc);
     const int c = wave + i * kWaves;
     if (NCH % kWaves == 0 || c < NCH) {
@@ -333,7 +333,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
       load_lds16(src, (lds_void *)(reinterpret_cast<char *>(&tbuf.x[0]) + c * 1024));
     }
   };
-  auto issue_stats = [&](uint64_t r0, auto Bc) {
+  auto issue_stats = [&](uint64_t r0, auto Bc) __attribute__((always_inline)) {
     TileBuf &tbuf = buf(Bc);
     const int part = wave & 1, rbase = part * 64;
     if (wave < 4 && (part == 0 || lane < kRowTile - 64)) {
@@ -343,7 +343,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
         load_lds16(xst + r0 + rbase + lane, (lds_void *)(reinterpret_cast<char *>(&tbuf.st[0]) + rbase * 16));
     }
   };
-  auto issue = [&](uint32_t tile, auto Bc) {
+  auto issue = [&](uint32_t tile, auto Bc) __attribute__((always_inline)) {
     const uint64_t r0 = (uint64_t)tile * kRowTile;
 #pragma unroll
     for (int i = 0; i < NCI; ++i) issue_chunk(r0, Bc, i);
@@ -354,13 +354,14 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   // 8(r/4) + 4 half + r%4 of the tile): m_t = min over the lane's 16 series
   // of xn2 - 2 s for query tile t; some pair passes (xn2 - 2 s <= C_t) iff
   // m_t <= C_t, as fminf skips NaN the way <= rejects it.
-  auto n2_of = [&](auto Bc, int crow, float4 (&x4)[4]) {
+  auto n2_of = [&](auto Bc, int crow, float4 (&x4)[4]) __attribute__((always_inline)) {
     const TileBuf &tbuf = buf(Bc);
 #pragma unroll
     for (int gi = 0; gi < 4; ++gi)
       x4[gi] = *reinterpret_cast<const float4 *>(&tbuf.n2[crow + 8 * gi + 4 * half]);
   };
-  auto pt_elem = [&](int r, const f32x16 &p0, const f32x16 &p1, const float4 (&x4)[4], float &m0, float &m1) {
+  auto pt_elem = [&](int r, const f32x16 &p0, const f32x16 &p1, const float4 (&x4)[4], float &m0,
+                     float &m1) __attribute__((always_inline)) {
     const float xr = x4[r >> 2][r & 3];
     const float v0 = fmaf(p0[r], -2.0f, xr), v1 = fmaf(p1[r], -2.0f, xr);
     m0 = r == 0 ? v0 : fminf(m0, v0);
@@ -375,15 +376,16 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   // BNc, a third per sub-tile, at batch starts: all waves issuing a whole
   // tile at once would queue on the CU's copy path and stall, while spread
   // over the MFMAs the copies proceed in the background.  With PT,
-  // the pre-test of the previous sub-tile (rows pcrow.., accumulators p0/p1)
-  // runs between the MFMAs, from the third k-step on (its accumulators are
-  // complete by then): the matrix cores stay busy while the wave, alone on
-  // its SIMD, does the vector work.
+  // the pre-test of the previous sub-tile (accumulators p0/p1, its |x|^2 in
+  // x4, read by the caller) runs between the MFMAs, from the third k-step
+  // on (its accumulators are complete by then): the matrix cores stay busy
+  // while the wave, alone on its SIMD, does the vector work.
   constexpr int G = KS % 4 == 0 ? 4 : (KS % 3 == 0 ? 3 : 2);  // k-steps per operand batch
   constexpr int NB = KS / G;
   static_assert(KS % G == 0, "operand batches");
-  auto mfma_sub = [&](auto Bc, auto SUBc, f32x16 &a0, f32x16 &a1, auto PTc, int pcrow, const f32x16 &p0,
-                      const f32x16 &p1, float4 (&x4)[4], float &m0, float &m1, uint64_t r0n, auto BNc) {
+  auto mfma_sub = [&](auto Bc, auto SUBc, f32x16 &a0, f32x16 &a1, auto PTc, const f32x16 &p0, const f32x16 &p1,
+                      const float4 (&x4)[4], float &m0, float &m1, uint64_t r0n,
+                      auto BNc) __attribute__((always_inline)) {
     constexpr bool PT = decltype(PTc)::value;
     constexpr int SUB = decltype(SUBc)::value;
     constexpr int c_lo = SUB * NCI / 3, c_n = (SUB + 1) * NCI / 3 - c_lo;  // this sub-tile's copy chunks
@@ -394,7 +396,6 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     // pre-test elements done after k-step k: 0 for k < 2, then spread evenly
     auto done = [](int k) { return k < 2 ? 0 : (k - 2) * 16 / (KS - 2); };
     __builtin_amdgcn_sched_barrier(0);
-    if constexpr (PT) n2_of(Bc, pcrow, x4);
     a0 = f32x16{};
     a1 = f32x16{};
     bf16x8 opc[G], opn[G];
@@ -431,12 +432,13 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
       }
     }
   };
-  auto pretest = [&](auto Bc, int crow, const f32x16 &a0, const f32x16 &a1, float4 (&x4)[4], float &m0, float &m1) {
-    n2_of(Bc, crow, x4);
+  auto pretest = [&](const f32x16 &a0, const f32x16 &a1, const float4 (&x4)[4], float &m0,
+                     float &m1) __attribute__((always_inline)) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) pt_elem(r, a0, a1, x4, m0, m1);
   };
-  // Pairs of a sub-tile whose pre-test passed (rare).  HIST: the pairs
+  // Pairs of a sub-tile whose pre-test passed (rare; reads no LDS, so it
+  // may run after the tile's buffer is reused).  HIST: the pairs
   // whose upper bound with their own slack is within the threshold go to
   // the query's histogram.  Otherwise every pair that passes the pre-test
   // goes, as {row, xn2 - 2 s, query}, to the wave's slice of the region:
@@ -444,9 +446,8 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   // ballot, and a uniform skip when no lane passes (the passing case is laid
   // out of line, so the checks stay contiguous in the instruction cache);
   // only the query tiles t with a pass (h0, h1) are looked at.
-  auto slow = [&](auto Bc, int crow, uint64_t r0, const f32x16 &a0, const f32x16 &a1, float C0, float C1,
-                  const float4 (&x4)[4], bool h0, bool h1) {
-    const TileBuf &tbuf = buf(Bc);
+  auto slow = [&](int crow, uint64_t r0, const f32x16 &a0, const f32x16 &a1, float C0, float C1,
+                  const float4 (&x4)[4], bool h0, bool h1) __attribute__((always_inline)) {
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       if (!(t == 0 ? h0 : h1)) continue;
@@ -461,7 +462,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
         const uint64_t grow = r0 + tr;
         if constexpr (HIST) {
           if (pass && grow < n) {
-            const float4 xsv = tbuf.st[tr];
+            const float4 xsv = xst[grow];
             const float ub = (v + qn2[t]) + (qe[t] * xsv.y + qbn[t] * xsv.z + kEvalSlack);
             if (ub <= thr[t]) {
               const float lo = hlo[t];
@@ -498,7 +499,14 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   if (my_n > 1) issue(tile_at(1, 0), B1{});
   // one tile: j in buffer Bc (= j % 3); issues tile j + 2 into the buffer
   // that held tile j - 1
-  auto step = [&](auto Bc, uint32_t j) {
+  // sub-tile 2 of the previous step (accumulators, |x|^2, pre-test
+  // constants, first row), tested during the next step's first MFMAs;
+  // nothing passes before the first tile
+  f32x16 Y0 = {}, Y1 = {};
+  float4 xc[4] = {};
+  float Cp0 = -FLT_MAX, Cp1 = -FLT_MAX;
+  uint64_t r0p = 0;
+  auto step = [&](auto Bc, uint32_t j) __attribute__((always_inline)) {
     constexpr int B = decltype(Bc)::value;
     using BN = std::integral_constant<int, (B + 2) % kBuf>;
     // this wave's loads of tile j have landed (those of tile j + 1 may still
@@ -524,37 +532,53 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     // copies every step, and no branch among the MFMAs)
     const uint64_t r0n = (uint64_t)tile_at(j + 2, j) * kRowTile;
     const uint64_t r0 = (uint64_t)tile_at(j, j) * kRowTile;
+    using PT = std::true_type;
+    using S0 = std::integral_constant<int, 0>;
+    using S1 = std::integral_constant<int, 1>;
+    using S2 = std::integral_constant<int, 2>;
+    f32x16 A0, A1, X0, X1;
+    float4 xa[4], xb[4];
+    float m0, m1;
+    bool h0, h1;
+    // sub-tiles 0..2, the MFMAs of one beside the pre-test of the one
+    // before (for sub-tile 0: the previous step's sub-tile 2)
+    mfma_sub(Bc, S0{}, A0, A1, PT{}, Y0, Y1, xc, m0, m1, r0n, BN{});
+    h0 = __ballot(m0 <= Cp0) != 0;
+    h1 = __ballot(m1 <= Cp1) != 0;
+    if (__builtin_expect(h0 || h1, 0)) slow(64, r0p, Y0, Y1, Cp0, Cp1, xc, h0, h1);
     // pre-test constant: xn2 - 2 s <= C  <=>  bound with the tile's largest slack <= threshold
     const float2 tsl = slack_at(j);
     const float C0 = (thr[0] - qn2[0]) + (qe[0] * tsl.x + qbn[0] * tsl.y + kEvalSlack);
     const float C1 = (thr[1] - qn2[1]) + (qe[1] * tsl.x + qbn[1] * tsl.y + kEvalSlack);
-    using NoPT = std::false_type;
-    using PT = std::true_type;
-    f32x16 A0, A1, X0, X1;
-    float4 xa[4], xb[4];
-    float ma0, ma1, mb0, mb1;
-    // sub-tiles 0..2, the MFMAs of one beside the pre-test of the previous
-    using S0 = std::integral_constant<int, 0>;
-    using S1 = std::integral_constant<int, 1>;
-    using S2 = std::integral_constant<int, 2>;
-    mfma_sub(Bc, S0{}, A0, A1, NoPT{}, 0, A0, A1, xa, ma0, ma1, r0n, BN{});
-    mfma_sub(Bc, S1{}, X0, X1, PT{}, 0, A0, A1, xa, ma0, ma1, r0n, BN{});
-    bool h0 = __ballot(ma0 <= C0) != 0, h1 = __ballot(ma1 <= C1) != 0;
-    if (__builtin_expect(h0 || h1, 0)) slow(Bc, 0, r0, A0, A1, C0, C1, xa, h0, h1);
-    mfma_sub(Bc, S2{}, A0, A1, PT{}, 32, X0, X1, xb, mb0, mb1, r0n, BN{});
-    h0 = __ballot(mb0 <= C0) != 0;
-    h1 = __ballot(mb1 <= C1) != 0;
-    if (__builtin_expect(h0 || h1, 0)) slow(Bc, 32, r0, X0, X1, C0, C1, xb, h0, h1);
-    pretest(Bc, 64, A0, A1, xa, ma0, ma1);
-    h0 = __ballot(ma0 <= C0) != 0;
-    h1 = __ballot(ma1 <= C1) != 0;
-    if (__builtin_expect(h0 || h1, 0)) slow(Bc, 64, r0, A0, A1, C0, C1, xa, h0, h1);
+    n2_of(Bc, 0, xa);
+    mfma_sub(Bc, S1{}, X0, X1, PT{}, A0, A1, xa, m0, m1, r0n, BN{});
+    h0 = __ballot(m0 <= C0) != 0;
+    h1 = __ballot(m1 <= C1) != 0;
+    if (__builtin_expect(h0 || h1, 0)) slow(0, r0, A0, A1, C0, C1, xa, h0, h1);
+    n2_of(Bc, 32, xb);
+    mfma_sub(Bc, S2{}, Y0, Y1, PT{}, X0, X1, xb, m0, m1, r0n, BN{});
+    h0 = __ballot(m0 <= C0) != 0;
+    h1 = __ballot(m1 <= C1) != 0;
+    if (__builtin_expect(h0 || h1, 0)) slow(32, r0, X0, X1, C0, C1, xb, h0, h1);
+    // sub-tile 2 is tested during the next step; its |x|^2 is read now, as
+    // the next barrier frees this buffer for new copies
+    n2_of(Bc, 64, xc);
+    Cp0 = C0;
+    Cp1 = C1;
+    r0p = r0;
   };
   static_assert(kBuf == 3, "the loop below cycles three buffers");
   for (uint32_t j = 0; j < my_n; j += 3) {
     step(B0{}, j);
     if (j + 1 < my_n) step(B1{}, j + 1);
     if (j + 2 < my_n) step(B2{}, j + 2);
+  }
+  // the last tile's sub-tile 2
+  {
+    float m0, m1;
+    pretest(Y0, Y1, xc, m0, m1);
+    const bool h0 = __ballot(m0 <= Cp0) != 0, h1 = __ballot(m1 <= Cp1) != 0;
+    if (h0 || h1) slow(64, r0p, Y0, Y1, Cp0, Cp1, xc, h0, h1);
   }
   // the last steps' copies (re-reads into free buffers) land before the
   // block's LDS is released
