@@ -51,6 +51,7 @@ constexpr uint32_t kRegionCap = 65536;     // survivors per filter block before 
 constexpr float kEvalSlack = 1.0e-3f;      // FP32 evaluation of the bound
 constexpr uint32_t kBf16SurvivorCap = 524288;  // survivors kept per query (8 B each)
 constexpr float kHistLo = 0.6f;            // histogram range: [0.6 bound, threshold]
+constexpr uint32_t kPaceSpins = 4096;      // bounded wait for sibling filter blocks (s_sleep 4 each)
 
 __device__ __forceinline__ uint16_t to_bf16_bits(float f)
 {
@@ -197,7 +198,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
               const uint32_t *__restrict__ tiles, uint32_t tile_stride, const uint32_t *__restrict__ ntiles_g,
               uint32_t scap, uint64_t *__restrict__ surv, uint32_t *__restrict__ scnt, const float *__restrict__ thr0,
               uint32_t *__restrict__ hist, uint4 *__restrict__ region, uint32_t region_cap,
-              uint32_t *__restrict__ region_cnt)
+              uint32_t *__restrict__ region_cnt, uint32_t *prog)
 {
   constexpr int L = KS * 16;
   constexpr int VPR = L / 8;                       // 16-byte vectors per row
@@ -499,6 +500,33 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   if (my_n > 1) issue(tile_at(1, 0), B1{});
   // one tile: j in buffer Bc (= j % 3); issues tile j + 2 into the buffer
   // that held tile j - 1
+  // Sibling blocks (the same x in the other block groups, so on the same
+  // XCD) walk the same tiles.  Every 64 tiles a block publishes its
+  // progress and waits until its siblings have reached the same tile, so
+  // that one block's copies from HBM are the others' L2 hits and the
+  // batch reads the series about once instead of once per group.  The
+  // wait is bounded: a sibling that is not resident (CUs held by another
+  // kernel) costs one timeout, after which the block stops waiting.
+  // Results never depend on it.
+  const bool pace_grid = tile_stride == 0 && gridDim.y > 1;
+  bool pace = pace_grid;  // read by wave 0 only
+  auto keep_pace = [&](uint32_t j) __attribute__((always_inline)) {
+    if (!pace_grid) return;  // uniform over the block
+    if (wave == 0 && pace) {
+      if (lane == 0) __hip_atomic_store(&prog[blk], j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const bool sib = (uint32_t)lane < gridDim.y && (uint32_t)lane != blockIdx.y;
+      const uint32_t *p = prog + (sib ? lane * gridDim.x + blockIdx.x : 0u);
+      uint32_t spins = 0;
+      bool behind;
+      do {
+        const uint32_t v = sib ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0xFFFFFFFFu;
+        behind = __ballot(v < j) != 0;
+        if (behind) __builtin_amdgcn_s_sleep(4);
+      } while (behind && ++spins < kPaceSpins);
+      if (behind) pace = false;
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  };
   // sub-tile 2 of the previous step (accumulators, |x|^2, pre-test
   // constants, first row), tested during the next step's first MFMAs;
   // nothing passes before the first tile
@@ -526,6 +554,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
       cid = nid;
       cts = nts;
       fetch_chunk((j >> 6) + 1, nid, nts);
+      keep_pace(j);
     }
     // the tile two ahead, copied during this step's MFMAs (past the end
     // of the list, tile 0 again into the free buffer: the same count of
@@ -581,8 +610,10 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     if (h0 || h1) slow(64, r0p, Y0, Y1, Cp0, Cp1, xc, h0, h1);
   }
   // the last steps' copies (re-reads into free buffers) land before the
-  // block's LDS is released
+  // block's LDS is released; siblings no longer wait for this block
   __builtin_amdgcn_s_waitcnt(0);
+  if (pace_grid && wave == 0 && lane == 0)
+    __hip_atomic_store(&prog[blk], 0xFFFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (!HIST && lane == 0) region_cnt[(uint64_t)blk * kWaves + wave] = wcnt < wcap ? wcnt : wcap;
 }
 
@@ -660,21 +691,23 @@ static uint32_t bf16_grid_x(uint32_t groups)
 void launch_bf16_filter(hipStream_t st, const DevIndex &dv, uint32_t nq, const uint16_t *Qb, const float4 *qst,
                         const uint32_t *bound, const uint32_t *qperm, const uint32_t *tiles,
                         const uint32_t *ntiles_g, bool shared_list, uint32_t scap, uint64_t *surv, uint32_t *scnt,
-                        const float *thr0, uint32_t *hist, uint4 *region, uint32_t *region_cnt)
+                        const float *thr0, uint32_t *hist, uint4 *region, uint32_t *region_cnt,
+                        uint32_t *prog)
 {
   const uint32_t stride = shared_list ? 0 : dv.ntiles;
   const uint32_t qpb = bf16_qpb(dv.L);
   const uint32_t groups = (nq + qpb - 1) / qpb;
   dim3 grid(bf16_grid_x(groups), groups);
+  launch_fill(st, prog, grid.x * grid.y, 0);
 #define HKG_BF16_LAUNCH(KS)                                                                                       \
   if (hist)                                                                                                       \
     hipLaunchKernelGGL((k_bf16_filter<KS, true>), grid, dim3(filter_waves(KS) * 64), 0, st, dv.n, nq, dv.Xb,     \
                        dv.xn2, dv.xst, dv.tile_slack, Qb, qst, bound, qperm, tiles, stride, ntiles_g, scap,       \
-                       surv, scnt, thr0, hist, region, kRegionCap, region_cnt);                                   \
+                       surv, scnt, thr0, hist, region, kRegionCap, region_cnt, prog);                             \
   else                                                                                                            \
     hipLaunchKernelGGL((k_bf16_filter<KS, false>), grid, dim3(filter_waves(KS) * 64), 0, st, dv.n, nq, dv.Xb,    \
                        dv.xn2, dv.xst, dv.tile_slack, Qb, qst, bound, qperm, tiles, stride, ntiles_g, scap,       \
-                       surv, scnt, thr0, hist, region, kRegionCap, region_cnt)
+                       surv, scnt, thr0, hist, region, kRegionCap, region_cnt, prog)
   switch (dv.L) {
   case 96: HKG_BF16_LAUNCH(6); break;
   case 128: HKG_BF16_LAUNCH(8); break;
@@ -876,7 +909,8 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
         hipMalloc((void **)&b->thr, (size_t)b->B * 4) != hipSuccess ||
         hipMalloc((void **)&b->hist, (size_t)b->B * kBf16Bins * 4) != hipSuccess ||
         hipMalloc((void **)&b->bf_region, (size_t)bf16_max_blocks() * kRegionCap * sizeof(uint4)) != hipSuccess ||
-        hipMalloc((void **)&b->bf_region_cnt, (size_t)bf16_max_blocks() * 4 * sizeof(uint32_t)) != hipSuccess) {
+        hipMalloc((void **)&b->bf_region_cnt, (size_t)bf16_max_blocks() * 4 * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void **)&b->bf_prog, (size_t)bf16_max_blocks() * sizeof(uint32_t)) != hipSuccess) {
       (void)hipGetLastError();
       set_error("no device memory for the BF16 path scratch");
       return HKG_ENOMEM;
@@ -928,7 +962,7 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
     // within about one bin of the k-th smallest upper bound
     launch_fill(st, b->hist, nq * kBf16Bins, 0);
     launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, kSharedList, b->bf_cap,
-                       b->bf_surv, b->bf_cnt, thr0, b->hist, b->bf_region, b->bf_region_cnt);
+                       b->bf_surv, b->bf_cnt, thr0, b->hist, b->bf_region, b->bf_region_cnt, b->bf_prog);
     if (ctr) {
       nt_pass1.resize(lists);
       HKG_HIP_TRY(hipMemcpyAsync(nt_pass1.data(), b->ntiles_g, lists * 4, hipMemcpyDeviceToHost, st));
@@ -939,7 +973,7 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
   }
   // pass 2: collect the survivors of the certified lower bound
   launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, kSharedList, b->bf_cap,
-                     b->bf_surv, b->bf_cnt, thr0, nullptr, b->bf_region, b->bf_region_cnt);
+                     b->bf_surv, b->bf_cnt, thr0, nullptr, b->bf_region, b->bf_region_cnt, b->bf_prog);
   if (pf) pf->mark(st, HKG_STAGE_REFINE);
   launch_bf16_refine(st, nq, dev.X, L, b->Q, b->kq, b->kmax, dev.ids, b->cap, b->cand, b->cand_cnt, b->bf_cap,
                      b->bf_surv, b->bf_cnt, thr0, b->bound, idmap64, b->out_ids, b->out_d2, overflow, b->counters);

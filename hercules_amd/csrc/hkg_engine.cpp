@@ -315,6 +315,7 @@ void QueryBuffers::free_bf16()
   dfree(hist);
   dfree(bf_region);
   dfree(bf_region_cnt);
+  dfree(bf_prog);
 }
 
 QueryBuffers::~QueryBuffers()

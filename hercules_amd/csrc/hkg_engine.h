@@ -95,8 +95,9 @@ struct QueryBuffers {
   uint32_t *ntiles_g = nullptr; // groups
   uint32_t *thr = nullptr;      // B: filter threshold (float bits)
   uint32_t *hist = nullptr;     // B x kBf16Bins: counts of series by BF16 upper bound
-  uint4 *bf_region = nullptr;   // per filter block: survivors {row, bound bits, query, 0}
-  uint32_t *bf_region_cnt = nullptr;  // per filter block: survivors in its region
+  uint4 *bf_region = nullptr;   // per filter block: pre-test passes {row, xn2 - 2 s bits, query, 0}
+  uint32_t *bf_region_cnt = nullptr;  // per filter wave: pre-test passes in its region slice
+  uint32_t *bf_prog = nullptr;  // per filter block: tiles done (sibling blocks keep pace)
   void free_bf16();
   bool busy = false;            // held by a query call (guarded by Index::mu_bufs)
   // captured stage sequence of the batched path
