@@ -442,7 +442,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   // may run after the tile's buffer is reused).  HIST: the pairs
   // whose upper bound with their own slack is within the threshold go to
   // the query's histogram.  Otherwise every pair that passes the pre-test
-  // goes, as {row, xn2 - 2 s, query}, to the wave's slice of the region:
+  // goes, as {row, xn2 - 2 s, query, query slot}, to the wave's slice of the region:
   // k_bf16_scatter tests it with its own slack.  Per pair: one FMA and a
   // ballot, and a uniform skip when no lane passes (the passing case is laid
   // out of line, so the checks stay contiguous in the instruction cache);
@@ -477,7 +477,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
           wcnt += (uint32_t)__popcll(m);
           if (pass) {
             if (pos < wcap) {
-              region[wbase + pos] = make_uint4((uint32_t)grow, __float_as_uint(v), qid[t], 0u);
+              region[wbase + pos] = make_uint4((uint32_t)grow, __float_as_uint(v), qid[t], (uint32_t)(t * 32 + l32));
             } else if (grow < n) {
               // the wave's slice is full: tested here, straight to the query's list
               const float4 xsv = xst[grow];
@@ -618,23 +618,49 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
 }
 
 // The pairs every filter wave found (one region slice each, {row, xn2 -
-// 2 s, query}): the lower bound with the row's own slack, and the pairs
-// within the query's threshold into its list, keyed by that bound.
-__global__ void k_bf16_scatter(const uint4 *__restrict__ region, uint32_t slice_cap,
-                               const uint32_t *__restrict__ region_cnt, uint64_t n, const float4 *__restrict__ xst,
-                               const float4 *__restrict__ qst, const float *__restrict__ thr0, uint32_t scap,
-                               uint64_t *__restrict__ surv, uint32_t *__restrict__ scnt)
+// 2 s, query, slot of the query among the wave's 64}): the lower bound with
+// the row's own slack, and the pairs within the query's threshold into its
+// list, keyed by that bound.  One block per slice: positions are counted
+// per query slot in LDS, then one global atomic per query reserves them.
+constexpr uint32_t kScatterMax = 16384;  // entries per slice (kRegionCap / filter waves)
+static_assert(kRegionCap / filter_waves(16) <= kScatterMax, "region slice larger than the scatter's LDS");
+__global__ void __launch_bounds__(256) k_bf16_scatter(const uint4 *__restrict__ region, uint32_t slice_cap,
+                                                      const uint32_t *__restrict__ region_cnt, uint64_t n,
+                                                      const float4 *__restrict__ xst, const float4 *__restrict__ qst,
+                                                      const float *__restrict__ thr0, uint32_t scap,
+                                                      uint64_t *__restrict__ surv, uint32_t *__restrict__ scnt)
 {
+  __shared__ uint32_t s_cnt[kQPerWave], s_q[kQPerWave], s_base[kQPerWave];
+  __shared__ uint16_t s_pos[kScatterMax];
+  __shared__ float s_lb[kScatterMax];
   const uint32_t sl = blockIdx.x;
-  const uint32_t cnt = region_cnt[sl];
+  const uint32_t cnt = min(region_cnt[sl], min(slice_cap, kScatterMax));
+  const uint4 *reg = region + (uint64_t)sl * slice_cap;
+  if (threadIdx.x < kQPerWave) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
   for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
-    const uint4 e = region[(uint64_t)sl * slice_cap + i];
-    if (e.x >= n) continue;  // padding rows of the last tile
-    const float4 xsv = xst[e.x], qs = qst[e.z];
-    const float lb = (__uint_as_float(e.y) + qs.x) - (qs.y * xsv.y + qs.z * xsv.z + kEvalSlack);
-    if (!(lb <= thr0[e.z])) continue;
-    const uint32_t pos = atomicAdd(&scnt[e.z], 1u);
-    if (pos < scap) surv[(uint64_t)e.z * scap + pos] = ((uint64_t)__float_as_uint(fmaxf(lb, 0.0f)) << 32) | e.x;
+    const uint4 e = reg[i];
+    uint16_t pos = 0xFFFFu;
+    if (e.x < n) {  // not a padding row of the last tile
+      const float4 xsv = xst[e.x], qs = qst[e.z];
+      const float lb = (__uint_as_float(e.y) + qs.x) - (qs.y * xsv.y + qs.z * xsv.z + kEvalSlack);
+      if (lb <= thr0[e.z]) {
+        pos = (uint16_t)atomicAdd(&s_cnt[e.w], 1u);
+        s_q[e.w] = e.z;
+        s_lb[i] = fmaxf(lb, 0.0f);
+      }
+    }
+    s_pos[i] = pos;
+  }
+  __syncthreads();
+  if (threadIdx.x < kQPerWave && s_cnt[threadIdx.x] > 0)
+    s_base[threadIdx.x] = atomicAdd(&scnt[s_q[threadIdx.x]], s_cnt[threadIdx.x]);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+    if (s_pos[i] == 0xFFFFu) continue;
+    const uint4 e = reg[i];
+    const uint32_t pos = s_base[e.w] + s_pos[i];
+    if (pos < scap) surv[(uint64_t)e.z * scap + pos] = ((uint64_t)__float_as_uint(s_lb[i]) << 32) | e.x;
   }
 }
 
