@@ -958,8 +958,8 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
   if (pf) pf->mark(st, HKG_STAGE_EAPCA);
   launch_best_leaf(st, nq, nleaf, b->lb, b->best);
   if (pf) pf->mark(st, HKG_STAGE_SELECT_LEAF);
-  launch_scan_best(st, nq, dev.X, L, b->Q, b->kq, b->best, dev.leaf_pos, dev.leaf_size, dev.ids, b->cap, b->cand,
-                   b->cand_cnt, b->bound, b->counters);
+  launch_scan_best_bf16(st, nq, dev.X, dev.Xb, dev.xn2, dev.max_leaf, L, b->Q, b->kq, b->best, dev.leaf_pos,
+                        dev.leaf_size, dev.ids, b->cap, b->cand, b->cand_cnt, b->bound, b->counters);
   if (pf) pf->mark(st, HKG_STAGE_APPROX);
   // batch order by first bound, leaves and row tiles each block group needs
   const uint32_t groups = (nq + qpb - 1) / qpb;

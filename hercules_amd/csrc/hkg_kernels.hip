@@ -431,6 +431,121 @@ __global__ void __launch_bounds__(kBlock) k_scan_best(const float *__restrict__ 
   if (threadIdx.x == 0) atomicAdd(&counters[kCtrApprox], (unsigned long long)size);
 }
 
+// Approximate phase of the BF16 path: in each of the query's best leaves the
+// rows are ranked by xn2 - 2 q.x~ (q in FP32, x~ the row's BF16 copy: half
+// the bytes of the rows), and only the M best-ranked ones get exact
+// distances (the reference's order, as the full scan).  The k-th smallest
+// of those is a k-th distance bound like the full scan's -- exact distances
+// of real rows -- at worst less tight when the ranking misorders rows near
+// the k-th.  Dynamic LDS: L floats of the query, then one ranking key per
+// row of the leaf, a region the exact refinement reuses once the rows are
+// chosen (so three blocks fit a CU).
+constexpr uint32_t kApproxMargin = 32;  // rows refined beyond k
+__global__ void __launch_bounds__(kBlock) k_scan_best_bf16(const float *__restrict__ X,
+                                                           const uint16_t *__restrict__ Xb,
+                                                           const float *__restrict__ xn2, uint32_t L,
+                                                           const float *__restrict__ Q,
+                                                           const uint32_t *__restrict__ kq,
+                                                           const uint32_t *__restrict__ best,
+                                                           const uint64_t *__restrict__ leaf_pos,
+                                                           const uint32_t *__restrict__ leaf_size,
+                                                           const uint32_t *__restrict__ ids, uint32_t cap,
+                                                           uint64_t *__restrict__ cand,
+                                                           uint32_t *__restrict__ cand_cnt,
+                                                           uint32_t *__restrict__ bound,
+                                                           unsigned long long *__restrict__ counters)
+{
+  extern __shared__ float dyn[];
+  float *qs = dyn;
+  uint32_t *key = reinterpret_cast<uint32_t *>(dyn + L);
+  RefineShared &rs = *reinterpret_cast<RefineShared *>(dyn + L);  // after the keys are done with
+  __shared__ TopK tk;
+  __shared__ uint64_t rows[kBlock];
+  __shared__ uint32_t scan[kBlock];
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t s_prefix, s_rank, s_nsel, s_ties;
+  const uint32_t q = blockIdx.x;
+  const int k = (int)kq[q];
+  const uint32_t leaf = best[q * kBestLeaves + blockIdx.y];
+  if (leaf == 0xFFFFFFFFu) return;  // uniform for the block
+  load_query(Q, q, L, qs);
+  topk_reset(tk);
+  const uint64_t pos = leaf_pos[leaf];
+  const uint32_t size = leaf_size[leaf];
+  __syncthreads();
+  // ranking keys: one row per thread, read 256 bytes (16 loads) at a time;
+  // the float is mapped to an unsigned key of the same order
+  const uint32_t nc = L / 8;  // 16-byte chunks per row
+  for (uint32_t i = threadIdx.x; i < size; i += kBlock) {
+    const uint4 *xr = reinterpret_cast<const uint4 *>(Xb + (pos + i) * L);
+    float dot = 0.0f;
+    for (uint32_t c0 = 0; c0 < nc; c0 += 16) {
+      uint4 v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) v[u] = c0 + u < nc ? xr[c0 + u] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        if (c0 + u >= nc) break;
+        const float4 qa = *reinterpret_cast<const float4 *>(qs + 8 * (c0 + u));
+        const float4 qb = *reinterpret_cast<const float4 *>(qs + 8 * (c0 + u) + 4);
+        dot = fmaf(qa.x, __uint_as_float(v[u].x << 16), dot);
+        dot = fmaf(qa.y, __uint_as_float(v[u].x & 0xFFFF0000u), dot);
+        dot = fmaf(qa.z, __uint_as_float(v[u].y << 16), dot);
+        dot = fmaf(qa.w, __uint_as_float(v[u].y & 0xFFFF0000u), dot);
+        dot = fmaf(qb.x, __uint_as_float(v[u].z << 16), dot);
+        dot = fmaf(qb.y, __uint_as_float(v[u].z & 0xFFFF0000u), dot);
+        dot = fmaf(qb.z, __uint_as_float(v[u].w << 16), dot);
+        dot = fmaf(qb.w, __uint_as_float(v[u].w & 0xFFFF0000u), dot);
+      }
+    }
+    const uint32_t b = __float_as_uint(fmaf(-2.0f, dot, xn2[pos + i]));
+    key[i] = b ^ ((b >> 31) ? 0xFFFFFFFFu : 0x80000000u);
+  }
+  __syncthreads();
+  // the M-th smallest key: radix select, 8 bits per pass from the top
+  const uint32_t M = min(size, (uint32_t)min(k + (int)kApproxMargin, kBlock));
+  if (threadIdx.x == 0) {
+    s_prefix = 0;
+    s_rank = M;
+    s_nsel = 0;
+    s_ties = 0;
+  }
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t prefix = s_prefix, hi = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
+    for (uint32_t i = threadIdx.x; i < size; i += kBlock) {
+      const uint32_t kk = key[i];
+      if ((kk & hi) == prefix) atomicAdd(&hist[(kk >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t cum = 0, d = 0;
+      const uint32_t r = s_rank;
+      for (; d < 255; ++d) {
+        if (cum + hist[d] >= r) break;
+        cum += hist[d];
+      }
+      s_prefix = prefix | (d << shift);
+      s_rank = r - cum;
+    }
+    __syncthreads();
+  }
+  // the rows below the M-th key, and as many equal to it as complete M
+  const uint32_t kth = s_prefix, ties = s_rank;
+  for (uint32_t i = threadIdx.x; i < size; i += kBlock) {
+    const uint32_t kk = key[i];
+    if (kk < kth || (kk == kth && atomicAdd(&s_ties, 1u) < ties)) rows[atomicAdd(&s_nsel, 1u)] = pos + i;
+  }
+  __syncthreads();
+  uint32_t nref = 0;
+  refine_rows(X, L, qs, rows, (int)s_nsel, ids, tk.kth_bits, tk, rs, nref);
+  __syncthreads();
+  topk_merge(tk, scan, k);
+  flush_topk(tk, k, q, cap, cand, cand_cnt, bound);
+  if (threadIdx.x == 0) atomicAdd(&counters[kCtrApprox], (unsigned long long)size);
+}
+
 // dense work list of every (query, leaf) pair: key = EAPCA bound bits when
 // the pair survives the first bound, else all ones; the value is the pair's
 // index q * nleaf + leaf.  Sorting by key gives a best-first order over all
@@ -880,6 +995,21 @@ void launch_eapca(hipStream_t st, uint32_t nq, uint32_t nleaf, uint32_t L, const
 void launch_best_leaf(hipStream_t st, uint32_t nq, uint32_t nleaf, const float *lb, uint32_t *best)
 {
   hipLaunchKernelGGL(k_best_leaf, dim3(nq), dim3(kBlock), 0, st, nleaf, lb, best);
+}
+
+void launch_scan_best_bf16(hipStream_t st, uint32_t nq, const float *X, const uint16_t *Xb, const float *xn2,
+                           uint32_t max_leaf, uint32_t L, const float *Q, const uint32_t *kq, const uint32_t *best,
+                           const uint64_t *leaf_pos, const uint32_t *leaf_size, const uint32_t *ids, uint32_t cap,
+                           uint64_t *cand, uint32_t *cand_cnt, uint32_t *bound, unsigned long long *counters)
+{
+  const size_t dyn = (size_t)L * sizeof(float) + std::max<size_t>((size_t)max_leaf * sizeof(uint32_t), sizeof(RefineShared));
+  if (dyn > kApproxMaxDynLds) {
+    // leaves too large for the ranking keys in LDS: every row exactly
+    launch_scan_best(st, nq, X, L, Q, kq, best, leaf_pos, leaf_size, ids, cap, cand, cand_cnt, bound, counters);
+    return;
+  }
+  hipLaunchKernelGGL(k_scan_best_bf16, dim3(nq, kBestLeaves), dim3(kBlock), dyn, st, X, Xb, xn2, L, Q, kq, best,
+                     leaf_pos, leaf_size, ids, cap, cand, cand_cnt, bound, counters);
 }
 
 void launch_scan_best(hipStream_t st, uint32_t nq, const float *X, uint32_t L, const float *Q,
