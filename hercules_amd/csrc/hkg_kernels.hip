@@ -884,7 +884,19 @@ __global__ void __launch_bounds__(kBlock) k_bf16_refine(
   const int first_bin = (int)s_bin;
   uint32_t nref = 0;
   for (int pass = 0; pass < 2; ++pass) {
+    // the list is read four rounds at a time (four loads in flight per
+    // thread): a round's work is short, so one load per round would leave
+    // a long list waiting on memory latency
+    uint64_t ev[4];
     for (uint32_t t0 = 0; t0 < ns; t0 += kBlock) {
+      const uint32_t u = (t0 / kBlock) & 3;
+      if (u == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const uint32_t iv = t0 + v * kBlock + threadIdx.x;
+          ev[v] = iv < ns ? sv[iv] : 0;
+        }
+      }
       if (threadIdx.x == 0) s_nsel = 0;
       __syncthreads();
       const uint32_t bnd = tk.kth_bits;
@@ -892,7 +904,7 @@ __global__ void __launch_bounds__(kBlock) k_bf16_refine(
       bool take = false;
       uint64_t e = 0;
       if (i < ns) {
-        e = sv[i];
+        e = u == 0 ? ev[0] : u == 1 ? ev[1] : u == 2 ? ev[2] : ev[3];
         const float lbv = __uint_as_float((uint32_t)(e >> 32));
         const int bin = min(kSurvBins - 1, (int)(lbv * scale));
         const bool in_first = bin <= first_bin;
