@@ -16,10 +16,14 @@
  *                             reference's own table
  *   orc_knn_bruteforce        the reference CLI's answers (serial, SIMS and
  *                             PSCAN paths) and brute force in FP64
+ *   orc_knn_mode              the reference CLI's answers with --epsilon /
+ *                             --approx-stop-condition / --nprobes on the
+ *                             committed golden indexes
  *   generator                 a specification, not a reference function: its
  *                             invariants (z-normalisation, determinism) and
  *                             bit-equality with the product's HIP generator
  */
+#define _GNU_SOURCE /* qsort_r */
 #include "hkg_oracle.h"
 
 #include <float.h>
@@ -479,6 +483,371 @@ int orc_knn_bruteforce(const float *X, uint64_t N, int L, const float *Q, uint32
     free(jobs[t].id);
     free(jobs[t].cur);
   }
+  free(jobs);
+  free(th);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* approximate query modes (epsilon, ng) on an index tree                    */
+/* ------------------------------------------------------------------------ */
+
+/* pqueue.c:63-200 with cmp_pri = (next > curr) (hercules_query_engine.h:36-39):
+   a binary min-heap in d[1..size-1] ordered by the bound only, so items of
+   equal bound leave in the order the heap's moves give them */
+struct orc_pq_item {
+  float pri;
+  int32_t node;
+};
+struct orc_pq {
+  struct orc_pq_item *d;
+  size_t size; /* 1 + number of items */
+};
+
+static void orc_pq_bubble_up(struct orc_pq *q, size_t i)
+{
+  struct orc_pq_item mv = q->d[i];
+  while (i > 1 && (double)q->d[i >> 1].pri > (double)mv.pri) {
+    q->d[i] = q->d[i >> 1];
+    i >>= 1;
+  }
+  q->d[i] = mv;
+}
+
+static size_t orc_pq_min_child(const struct orc_pq *q, size_t i)
+{
+  size_t c = i << 1;
+  if (c >= q->size) return 0;
+  if (c + 1 < q->size && (double)q->d[c].pri > (double)q->d[c + 1].pri) ++c;
+  return c;
+}
+
+static void orc_pq_percolate_down(struct orc_pq *q, size_t i)
+{
+  struct orc_pq_item mv = q->d[i];
+  size_t c;
+  while ((c = orc_pq_min_child(q, i)) && (double)mv.pri > (double)q->d[c].pri) {
+    q->d[i] = q->d[c];
+    i = c;
+  }
+  q->d[i] = mv;
+}
+
+static void orc_pq_insert(struct orc_pq *q, float pri, int32_t node)
+{
+  size_t i = q->size++;
+  q->d[i].pri = pri;
+  q->d[i].node = node;
+  orc_pq_bubble_up(q, i);
+}
+
+static int orc_pq_pop(struct orc_pq *q, struct orc_pq_item *out)
+{
+  if (q->size == 1) return 0;
+  *out = q->d[1];
+  q->d[1] = q->d[--q->size];
+  orc_pq_percolate_down(q, 1);
+  return 1;
+}
+
+enum { OF_LEAF = 0, OF_LEFT, OF_RIGHT, OF_SEGOFF, OF_NSEG, OF_FROM, OF_TO, OF_IDX, OF_POS, OF_SIZE };
+
+struct orc_tree {
+  const int64_t *nd;
+  const float *split_value;
+  const int16_t *seg_end;
+  const float *ind;
+  const float *X;
+  const uint8_t *sax;
+  int L;
+  uint32_t nnodes;
+  uint32_t nleaves;    /* leaves with rows (the reference's leaves_counter) */
+  uint64_t nseries;    /* rows under the root */
+  float bp[255];
+};
+
+static const int64_t *orc_node(const struct orc_tree *t, int32_t i) { return t->nd + (int64_t)i * ORC_NODE_FIELDS; }
+
+/* calculate_node_min_distance (hercules_node.c:879-924) in the FMA build */
+static float orc_node_lb(const struct orc_tree *t, int32_t i, const float *q)
+{
+  const int64_t *n = orc_node(t, i);
+  if (n[OF_NSEG] == 0) return 0.0f; /* empty leaf: no synopsis on disk */
+  return orc_lb_eapca(q, (int)n[OF_NSEG], t->seg_end + n[OF_SEGOFF], t->ind + 4 * n[OF_SEGOFF], 1);
+}
+
+/* the running answer of a search: distinct distances, FLT_MAX past the
+   filled ranks (queue_bounded_sorted_insert, hercules_query_engine.c:4034-4082) */
+struct orc_run {
+  float *d;
+  uint64_t *id;
+  uint32_t cur, k;
+};
+
+static float orc_kth(const struct orc_run *r) { return r->cur == r->k ? r->d[r->k - 1] : FLT_MAX; }
+
+/* calculate_node_knn_distance (hercules_node.c:708-790), the leaf loop of
+   approximate_greedy_knn_search (hercules_query_engine.c:111-139) and of
+   exact_de_knn_search_noapprox (:2484-2500): every row against the k-th
+   distance of the moment, inserted when below it.  Returns the k-th
+   distance read before the last row (the value the caller's variable
+   keeps, :2486) or `kth_in` for an empty leaf. */
+static float orc_scan_leaf(const struct orc_tree *t, int32_t i, const float *q, struct orc_run *r, float kth_in)
+{
+  const int64_t *n = orc_node(t, i);
+  float bound = kth_in;
+  for (int64_t j = 0; j < n[OF_SIZE]; ++j) {
+    uint64_t row = (uint64_t)n[OF_POS] + (uint64_t)j;
+    bound = orc_kth(r);
+    float d = orc_ed2(q, t->X + row * (uint64_t)t->L, t->L, bound);
+    if (d < bound) orc_topk_insert(r->d, r->id, &r->cur, r->k, d, row);
+  }
+  return bound;
+}
+
+/* node_split_policy_route_to_left (hercules_node_split.c:172-194) */
+static int orc_route_left(const struct orc_tree *t, int32_t i, const float *q)
+{
+  const int64_t *n = orc_node(t, i);
+  float ind[2];
+  orc_seg_mean_std(q, (int)n[OF_FROM], (int)n[OF_TO], &ind[0], &ind[1]);
+  return ind[n[OF_IDX] & 1] < t->split_value[i];
+}
+
+struct orc_mode_params {
+  uint32_t k, nprobes;
+  float epsilon, eapca_th, sax_th;
+  int approx_budget;
+};
+
+struct orc_cand {
+  int32_t node;
+  float lb;
+};
+
+static int orc_cand_by_pos(const void *a, const void *b, void *tv)
+{
+  const struct orc_tree *t = (const struct orc_tree *)tv;
+  int64_t pa = orc_node(t, ((const struct orc_cand *)a)->node)[OF_POS];
+  int64_t pb = orc_node(t, ((const struct orc_cand *)b)->node)[OF_POS];
+  return pa < pb ? -1 : (pa > pb ? 1 : 0);
+}
+
+/* one query of a mode: marks the leaves whose rows the reference's search
+   examines, then answers over exactly those rows.  info[4]: leaves scanned
+   before the candidate stage, candidate leaves, the branch the candidates
+   took (0 none, 1 sequential, 2 iSAX + parallel refinement), candidate
+   leaves scanned. */
+static void orc_mode_query(const struct orc_tree *t, const float *q, const struct orc_mode_params *p,
+                           uint8_t *mark, struct orc_pq *pq, struct orc_cand *cand, uint64_t *ids, float *d2,
+                           uint32_t *info)
+{
+  const uint32_t k = p->k;
+  struct orc_run run = {malloc(sizeof(float) * k), malloc(sizeof(uint64_t) * k), 0, k};
+  memset(mark, 0, t->nnodes);
+  memset(info, 0, 4 * sizeof(uint32_t));
+  pq->size = 1;
+  uint32_t nvis = 0, ncand = 0;
+  struct orc_pq_item it;
+  if (p->nprobes == 0) {
+    /* approximate_greedy_knn_search (hercules_query_engine.c:70-174, with the
+       count kept in the caller's variable): best-first from the root, every
+       row of each popped leaf, up to approx_budget leaves */
+    orc_pq_insert(pq, orc_node_lb(t, 0, q), 0);
+    while (orc_pq_pop(pq, &it)) {
+      if (it.pri > orc_kth(&run)) break; /* the popped item is dropped */
+      const int64_t *n = orc_node(t, it.node);
+      if (n[OF_LEAF]) {
+        (void)orc_scan_leaf(t, it.node, q, &run, FLT_MAX);
+        mark[it.node] = 1;
+        ++nvis;
+        if ((int)nvis >= p->approx_budget) break;
+      } else {
+        float kth = orc_kth(&run);
+        for (int c = 0; c < 2; ++c) {
+          int32_t ch = (int32_t)n[OF_LEFT + c];
+          float lb = orc_node_lb(t, ch, q);
+          if (lb < kth) orc_pq_insert(pq, lb, ch);
+        }
+      }
+    }
+    /* exact_de_knn_search_psq_lf_sims_adaptive (:836-891): the same queue,
+       bounds against kth/(1+epsilon) of the approximate answer; the leaves
+       that pass are the candidates.  A leaf scanned above is never pushed
+       again (a node has one parent, expanded once), so the test against the
+       approximate leaf (:866, :878) changes nothing. */
+    const float kth0 = orc_kth(&run);
+    const float thr = kth0 / (1.0f + p->epsilon);
+    while (orc_pq_pop(pq, &it)) {
+      if (it.pri > thr) break;
+      const int64_t *n = orc_node(t, it.node);
+      if (n[OF_LEAF]) {
+        cand[ncand].node = it.node;
+        cand[ncand].lb = it.pri;
+        ++ncand;
+      } else {
+        for (int c = 0; c < 2; ++c) {
+          int32_t ch = (int32_t)n[OF_LEFT + c];
+          float lb = orc_node_lb(t, ch, q);
+          if (lb < thr) orc_pq_insert(pq, lb, ch);
+        }
+      }
+    }
+    info[1] = ncand;
+    if (ncand) {
+      /* :893-921: sequential when the leaf pruning is weak, else the iSAX
+         pass counts the rows whose bound is within kth0 (:2171-2205) and
+         goes sequential when that pruning is weak (:967-978) */
+      qsort_r(cand, ncand, sizeof(struct orc_cand), orc_cand_by_pos, (void *)t);
+      float flp = (float)(1.0 - (double)ncand / (double)(float)t->nleaves);
+      int sequential = flp < p->eapca_th;
+      if (!sequential) {
+        float paa[16];
+        orc_paa(q, t->L, 16, paa);
+        uint64_t lab = 0;
+        for (uint32_t c = 0; c < ncand; ++c) {
+          const int64_t *n = orc_node(t, cand[c].node);
+          if (!(cand[c].lb <= kth0)) continue;
+          for (int64_t j = 0; j < n[OF_SIZE]; ++j) {
+            float lbs = orc_lb_sax(paa, t->sax + ((uint64_t)n[OF_POS] + (uint64_t)j) * 16, t->L, t->bp);
+            if (lbs <= kth0) ++lab;
+          }
+        }
+        float slp = (float)(1.0 - (double)lab / (double)(float)t->nseries);
+        sequential = slp < p->sax_th;
+      }
+      info[2] = sequential ? 1 : 2;
+      if (sequential) {
+        /* exact_de_knn_search_noapprox (:2450-2507): candidates in file
+           order, each tested against the k-th distance its loop variable
+           holds, which lags the last insertion of the previous leaf */
+        float kth_b = orc_kth(&run);
+        for (uint32_t c = 0; c < ncand; ++c) {
+          if (cand[c].lb <= kth_b / (1.0f + p->epsilon)) {
+            kth_b = orc_scan_leaf(t, cand[c].node, q, &run, kth_b);
+            mark[cand[c].node] = 1;
+            ++info[3];
+          }
+        }
+      } else {
+        /* the iSAX and refinement workers (:1960-2030, :2140-2213) prune
+           with exact bounds only: every candidate's rows count */
+        for (uint32_t c = 0; c < ncand; ++c) mark[cand[c].node] = 1;
+        info[3] = ncand;
+      }
+    }
+  } else {
+    /* exact_ng_knn_search (hercules_query_engine.c:2623-2799): the leaf the
+       split policies route the query to (approximate_knn_search, :176-223),
+       then best-first from the root until nprobes leaves have been scanned,
+       stopping at a bound above the first leaf's best distance */
+    int32_t i = 0;
+    while (!orc_node(t, i)[OF_LEAF]) i = (int32_t)orc_node(t, i)[orc_route_left(t, i, q) ? OF_LEFT : OF_RIGHT];
+    const int32_t first = i;
+    (void)orc_scan_leaf(t, first, q, &run, FLT_MAX);
+    mark[first] = 1;
+    nvis = 1;
+    const float best = run.cur ? run.d[0] : FLT_MAX;
+    const int32_t best_leaf = run.cur ? first : -1;
+    orc_pq_insert(pq, orc_node_lb(t, 0, q), 0);
+    while (orc_pq_pop(pq, &it) && nvis < p->nprobes) {
+      if (it.pri > best) break;
+      const int64_t *n = orc_node(t, it.node);
+      if (n[OF_LEAF]) {
+        (void)orc_scan_leaf(t, it.node, q, &run, FLT_MAX);
+        mark[it.node] = 1;
+        ++nvis;
+      } else {
+        float kth = orc_kth(&run);
+        for (int c = 0; c < 2; ++c) {
+          int32_t ch = (int32_t)n[OF_LEFT + c];
+          float lb = orc_node_lb(t, ch, q);
+          if (lb < kth && ch != best_leaf) orc_pq_insert(pq, lb, ch);
+        }
+      }
+    }
+  }
+  info[0] = nvis;
+  /* the answer: the k smallest distinct distances over the marked leaves'
+     rows (the streaming inserts above keep exactly those), each with the
+     smallest row of that distance: marked leaves again, rows ascending */
+  run.cur = 0;
+  uint32_t nm = 0;
+  for (uint32_t j = 0; j < t->nnodes; ++j)
+    if (mark[j]) {
+      cand[nm].node = (int32_t)j;
+      cand[nm].lb = 0.0f;
+      ++nm;
+    }
+  qsort_r(cand, nm, sizeof(struct orc_cand), orc_cand_by_pos, (void *)t);
+  for (uint32_t a = 0; a < nm; ++a) (void)orc_scan_leaf(t, cand[a].node, q, &run, FLT_MAX);
+  for (uint32_t r = 0; r < k; ++r) {
+    d2[r] = r < run.cur ? run.d[r] : FLT_MAX;
+    ids[r] = r < run.cur ? run.id[r] : UINT64_MAX;
+  }
+  free(run.d);
+  free(run.id);
+}
+
+struct mode_job {
+  const struct orc_tree *t;
+  const struct orc_mode_params *p;
+  uint32_t q0, q1;
+  const float *Q;
+  uint64_t *ids;
+  float *d2;
+  uint32_t *info;
+};
+
+static void *orc_mode_worker(void *arg)
+{
+  struct mode_job *j = (struct mode_job *)arg;
+  const uint32_t nn = j->t->nnodes;
+  uint8_t *mark = malloc(nn);
+  struct orc_pq pq = {malloc(sizeof(struct orc_pq_item) * (nn + 2)), 1};
+  struct orc_cand *cand = malloc(sizeof(struct orc_cand) * (nn + 1));
+  const uint32_t k = j->p->k;
+  for (uint32_t qi = j->q0; qi < j->q1; ++qi)
+    orc_mode_query(j->t, j->Q + (uint64_t)qi * j->t->L, j->p, mark, &pq, cand, j->ids + (uint64_t)qi * k,
+                   j->d2 + (uint64_t)qi * k, j->info + 4 * (uint64_t)qi);
+  free(mark);
+  free(pq.d);
+  free(cand);
+  return NULL;
+}
+
+int orc_knn_mode(const int64_t *nodes, const float *split_value, uint32_t nnodes, const int16_t *seg_end,
+                 const float *ind, const float *X, const uint8_t *sax, int L, const float *Q, uint32_t nq,
+                 uint32_t k, float epsilon, int approx_budget, uint32_t nprobes, float eapca_th, float sax_th,
+                 uint64_t *ids, float *d2, uint32_t *info, int nthreads)
+{
+  if (k == 0 || L % 8 != 0 || nnodes == 0 || !(epsilon >= 0.0f)) return -1;
+  if (nthreads < 1) nthreads = 1;
+  if ((uint32_t)nthreads > nq) nthreads = nq ? (int)nq : 1;
+  struct orc_tree t;
+  memset(&t, 0, sizeof(t));
+  t.nd = nodes;
+  t.split_value = split_value;
+  t.seg_end = seg_end;
+  t.ind = ind;
+  t.X = X;
+  t.sax = sax;
+  t.L = L;
+  t.nnodes = nnodes;
+  t.nseries = (uint64_t)nodes[OF_SIZE];
+  for (uint32_t i = 0; i < nnodes; ++i)
+    if (nodes[(int64_t)i * ORC_NODE_FIELDS + OF_LEAF] && nodes[(int64_t)i * ORC_NODE_FIELDS + OF_NSEG]) ++t.nleaves;
+  orc_breakpoints256(t.bp);
+  struct orc_mode_params p = {k, nprobes, epsilon, eapca_th, sax_th, approx_budget};
+  struct mode_job *jobs = calloc(nthreads, sizeof(struct mode_job));
+  pthread_t *th = malloc(sizeof(pthread_t) * nthreads);
+  uint32_t per = (nq + nthreads - 1) / nthreads;
+  for (int i = 0; i < nthreads; ++i) {
+    uint32_t a = i * per < nq ? i * per : nq, b = a + per < nq ? a + per : nq;
+    jobs[i] = (struct mode_job){&t, &p, a, b, Q, ids, d2, info};
+    pthread_create(&th[i], NULL, orc_mode_worker, &jobs[i]);
+  }
+  for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
   free(jobs);
   free(th);
   return 0;

@@ -49,6 +49,27 @@ int orc_knn_bruteforce(const float *X, uint64_t N, int L, const float *Q, uint32
 /* FP64 squared distance (error reporting) */
 double orc_ed2_f64(const float *q, const float *x, int L);
 
+/* ---- approximate query modes on an index tree -------------------------- */
+/* Node table in pre-order, ORC_NODE_FIELDS int64 per node:
+     {is_leaf, left, right, seg_off, nseg, split_from, split_to, split_idx,
+      file_pos, size}
+   seg_end/ind: every node's segments at seg_off (ind: 4 per segment, {max
+   mean, min mean, max sd, min sd}); split_value per node; X and sax (16
+   bytes per row) in leaf-file order, so a row's id is its position in
+   leaves_raw.idx.
+   nprobes > 0: exact_ng_knn_search (hercules_query_engine.c:2623-2799);
+   else the epsilon search of exact_de_knn_search_psq_lf_sims_adaptive
+   (:706-1083) with approximate_greedy_knn_search's leaf budget
+   approx_budget (--approx-stop-condition) and the EAPCA / SAX thresholds
+   (--eapca-threshold, --sax-threshold).  info[4 per query]: leaves scanned
+   before the candidate stage, candidate leaves, the candidates' branch (0
+   none, 1 sequential, 2 iSAX + refinement), candidate leaves scanned. */
+#define ORC_NODE_FIELDS 10
+int orc_knn_mode(const int64_t *nodes, const float *split_value, uint32_t nnodes, const int16_t *seg_end,
+                 const float *ind, const float *X, const uint8_t *sax, int L, const float *Q, uint32_t nq,
+                 uint32_t k, float epsilon, int approx_budget, uint32_t nprobes, float eapca_th, float sax_th,
+                 uint64_t *ids, float *d2, uint32_t *info, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,9 +26,20 @@
  *    survivor with a shared fetch-and-add on candidates_idx (the counter the
  *    caller resets at line 1002, the same pattern as lines 2071 and 2174).
  *
- * With these three, the paper's multithreaded SIMS path
- * (exact_de_knn_search_psq_lf_sims_adaptive) answers exactly, which the
- * tests check against PSCAN and against the serial path.
+ *  - queue_bounded_sorted_insert (hercules_query_engine.c:4034-4082): the
+ *    same final list (distinct distances, ascending, at most k), built by
+ *    shifting entries up from the end instead of writing the new distance
+ *    into the last slot and bubbling it down.  The refinement workers read
+ *    knn_results[k-1].distance without the lock (:1987, and the worker below);
+ *    the reference's order lets them read the new, smaller distance there
+ *    while it is still being sorted into place, prune a row with it and
+ *    lose answers, so two runs can differ.  Here the last slot goes from the
+ *    old k-th distance straight to the new one.
+ *
+ * With these, the paper's multithreaded SIMS path
+ * (exact_de_knn_search_psq_lf_sims_adaptive) answers exactly and the same on
+ * every run, which the tests check against PSCAN, the serial path and the
+ * oracle.
  */
 #define _GNU_SOURCE
 #include <dlfcn.h>
@@ -130,6 +141,31 @@ void approximate_greedy_knn_search(ts_type *query_ts, ts_type *query_ts_reordere
       free(item);
     }
   }
+}
+
+int queue_bounded_sorted_insert(struct query_result *q, struct query_result d, unsigned int *cur_size,
+                                unsigned int k)
+{
+  for (unsigned int i = 0; i < *cur_size; ++i)
+    if (q[i].distance == d.distance)
+      return 0;
+  unsigned int n = *cur_size;
+  if (n < k) {
+    ++*cur_size;
+  } else if (!(d.distance < q[k - 1].distance)) {
+    return 0; /* would land in the last slot: the list stays as it is */
+  } else {
+    n = k - 1; /* the last entry drops out */
+  }
+  /* n is the first free slot; move larger entries up one, from the end */
+  while (n > 0 && q[n - 1].distance > d.distance) {
+    q[n].node = q[n - 1].node;
+    q[n].distance = q[n - 1].distance;
+    --n;
+  }
+  q[n].node = d.node;
+  q[n].distance = d.distance;
+  return 0;
 }
 
 void *exact_de_knn_read_worker_lf_sims(void *arg)

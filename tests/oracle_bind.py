@@ -63,6 +63,12 @@ class Oracle:
         L.orc_knn_bruteforce.restype = C.c_int
         L.orc_knn_bruteforce.argtypes = [_f32p, C.c_uint64, C.c_int, _f32p, C.c_uint32, C.c_uint32,
                                          _u64p, _f32p, C.c_int]
+        _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+        _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+        L.orc_knn_mode.restype = C.c_int
+        L.orc_knn_mode.argtypes = [_i64p, _f32p, C.c_uint32, _i16p, _f32p, _f32p, _u8p, C.c_int, _f32p,
+                                   C.c_uint32, C.c_uint32, C.c_float, C.c_int, C.c_uint32, C.c_float, C.c_float,
+                                   _u64p, _f32p, _u32p, C.c_int]
         self.lib = L
         bp = np.zeros(255, np.float32)
         L.orc_breakpoints256(bp)
@@ -114,6 +120,24 @@ class Oracle:
         rc = self.lib.orc_knn_bruteforce(X, X.shape[0], X.shape[1], Q, nq, k, ids, d2, threads)
         assert rc == 0
         return ids, d2
+
+    def knn_mode(self, tables: dict, Xleaf, sax, Q, k, epsilon=0.0, approx_budget=80, nprobes=0,
+                 eapca_th=0.25, sax_th=0.9, threads=8):
+        """The reference's epsilon / ng search over an index tree (tables from
+        root_idx.node_tables); Xleaf and sax (n x 16 u8) in leaves_raw.idx
+        order.  Returns ids (rows of Xleaf), d2 and per query {leaves scanned
+        first, candidate leaves, candidate branch, candidate leaves scanned}."""
+        nq = Q.shape[0]
+        ids = np.empty((nq, k), np.uint64)
+        d2 = np.empty((nq, k), np.float32)
+        info = np.zeros((nq, 4), np.uint32)
+        nd = tables["nodes"]
+        rc = self.lib.orc_knn_mode(nd, tables["split_value"], nd.shape[0], tables["seg_end"], tables["ind"],
+                                   Xleaf, np.ascontiguousarray(sax, np.uint8).reshape(-1), Xleaf.shape[1], Q, nq, k,
+                                   float(epsilon), int(approx_budget), int(nprobes), float(eapca_th), float(sax_th),
+                                   ids, d2, info, threads)
+        assert rc == 0
+        return ids, d2, info
 
 
 def ref_available() -> bool:

@@ -11,73 +11,13 @@ from __future__ import annotations
 import hashlib
 import json
 import os
-import struct
 
 import numpy as np
 import pytest
 
+from root_idx import parse_root_idx
+
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-
-
-def parse_root_idx(path: str) -> dict:
-    buf = open(path, "rb").read()
-    off = 0
-
-    def take(fmt):
-        nonlocal off
-        v = struct.unpack_from("<" + fmt, buf, off)
-        off += struct.calcsize("<" + fmt)
-        return v if len(v) > 1 else v[0]
-
-    hdr = dict(zip(["leaf_count", "buffer_mb", "L", "init_segments", "paa_segments", "sax_bits", "max_leaf",
-                    "serial", "sims"], take("QdIIiiIii")))
-    leaves, nodes = [], []
-
-    def sketches():
-        S = take("h")
-        seg_end = list(take(f"{S}h")) if S > 1 else [take("h")]
-        ind = []
-        for _ in range(S):
-            n = take("i")
-            assert n == 4
-            ind.append(list(take("4f")))
-        return seg_end, ind
-
-    def node(depth):
-        is_leaf, level = take("BI")
-        me = {"is_leaf": is_leaf, "level": level}
-        nodes.append(me)
-        if not is_leaf:
-            me["left"] = node(depth + 1)
-            me["right"] = node(depth + 1)
-            me["size"] = take("I")
-            me["split_segment"] = take("h")
-            pol = buf[off:off + 48]
-            off_ = off + 48
-            me["split_from"], me["split_to"] = struct.unpack_from("<hh", pol, 0)
-            me["split_idx"], me["split_value"] = struct.unpack_from("<if", pol, 12)
-            globals_off(off_)
-            me["seg_end"], me["ind"] = sketches()
-            return me
-        me["size"] = take("I")
-        flen = take("i")
-        if flen == 0:
-            me["size"] = 0
-            return me
-        me["name"] = buf[off:off + flen].decode(errors="replace")
-        globals_off(off + flen)
-        me["seg_end"], me["ind"] = sketches()
-        me["file_pos"] = take("Q") if hdr["serial"] else None
-        leaves.append(me)
-        return me
-
-    def globals_off(v):
-        nonlocal off
-        off = v
-
-    root = node(0)
-    assert off == len(buf), "trailing bytes"
-    return {"header": hdr, "root": root, "leaves": leaves, "nodes": nodes}
 
 
 @pytest.mark.parametrize("name", ["tiny", "emb"])
