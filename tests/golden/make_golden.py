@@ -6,7 +6,7 @@ FP32 squared distances per rank (printed in hex by oracle/ref_interpose.c)
 and the oracle's brute-force ids and distances.  Datasets are not stored:
 they are regenerated from the seeds by the generator specification.
 
-    python tests/golden/make_golden.py [tiny|c1|emb|all]
+    python tests/golden/make_golden.py [tiny|c1|emb|all|modes]
 """
 from __future__ import annotations
 
@@ -106,7 +106,59 @@ def make(name: str) -> None:
         shutil.rmtree(work, ignore_errors=True)
 
 
+# approximate modes on the committed reference-built indexes:
+# (k, epsilon, approx budget, eapca threshold, sax threshold) and (k, nprobes)
+MODE_CONFIGS = {
+    "tiny": [dict(k=10, epsilon=5.0, budget=1, eapca=0.25, sax=0.9),
+             dict(k=10, epsilon=20.0, budget=1, eapca=0.5, sax=0.999),
+             dict(k=100, epsilon=0.5, budget=3, eapca=0.25, sax=0.9),
+             dict(k=1, epsilon=2.0, budget=1, eapca=0.9, sax=0.99),
+             dict(k=100, epsilon=3.0, budget=1, eapca=0.99, sax=0.99),
+             dict(k=10, epsilon=1.0, budget=80, eapca=0.25, sax=0.9),
+             dict(k=1, nprobes=1), dict(k=10, nprobes=3), dict(k=100, nprobes=7), dict(k=10, nprobes=25)],
+    "emb": [dict(k=10, epsilon=4.0, budget=1, eapca=0.25, sax=0.9),
+            dict(k=10, epsilon=10.0, budget=2, eapca=0.8, sax=0.99),
+            dict(k=1, epsilon=1.0, budget=1, eapca=0.25, sax=0.999),
+            dict(k=1, nprobes=2), dict(k=10, nprobes=5)],
+}
+
+
+def mode_flags(c: dict) -> list:
+    if c.get("nprobes"):
+        return ["--nprobes", c["nprobes"]]
+    return ["--epsilon", c["epsilon"], "--approx-stop-condition", c["budget"]]
+
+
+def make_modes(name: str) -> None:
+    """The reference CLI's answers in its epsilon and nprobes modes on the
+    committed reference-built index (same rows, same tree)."""
+    import golden_data as gd
+    o = Oracle()
+    m = gd.meta(name)
+    work = tempfile.mkdtemp(prefix=f"golden_modes_{name}_", dir=os.environ.get("TMPDIR", "/tmp"))
+    try:
+        gd.reference_index(o, name, os.path.join(work, "idx"))
+        Q = gd.queries(o, m)
+        rr.write_f32(os.path.join(work, "q.bin"), Q)
+        out = {}
+        for i, c in enumerate(MODE_CONFIGS[name]):
+            rd2, _, _, _ = rr.query(os.path.join(work, "idx"), os.path.join(work, "q.bin"), Q.shape[0], m["L"],
+                                    c["k"], threads=8, sims=True, eapca_threshold=c.get("eapca", 0.25),
+                                    sax_threshold=c.get("sax", 0.9), extra=mode_flags(c))
+            out[f"m{i}"] = rd2
+        np.savez_compressed(os.path.join(HERE, f"{name}_modes.npz"), **out)
+        with open(os.path.join(HERE, f"{name}_modes.json"), "w") as fh:
+            json.dump(MODE_CONFIGS[name], fh, indent=1)
+        print(name, "modes", len(out))
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 if __name__ == "__main__":
     which = sys.argv[1] if len(sys.argv) > 1 else "all"
-    for n in (CONFIGS if which == "all" else [which]):
-        make(n)
+    if which == "modes":
+        for n in MODE_CONFIGS:
+            make_modes(n)
+    else:
+        for n in (CONFIGS if which == "all" else [which]):
+            make(n)

@@ -29,3 +29,31 @@ def queries(gen, m: dict) -> np.ndarray:
                                   m["L"])
         Q = np.concatenate([Q, Qn])
     return np.ascontiguousarray(Q)
+
+
+def mode_configs(name: str) -> list:
+    with open(os.path.join(GOLDEN, f"{name}_modes.json")) as f:
+        return json.load(f)
+
+
+def mode_answers(name: str):
+    return np.load(os.path.join(GOLDEN, f"{name}_modes.npz"))
+
+
+def reference_index(gen, name: str, out_dir: str):
+    """Write the reference-built index of a fixture (its committed root.idx,
+    the generated rows in its leaf order, the oracle's iSAX words of those
+    rows) into out_dir; returns (rows in leaf order, iSAX words)."""
+    m = meta(name)
+    rowmap = np.load(os.path.join(GOLDEN, f"{name}_rowmap.npy")).astype(np.int64)
+    X = gen.series(m["kind"], m["seed"], 0, m["N"], m["L"])
+    raw = np.ascontiguousarray(X[rowmap])
+    sims = np.stack([gen.sax(gen.paa(r)) for r in raw])
+    os.makedirs(out_dir, exist_ok=True)
+    with open(os.path.join(GOLDEN, f"{name}_root.idx"), "rb") as f:
+        root = f.read()
+    with open(os.path.join(out_dir, "root.idx"), "wb") as f:
+        f.write(root)
+    raw.tofile(os.path.join(out_dir, "leaves_raw.idx"))
+    sims.tofile(os.path.join(out_dir, "leaves_sims.idx"))
+    return raw, sims

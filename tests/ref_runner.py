@@ -49,10 +49,12 @@ _DIST = re.compile(r"^Query_exact_distance\t(\S+)\t\S+\t(\d+)\t(\d+)$", re.M)
 
 
 def query(index_dir: str, queries_path: str, nq: int, L: int, k: int, threads: int = 8, sims: bool = True,
-          sax_threshold: float = 0.9, eapca_threshold: float = 0.25, timeout: float = 900):
+          sax_threshold: float = 0.9, eapca_threshold: float = 0.25, timeout: float = 900, extra=()):
     """Run the reference query path; returns (d2 (nq,k) float32 from the
     exact hex print, printed sqrt distances (nq,k), wall seconds, stdout).
-    sims=False with threads=1 runs the serial path exact_de_knn_search."""
+    sims=False with threads=1 runs the serial path exact_de_knn_search.
+    extra: further options of the table (--epsilon, --nprobes,
+    --approx-stop-condition)."""
     index_dir = os.path.abspath(index_dir) + "/"
     cmd = [HERCULES, "--mode", "1", "--index-path", index_dir, "--queries", os.path.abspath(queries_path),
            "--queries-size", str(nq), "--timeseries-size", str(L), "--k", str(k), "--serial", "1",
@@ -60,6 +62,7 @@ def query(index_dir: str, queries_path: str, nq: int, L: int, k: int, threads: i
            "--eapca-threshold", str(eapca_threshold), "--sax-threshold", str(sax_threshold)]
     if sims:
         cmd.append("--sims")
+    cmd += [str(a) for a in extra]
     import time
     t0 = time.perf_counter()
     r = subprocess.run(cmd, capture_output=True, encoding="utf-8", errors="replace", timeout=timeout)
