@@ -7,6 +7,7 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <map>
 #include <memory>
@@ -121,6 +122,12 @@ Index::~Index()
   dfree(dev.leaf_seg_off);
   dfree(dev.seg_end);
   dfree(dev.seg_ind);
+  dfree(dev.nd_link);
+  dfree(dev.nd_split);
+  dfree(dev.nd_seg);
+  dfree(dev.nd_seg_end);
+  dfree(dev.nd_ind);
+  dfree(dev.leaf_node);
   dfree(dev.Xb);
   dfree(dev.xst);
   dfree(dev.xn2);
@@ -191,6 +198,38 @@ hkg_status Index::upload_tree()
     HKG_HIP_TRY(hipMemcpy(dev.seg_end, se.data(), se.size() * 2, hipMemcpyHostToDevice));
     HKG_HIP_TRY(hipMemcpy(dev.seg_ind, ind.data(), ind.size() * sizeof(float4), hipMemcpyHostToDevice));
   }
+  // every node, for the approximate modes
+  const uint32_t nn = (uint32_t)t.nodes.size();
+  std::vector<int4> link(nn), split(nn);
+  std::vector<uint2> nseg(nn);
+  for (uint32_t i = 0; i < nn; ++i) {
+    const HostNode &nd = t.nodes[i];
+    link[i] = make_int4(nd.left, nd.right, nd.leaf_idx, nd.is_leaf ? 1 : 0);
+    int32_t vbits;
+    memcpy(&vbits, &nd.indicator_split_value, 4);
+    split[i] = make_int4(nd.split_from, nd.split_to, nd.indicator_split_idx, vbits);
+    nseg[i] = make_uint2(nd.seg_off, nd.nseg);
+  }
+  std::vector<float4> all_ind(t.seg_end.size());
+  for (size_t s = 0; s < t.seg_end.size(); ++s)
+    all_ind[s] = make_float4(t.ind[4 * s], t.ind[4 * s + 1], t.ind[4 * s + 2], t.ind[4 * s + 3]);
+  dev.nnodes = nn;
+  if ((rc = dalloc(&dev.nd_link, std::max<uint32_t>(nn, 1), &dev.device_bytes)) != HKG_OK) return rc;
+  if ((rc = dalloc(&dev.nd_split, std::max<uint32_t>(nn, 1), &dev.device_bytes)) != HKG_OK) return rc;
+  if ((rc = dalloc(&dev.nd_seg, std::max<uint32_t>(nn, 1), &dev.device_bytes)) != HKG_OK) return rc;
+  if ((rc = dalloc(&dev.nd_seg_end, std::max<size_t>(t.seg_end.size(), 1), &dev.device_bytes)) != HKG_OK) return rc;
+  if ((rc = dalloc(&dev.nd_ind, std::max<size_t>(t.seg_end.size(), 1), &dev.device_bytes)) != HKG_OK) return rc;
+  if ((rc = dalloc(&dev.leaf_node, std::max<uint32_t>(nl, 1), &dev.device_bytes)) != HKG_OK) return rc;
+  if (nn) {
+    HKG_HIP_TRY(hipMemcpy(dev.nd_link, link.data(), nn * sizeof(int4), hipMemcpyHostToDevice));
+    HKG_HIP_TRY(hipMemcpy(dev.nd_split, split.data(), nn * sizeof(int4), hipMemcpyHostToDevice));
+    HKG_HIP_TRY(hipMemcpy(dev.nd_seg, nseg.data(), nn * sizeof(uint2), hipMemcpyHostToDevice));
+  }
+  if (!t.seg_end.empty()) {
+    HKG_HIP_TRY(hipMemcpy(dev.nd_seg_end, t.seg_end.data(), t.seg_end.size() * 2, hipMemcpyHostToDevice));
+    HKG_HIP_TRY(hipMemcpy(dev.nd_ind, all_ind.data(), all_ind.size() * sizeof(float4), hipMemcpyHostToDevice));
+  }
+  if (nl) HKG_HIP_TRY(hipMemcpy(dev.leaf_node, t.leaves.data(), nl * 4, hipMemcpyHostToDevice));
   dev.has_tree = true;
   return HKG_OK;
 }
@@ -340,6 +379,9 @@ QueryBuffers::~QueryBuffers()
   dfree(counters);
   dfree(out_ids);
   dfree(out_d2);
+  dfree(nlb);
+  dfree(heap);
+  dfree(info);
 }
 
 hkg_status Index::get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuffers **out)
@@ -481,6 +523,50 @@ void Index::enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Pr
   if (pf) pf->mark(st, HKG_STAGE_TOPK);
 }
 
+// The approximate modes: exact node bounds, one block per query replaying
+// the reference's traversal, then (epsilon search, refined branch) the
+// candidate leaves through the work list and the iSAX filter + refinement.
+hkg_status Index::enqueue_mode_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf, bool seeded)
+{
+  const uint32_t L = dev.L, nleaf = dev.nleaf, nn = dev.nnodes;
+  hkg_status rc;
+  if (!b.nlb && (rc = dalloc(&b.nlb, (uint64_t)b.B * nn, &b.bytes)) != HKG_OK) return rc;
+  if (!b.info && (rc = dalloc(&b.info, (uint64_t)b.B * 4, &b.bytes)) != HKG_OK) return rc;
+  if (nn + 1 > kModeHeapLds && !b.heap && (rc = dalloc(&b.heap, (uint64_t)b.B * (nn + 1), &b.bytes)) != HKG_OK)
+    return rc;
+  ModeParams mp = call_settings().mp;
+  mp.nleaves = nleaf;
+  mp.nseries = dev.n;
+  uint32_t *nwork = b.misc + 0, *work_next = b.misc + 1, *overflow = b.misc + 2;
+  launch_reset(st, nq, b.bound, b.cand_cnt, seeded ? b.seed : nullptr);
+  launch_fill(st, b.misc, 8, 0);
+  launch_fill(st, reinterpret_cast<uint32_t *>(b.counters), 2 * kNumCounters, 0);
+  if (pf) pf->mark(st, -1);
+  launch_prep(st, b.Q, nq, L, b.paa, b.psum, b.qn2);
+  if (pf) pf->mark(st, HKG_STAGE_PREP);
+  launch_eapca(st, nq, nleaf, L, dev.leaf_seg_off, dev.seg_end, dev.seg_ind, b.psum, b.lb);
+  launch_node_lb(st, nq, nn, L, b.Q, dev.nd_seg, dev.nd_seg_end, dev.nd_ind, b.nlb);
+  if (pf) pf->mark(st, HKG_STAGE_EAPCA);
+  const uint32_t pairs = nq * nleaf;
+  launch_mode_keys(st, pairs, b.work_keys, b.work_vals);
+  launch_mode_search(st, nq, dev.X, dev.sax, L, b.Q, b.paa, b.kq, nn, dev.nd_link, dev.nd_split, b.nlb, nleaf,
+                     dev.leaf_pos, dev.leaf_size, dev.leaf_node, dev.ids, b.lb, b.heap, mp, b.cap, b.cand,
+                     b.cand_cnt, b.bound, b.work_keys, nwork, b.info, b.counters);
+  if (pf) pf->mark(st, HKG_STAGE_APPROX);
+  if (mp.nprobes == 0) {
+    uint32_t *keys2 = b.work_keys + b.work_cap, *vals2 = b.work_vals + b.work_cap;
+    HKG_HIP_TRY(launch_sort_work(st, b.sort_tmp, b.sort_bytes, b.work_keys, keys2, b.work_vals, vals2, pairs));
+    if (pf) pf->mark(st, HKG_STAGE_COLLECT);
+    launch_filter_refine(st, persistent_grid(), dev.X, dev.sax, L, b.Q, b.paa, b.kq, dev.leaf_pos, dev.leaf_size,
+                         dev.ids, nleaf, b.lb, vals2, nwork, work_next, b.cap, b.cand, b.cand_cnt, b.bound,
+                         b.counters);
+    if (pf) pf->mark(st, HKG_STAGE_REFINE);
+  }
+  launch_select(st, nq, b.kmax, b.kq, b.cap, b.cand, b.cand_cnt, idmap64, b.out_ids, b.out_d2, overflow);
+  if (pf) pf->mark(st, HKG_STAGE_TOPK);
+  return HKG_OK;
+}
+
 // one pass over X per group of kScanQ queries
 uint64_t Index::total_scan_bytes(uint32_t nq, uint32_t batch) const
 {
@@ -544,7 +630,10 @@ hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, 
     HKG_HIP_TRY(hipStreamSynchronize(st));
     use_graph = false;
   }
-  if (use_graph && !scan && !pf) {
+  if (call_settings().mode && !scan) {
+    rc = enqueue_mode_batch(*b, nq, st, pf, seed_host != nullptr);
+    if (rc != HKG_OK) return rc;
+  } else if (use_graph && !scan && !pf) {
     if (!b->graph_exec || b->graph_stream != st || b->graph_nq != nq) {
       if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
       b->graph_exec = nullptr;
@@ -612,13 +701,21 @@ hkg_status Index::fix_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax, hipS
     memcpy(&bits, &kth, 4);
     seed[i] = kth < FLT_MAX ? bits : kInfBits;
   }
+  // an approximate mode answers over the rows its traversal examines, so
+  // its last round is the same search with a buffer that cannot overflow
+  // (every flush holds at most k entries: one per query plus one per
+  // candidate leaf) rather than the full scan
+  const bool mode = call_settings().mode;
+  const uint32_t saved_cap = call_settings().cap_override;
   for (int round = 0; round < 4 && rc == HKG_OK; ++round) {
     ++*retries;
     QueryBuffers *sb = nullptr;
     Counters c;
     const bool last = round == 3;
-    rc = run_batch(last ? HKG_PATH_SCAN : HKG_PATH_BATCHED, tq, tk, m, kmax, false, st, &sb, &c, nullptr,
+    if (mode && last) call_settings().cap_override = kmax * (dev.nleaf + 2);
+    rc = run_batch(last && !mode ? HKG_PATH_SCAN : HKG_PATH_BATCHED, tq, tk, m, kmax, false, st, &sb, &c, nullptr,
                    last ? nullptr : seed.data());
+    call_settings().cap_override = saved_cap;
     if (rc != HKG_OK) break;
     BufLease lease{this, sb};
     std::vector<uint32_t> c2(m);
@@ -721,8 +818,19 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     return HKG_EINVAL;
   }
   if (nq == 0) return HKG_OK;
+  const bool mode = o.nprobes > 0 || o.epsilon != 0.0f;
+  if (!(o.epsilon >= 0.0f) || std::isinf(o.epsilon) || std::isnan(o.eapca_threshold) ||
+      std::isnan(o.sax_threshold)) {
+    set_error("epsilon must be finite and >= 0; thresholds must be numbers");
+    return HKG_EINVAL;
+  }
+  if (mode && !dev.has_tree) {
+    set_error("the epsilon and nprobes modes need an index tree (hkg_open_dir or a built index)");
+    return HKG_EINVAL;
+  }
   hkg_path path = o.path;
   if (path == HKG_PATH_AUTO) path = HKG_PATH_BATCHED;
+  if (mode) path = HKG_PATH_BATCHED;  // the modes run their own stage sequence
   if (path == HKG_PATH_BF16) {
     hkg_status rc = make_bf16();  // once per index
     if (rc != HKG_OK) return rc;
@@ -740,6 +848,8 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
   } restore{call_settings()};
   call_settings().cap_override = o.cand_cap;
   call_settings().bound_pass = o.bf16_bound_pass;
+  call_settings().mode = mode;
+  call_settings().mp = ModeParams{o.epsilon, o.eapca_threshold, o.sax_threshold, o.approx_leaves, o.nprobes, 0, 0};
   HKG_HIP_TRY(hipSetDevice(dev.device));
   if (!st) st = stream;
   hipEvent_t e0, e1;
@@ -835,6 +945,10 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     }
     stats->stage_bytes[HKG_STAGE_SCAN] = scan ? (uint64_t)launches * 0 + total_scan_bytes(nq, batch) : 0;
     stats->stage_bytes[HKG_STAGE_EAPCA] = scan ? 0 : (uint64_t)nq * dev.nleaf * 8 + total_segs() * 18ull * nq;
+    stats->mode_leaves_first = total.v[kCtrModeFirst];
+    stats->mode_candidates = total.v[kCtrModeCand];
+    stats->mode_candidates_scanned = total.v[kCtrModeScanned];
+    stats->mode_sequential = total.v[kCtrModeSeq];
     stats->bytes_algorithmic = 0;
     for (int i = 0; i < HKG_NUM_STAGES; ++i) stats->bytes_algorithmic += stats->stage_bytes[i];
   }
@@ -921,6 +1035,10 @@ void hkg_default_query_opts(hkg_query_opts *q)
   if (!q) return;
   memset(q, 0, sizeof(*q));
   q->path = HKG_PATH_AUTO;
+  q->delta = 1.0f;             // hercules.c:49
+  q->approx_leaves = 80;       // hercules.c:70
+  q->eapca_threshold = 0.25f;  // hercules.c:71
+  q->sax_threshold = 0.9f;     // hercules.c:72
 }
 
 const char *hkg_status_str(hkg_status s)

@@ -8,13 +8,14 @@
 #include <vector>
 
 #include "hkg_internal.h"
+#include "hkg_kernels.h"
 
 namespace hkg {
 
 const std::string &last_error();
 
 struct Counters {
-  unsigned long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned long long v[kNumCounters] = {};
   uint32_t overflow = 0;
   uint32_t nwork = 0;
   uint64_t bf16_rows = 0;    // rows the BF16 collecting pass read (summed over block groups)
@@ -99,6 +100,10 @@ struct QueryBuffers {
   uint32_t *bf_region_cnt = nullptr;  // per filter wave: pre-test passes in its region slice
   uint32_t *bf_prog = nullptr;  // per filter block: tiles done (sibling blocks keep pace)
   void free_bf16();
+  // approximate modes
+  float *nlb = nullptr;         // B x nnodes exact node bounds
+  uint64_t *heap = nullptr;     // B x (nnodes + 1) traversal heaps (when too big for LDS)
+  uint32_t *info = nullptr;     // B x 4 per-query traversal counts
   bool busy = false;            // held by a query call (guarded by Index::mu_bufs)
   // captured stage sequence of the batched path
   hipGraphExec_t graph_exec = nullptr;
@@ -130,6 +135,7 @@ struct Index {
   uint32_t cand_cap(hkg_path path, uint32_t nq, uint32_t kmax) const;
   void enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf, bool seeded);
   void enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf);
+  hkg_status enqueue_mode_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf, bool seeded);
   hkg_status run_batch(hkg_path path, const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax,
                        bool use_graph, hipStream_t st, QueryBuffers **used, Counters *ctr, Prof *pf,
                        const uint32_t *seed_host = nullptr);
@@ -152,6 +158,8 @@ struct CallSettings {
   uint32_t cap_override = 0;  // candidate (or survivor) entries per query requested by the caller
   uint32_t bound_pass = 0;    // hkg_query_opts.bf16_bound_pass
   uint32_t min_buffer_batch = 0;  // size buffer sets for at least this many queries (server workers)
+  bool mode = false;          // an approximate mode (epsilon > 0 or nprobes > 0) is on
+  ModeParams mp{};            // its parameters (nleaves / nseries filled per index)
 };
 CallSettings &call_settings();
 // BF16 path scratch of one query beyond the FP32 buffer set

@@ -83,6 +83,14 @@ struct DevIndex {
   uint32_t *leaf_seg_off = nullptr;  // nleaf + 1
   int16_t *seg_end = nullptr;        // total segments
   float4 *seg_ind = nullptr;         // total segments
+  // every node (pre-order), for the approximate modes' traversals
+  uint32_t nnodes = 0;
+  int4 *nd_link = nullptr;     // {left, right, leaf index or -1, is_leaf}
+  int4 *nd_split = nullptr;    // internal: {split_from, split_to, indicator (0 mean, 1 sd), value bits}
+  uint2 *nd_seg = nullptr;     // {first segment, segments} into nd_seg_end / nd_ind
+  int16_t *nd_seg_end = nullptr;
+  float4 *nd_ind = nullptr;
+  int32_t *leaf_node = nullptr;  // nleaf: node of each leaf
   // BF16 filter data (optional)
   uint16_t *Xb = nullptr;      // n x L bf16 bits (round to nearest even)
   float4 *xst = nullptr;       // n x {|x|^2 down, 2|x| up, 2(|x - x~| + g_L |x~|) up, 0}, x~ = bf16(x)

@@ -21,8 +21,21 @@ enum Counter : int {
   kCtrRefined = 2,     // series whose exact distance was computed
   kCtrBf16 = 3,        // series whose BF16 bound was evaluated
   kCtrApprox = 4,      // series of the best leaves scanned in the approximate stage
-  kNumCounters = 8
+  kCtrModeFirst = 5,   // approximate modes: leaves scanned before the candidate stage
+  kCtrModeCand = 6,    // approximate modes: candidate leaves
+  kCtrModeScanned = 7, // approximate modes: candidate leaves scanned or refined
+  kCtrModeSeq = 8,     // approximate modes: queries whose candidates went sequential
+  kNumCounters = 12
 };
+
+// parameters of the approximate modes (hkg_query_opts)
+struct ModeParams {
+  float epsilon, eapca_th, sax_th;
+  uint32_t budget, nprobes;
+  uint32_t nleaves;   // leaves with rows (the reference's leaves_counter)
+  uint64_t nseries;   // rows under the root
+};
+constexpr uint32_t kModeHeapLds = 2048;  // heap entries kept in LDS (else in scratch)
 
 hipError_t upload_breakpoints_dev();
 
@@ -55,6 +68,17 @@ void launch_filter_refine(hipStream_t st, uint32_t grid, const float *X, const u
                           const uint32_t *leaf_size, const uint32_t *ids, uint32_t nleaf, const float *lb,
                           const uint32_t *work, const uint32_t *nwork, uint32_t *work_next, uint32_t cap,
                           uint64_t *cand, uint32_t *cand_cnt, uint32_t *bound, unsigned long long *counters);
+// approximate modes
+void launch_node_lb(hipStream_t st, uint32_t nq, uint32_t nnodes, uint32_t L, const float *Q, const uint2 *nd_seg,
+                    const int16_t *seg_end, const float4 *ind, float *nlb);
+void launch_mode_keys(hipStream_t st, uint32_t pairs, uint32_t *keys, uint32_t *vals);
+void launch_mode_search(hipStream_t st, uint32_t nq, const float *X, const uint8_t *sax, uint32_t L, const float *Q,
+                        const float *paa, const uint32_t *kq, uint32_t nnodes, const int4 *nd_link,
+                        const int4 *nd_split, const float *nlb, uint32_t nleaf, const uint64_t *leaf_pos,
+                        const uint32_t *leaf_size, const int32_t *leaf_node, const uint32_t *ids, const float *lb,
+                        uint64_t *heap_g, const ModeParams &mp, uint32_t cap, uint64_t *cand, uint32_t *cand_cnt,
+                        uint32_t *bound, uint32_t *keys, uint32_t *nwork, uint32_t *info,
+                        unsigned long long *counters);
 void launch_scan(hipStream_t st, uint32_t grid_x, const float *X, uint64_t n, uint32_t L, const float *Q,
                  uint32_t nq, const uint32_t *kq, const uint32_t *ids, uint32_t cap, uint64_t *cand,
                  uint32_t *cand_cnt, uint32_t *bound);

@@ -1088,4 +1088,422 @@ void launch_select(hipStream_t st, uint32_t nq, uint32_t kmax, const uint32_t *k
                      out_d2, overflow);
 }
 
+
+// ---------------------------------------------------------------------------
+// approximate modes: the reference's best-first traversals
+// ---------------------------------------------------------------------------
+// The epsilon search of exact_de_knn_search_psq_lf_sims_adaptive
+// (hercules_query_engine.c:706-1083) and exact_ng_knn_search (:2623-2799)
+// decide which leaves they examine from the exact FP32 node bounds, a
+// binary heap's pop order and the k-th distance of the moment, so a block
+// per query replays them: lane 0 runs the heap, the block scans each leaf
+// the heap hands out.  Node bounds are the reference's own bits.
+
+__device__ __forceinline__ float tree8f(const float *v)
+{
+  return ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+}
+
+// calc_mean_stdev_SIMD (calc_utils.c:48-112): 8-chunks from the segment
+// start, the last zero-padded, FP32 sums; sd is NaN when the FP32 variance
+// is negative (every test it enters is then false, as in the reference)
+__device__ void ref_seg_stats(const float *x, int s, int e, float &mu, float &sd)
+{
+  const int n = e - s;
+  float sx = 0.0f, sq = 0.0f;
+  for (int b = s; b < e; b += 8) {
+    float c[8], c2[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      c[j] = b + j < e ? x[b + j] : 0.0f;
+      c2[j] = c[j] * c[j];
+    }
+    sx = sx + tree8f(c);
+    sq = sq + tree8f(c2);
+  }
+  mu = sx / (float)n;
+  sq = sq - (sx * sx) / (float)n;
+  sd = (float)sqrt((double)(sq / (float)n));
+}
+
+// calculate_node_min_distance (hercules_node.c:879-924) as GCC compiles it
+// for an FMA target: three contractions (both temp_dist += temp * temp and
+// sum += temp_dist * len), every other operation rounded on its own
+__device__ float lb_eapca_ref(const float *qs, const int16_t *se, const float4 *ind, int nseg)
+{
+  float sum = 0.0f;
+  int start = 0;
+  for (int i = 0; i < nseg; ++i) {
+    const int end = se[i];
+    float mu, sd;
+    ref_seg_stats(qs, start, end, mu, sd);
+    const float4 g = ind[i];  // {max mean, min mean, max sd, min sd}
+    float td = 0.0f;
+    if ((sd - g.z) * (sd - g.w) > 0.0f) {
+      float t = fminf(fabsf(sd - g.z), fabsf(sd - g.w));
+      td = fmaf(t, t, td);
+    }
+    if ((mu - g.x) * (mu - g.y) > 0.0f) {
+      float t = fminf(fabsf(mu - g.x), fabsf(mu - g.y));
+      td = fmaf(t, t, td);
+    }
+    sum = fmaf(td, (float)(end - start), sum);
+    start = end;
+  }
+  return sum;
+}
+
+// exact node bound of every (query, node): one lane per node
+__global__ void __launch_bounds__(kBlock) k_node_lb(uint32_t nnodes, uint32_t L, const float *__restrict__ Q,
+                                                    const uint2 *__restrict__ nd_seg,
+                                                    const int16_t *__restrict__ seg_end,
+                                                    const float4 *__restrict__ ind, float *__restrict__ nlb)
+{
+  extern __shared__ float qs[];
+  const uint32_t q = blockIdx.y;
+  load_query(Q, q, L, qs);
+  __syncthreads();
+  const uint32_t node = blockIdx.x * kBlock + threadIdx.x;
+  if (node >= nnodes) return;
+  const uint2 s = nd_seg[node];
+  nlb[(uint64_t)q * nnodes + node] = s.y ? lb_eapca_ref(qs, seg_end + s.x, ind + s.x, (int)s.y) : 0.0f;
+}
+
+// every (query, leaf) work item absent: key all ones, value its index
+__global__ void k_mode_keys(uint32_t pairs, uint32_t *__restrict__ keys, uint32_t *__restrict__ vals)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < pairs) {
+    keys[i] = 0xFFFFFFFFu;
+    vals[i] = i;
+  }
+}
+
+// pqueue.c:63-200 with cmp_pri = (next > curr): a min-heap in d[1, size)
+// ordered by the bound alone (entries: bound bits << 32 | node; bounds are
+// non-negative, so their bits order like their values)
+__device__ __forceinline__ uint32_t hpri(uint64_t e) { return (uint32_t)(e >> 32); }
+
+__device__ void heap_push(uint64_t *d, uint32_t &size, float pri, uint32_t node)
+{
+  uint32_t i = size++;
+  const uint64_t mv = ((uint64_t)__float_as_uint(pri) << 32) | node;
+  while (i > 1) {
+    const uint64_t p = d[i >> 1];
+    if (!(hpri(p) > hpri(mv))) break;
+    d[i] = p;
+    i >>= 1;
+  }
+  d[i] = mv;
+}
+
+__device__ bool heap_pop(uint64_t *d, uint32_t &size, uint64_t &out)
+{
+  if (size == 1) return false;
+  out = d[1];
+  const uint64_t mv = d[--size];
+  uint32_t i = 1;
+  for (;;) {
+    uint32_t c = i << 1;
+    if (c >= size) break;
+    if (c + 1 < size && hpri(d[c]) > hpri(d[c + 1])) ++c;
+    if (!(hpri(mv) > hpri(d[c]))) break;
+    d[i] = d[c];
+    i = c;
+  }
+  d[i] = mv;
+  return true;
+}
+
+// rows [beg, end) of a leaf into the block's running answer
+__device__ void scan_rows(const float *__restrict__ X, uint32_t L, const float *qs, uint64_t pos, uint32_t beg,
+                          uint32_t end, const uint32_t *__restrict__ ids, int k, TopK &tk, RefineShared &rs,
+                          uint64_t *rows, uint32_t *scan, uint32_t &nref)
+{
+  for (uint32_t t0 = beg; t0 < end; t0 += kBlock) {
+    const uint32_t nr = end - t0 < (uint32_t)kBlock ? end - t0 : (uint32_t)kBlock;
+    if (threadIdx.x < nr) rows[threadIdx.x] = pos + t0 + threadIdx.x;
+    __syncthreads();
+    refine_rows(X, L, qs, rows, (int)nr, ids, tk.kth_bits, tk, rs, nref);
+    __syncthreads();
+    topk_merge(tk, scan, k);
+  }
+}
+
+// one query per block.  ng (nprobes > 0): the leaf the split policies route
+// the query to, then best-first probes (exact_ng_knn_search).  epsilon:
+// approximate_greedy_knn_search's best-first leaves (up to the budget), the
+// candidate leaves whose bound passes kth/(1 + epsilon), and the branch the
+// reference takes for them (:893-978): sequential in file order against the
+// k-th distance its loop variable holds (exact_de_knn_search_noapprox,
+// :2450-2507), or every candidate refined exactly -- then handed to the
+// work list of k_filter_refine.  info per query: {leaves scanned first,
+// candidate leaves, branch (0 none, 1 sequential, 2 refined), candidate
+// leaves scanned}.
+__global__ void __launch_bounds__(kBlock) k_mode_search(
+    const float *__restrict__ X, const uint8_t *__restrict__ sax, uint32_t L, const float *__restrict__ Q,
+    const float *__restrict__ paa, const uint32_t *__restrict__ kq, uint32_t nnodes,
+    const int4 *__restrict__ nd_link, const int4 *__restrict__ nd_split, const float *__restrict__ nlb,
+    uint32_t nleaf, const uint64_t *__restrict__ leaf_pos, const uint32_t *__restrict__ leaf_size,
+    const int32_t *__restrict__ leaf_node, const uint32_t *__restrict__ ids, const float *__restrict__ lb,
+    uint64_t *__restrict__ heap_g, int heap_in_lds, ModeParams mp, uint32_t cap, uint64_t *__restrict__ cand,
+    uint32_t *__restrict__ cand_cnt, uint32_t *__restrict__ bound, uint32_t *__restrict__ keys,
+    uint32_t *__restrict__ nwork, uint32_t *__restrict__ info, unsigned long long *__restrict__ counters)
+{
+  extern __shared__ __align__(16) unsigned char mode_lds[];
+  float *qs = reinterpret_cast<float *>(mode_lds);
+  const uint32_t q = blockIdx.x;
+  uint64_t *hp = heap_in_lds ? reinterpret_cast<uint64_t *>(mode_lds + ((L * 4 + 15) & ~15u))
+                             : heap_g + (uint64_t)q * (nnodes + 1);
+  __shared__ TopK tk;
+  __shared__ RefineShared rs;
+  __shared__ uint64_t rows[kBlock];
+  __shared__ uint32_t scan[kBlock];
+  __shared__ float bp[256];
+  __shared__ float spaa[kPaaSegments];
+  __shared__ int32_t s_node;
+  __shared__ int s_go;
+  __shared__ uint32_t s_list[kBlock];
+  __shared__ int s_nlist;
+  __shared__ unsigned long long s_lab;
+  const int k = (int)kq[q];
+  const bool ng = mp.nprobes > 0;
+  const float *qnlb = nlb + (uint64_t)q * nnodes;
+  uint32_t *qkeys = keys + (uint64_t)q * nleaf;
+  load_query(Q, q, L, qs);
+  for (int i = threadIdx.x; i < 256; i += kBlock) bp[i] = c_bp[i];
+  if (threadIdx.x < kPaaSegments) spaa[threadIdx.x] = paa[q * kPaaSegments + threadIdx.x];
+  topk_reset(tk);
+  __syncthreads();
+  uint32_t nref = 0;
+  unsigned long long first_rows = 0;
+  // lane 0's state
+  uint32_t hsize = 1, nvis = 0, ncand = 0, nempty = 0;
+  float best = FLT_MAX;
+  int32_t best_leaf = -1;
+  if (ng) {
+    if (threadIdx.x == 0) {
+      int32_t i = 0;
+      for (int4 ln = nd_link[0]; !ln.w; ln = nd_link[i]) {
+        const int4 sp = nd_split[i];  // {from, to, indicator (0 mean, 1 sd), value bits}
+        float mu, sd;
+        ref_seg_stats(qs, sp.x, sp.y, mu, sd);
+        i = ((sp.z & 1) ? sd : mu) < __int_as_float(sp.w) ? ln.x : ln.y;
+      }
+      s_node = i;
+    }
+    __syncthreads();
+    const int32_t lf = nd_link[s_node].z;
+    if (lf >= 0) {
+      scan_rows(X, L, qs, leaf_pos[lf], 0, leaf_size[lf], ids, k, tk, rs, rows, scan, nref);
+      first_rows += leaf_size[lf];
+    }
+    if (threadIdx.x == 0) {
+      nvis = 1;
+      if (tk.cnt > 0) {
+        best = __uint_as_float((uint32_t)(tk.keys[0] >> 32));
+        best_leaf = s_node;
+      }
+      heap_push(hp, hsize, qnlb[0], 0);
+    }
+  } else if (threadIdx.x == 0) {
+    heap_push(hp, hsize, qnlb[0], 0);
+  }
+  // best-first leaves: the greedy approximate phase or the ng probes
+  const uint32_t budget = mp.budget ? mp.budget : 1u;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int32_t next = -1;
+      if (ng || nvis < budget) {
+        uint64_t it;
+        while (heap_pop(hp, hsize, it)) {
+          if (ng && nvis >= mp.nprobes) break;  // popped, then the probe count ends the loop
+          const float pri = __uint_as_float(hpri(it));
+          const uint32_t node = (uint32_t)it;
+          const float kth = __uint_as_float(tk.kth_bits);
+          if (pri > (ng ? best : kth)) break;   // the popped item is dropped
+          const int4 ln = nd_link[node];
+          if (ln.w) {
+            next = (int32_t)node;
+            break;
+          }
+          for (int c = 0; c < 2; ++c) {
+            const int32_t ch = c ? ln.y : ln.x;
+            const float b = qnlb[ch];
+            if (b < kth && ch != best_leaf) heap_push(hp, hsize, b, (uint32_t)ch);
+          }
+        }
+      }
+      s_node = next;
+    }
+    __syncthreads();
+    const int32_t node = s_node;
+    if (node < 0) break;
+    const int32_t lf = nd_link[node].z;
+    if (lf >= 0) {
+      scan_rows(X, L, qs, leaf_pos[lf], 0, leaf_size[lf], ids, k, tk, rs, rows, scan, nref);
+      first_rows += leaf_size[lf];
+    }
+    if (threadIdx.x == 0) ++nvis;
+    __syncthreads();
+  }
+  uint32_t branch = 0, nscanned = 0;
+  if (!ng) {
+    __shared__ float s_kth0;
+    __shared__ int s_branch;
+    if (threadIdx.x == 0) {
+      // candidate leaves: the same queue against kth0 / (1 + epsilon)
+      const float kth0 = __uint_as_float(tk.kth_bits);
+      const float thr = kth0 / (1.0f + mp.epsilon);
+      uint64_t it;
+      while (heap_pop(hp, hsize, it)) {
+        const float pri = __uint_as_float(hpri(it));
+        if (pri > thr) break;
+        const uint32_t node = (uint32_t)it;
+        const int4 ln = nd_link[node];
+        if (ln.w) {
+          ++ncand;
+          if (ln.z >= 0) qkeys[ln.z] = __float_as_uint(lb[(uint64_t)q * nleaf + ln.z]);
+          else ++nempty;
+          continue;
+        }
+        for (int c = 0; c < 2; ++c) {
+          const int32_t ch = c ? ln.y : ln.x;
+          const float b = qnlb[ch];
+          if (b < thr) heap_push(hp, hsize, b, (uint32_t)ch);
+        }
+      }
+      s_kth0 = kth0;
+      int br = 0;
+      if (ncand) {
+        const float flp = (float)(1.0 - (double)ncand / (double)(float)mp.nleaves);
+        br = flp < mp.eapca_th ? 1 : 3;  // 3: decided by the iSAX count
+      }
+      s_branch = br;
+      s_lab = 0;
+    }
+    __syncthreads();
+    const float kth0 = s_kth0;
+    int br = s_branch;
+    if (br == 3) {
+      // rows of the candidate leaves whose iSAX bound is within kth0 (:2171-2205)
+      const float ratio = (float)(L / kPaaSegments);
+      unsigned long long mine = 0;
+      for (uint32_t l0 = 0; l0 < nleaf; l0 += kBlock) {
+        if (threadIdx.x == 0) s_nlist = 0;
+        __syncthreads();
+        const uint32_t l = l0 + threadIdx.x;
+        if (l < nleaf && qkeys[l] != 0xFFFFFFFFu && qnlb[leaf_node[l]] <= kth0) s_list[atomicAdd(&s_nlist, 1)] = l;
+        __syncthreads();
+        const int nl = s_nlist;
+        for (int j = 0; j < nl; ++j) {
+          const uint32_t lf = s_list[j];
+          const uint64_t pos = leaf_pos[lf];
+          const uint32_t size = leaf_size[lf];
+          for (uint32_t r = threadIdx.x; r < size; r += kBlock) {
+            const uint4 w = *reinterpret_cast<const uint4 *>(sax + (pos + r) * kPaaSegments);
+            mine += lb_sax_word(spaa, w, bp, ratio) <= kth0 ? 1ull : 0ull;
+          }
+        }
+        __syncthreads();
+      }
+      atomicAdd(&s_lab, mine);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        const float slp = (float)(1.0 - (double)s_lab / (double)(float)mp.nseries);
+        s_branch = slp < mp.sax_th ? 1 : 2;
+      }
+      __syncthreads();
+      br = s_branch;
+    }
+    branch = (uint32_t)br;
+    if (br == 1) {
+      // candidates in file (leaf) order, each against the lagging k-th distance
+      __shared__ float s_kthb;
+      if (threadIdx.x == 0) s_kthb = __uint_as_float(tk.kth_bits);
+      for (uint32_t l0 = 0; l0 < nleaf; l0 += kBlock) {
+        const uint32_t l = l0 + threadIdx.x;
+        const bool is_c = l < nleaf && qkeys[l] != 0xFFFFFFFFu;
+        __syncthreads();
+        // ordered compaction of this chunk's candidates
+        scan[threadIdx.x] = is_c ? 1u : 0u;
+        __syncthreads();
+        for (int off = 1; off < kBlock; off <<= 1) {
+          uint32_t v = threadIdx.x >= (uint32_t)off ? scan[threadIdx.x - off] : 0u;
+          __syncthreads();
+          scan[threadIdx.x] += v;
+          __syncthreads();
+        }
+        if (is_c) {
+          s_list[scan[threadIdx.x] - 1] = l;
+          qkeys[l] = 0xFFFFFFFFu;  // handled here, not by the work list
+        }
+        const int nl = (int)scan[kBlock - 1];
+        __syncthreads();
+        for (int j = 0; j < nl; ++j) {
+          const uint32_t lf = s_list[j];
+          if (threadIdx.x == 0) s_go = qnlb[leaf_node[lf]] <= s_kthb / (1.0f + mp.epsilon);
+          __syncthreads();
+          const uint32_t size = leaf_size[lf];
+          if (s_go && size > 0) {
+            const uint64_t pos = leaf_pos[lf];
+            scan_rows(X, L, qs, pos, 0, size - 1, ids, k, tk, rs, rows, scan, nref);
+            if (threadIdx.x == 0) s_kthb = __uint_as_float(tk.kth_bits);  // read before the last row
+            scan_rows(X, L, qs, pos, size - 1, size, ids, k, tk, rs, rows, scan, nref);
+            ++nscanned;
+          }
+          __syncthreads();
+        }
+      }
+    } else if (br == 2) {
+      nscanned = ncand - nempty;
+      if (threadIdx.x == 0 && nscanned) atomicAdd(nwork, nscanned);
+    }
+  }
+  flush_topk(tk, k, q, cap, cand, cand_cnt, bound);
+  if (threadIdx.x == 0) {
+    info[4 * q + 0] = nvis;
+    info[4 * q + 1] = ncand;
+    info[4 * q + 2] = branch;
+    info[4 * q + 3] = nscanned;
+    atomicAdd(&counters[kCtrApprox], first_rows);
+    atomicAdd(&counters[kCtrModeFirst], (unsigned long long)nvis);
+    atomicAdd(&counters[kCtrModeCand], (unsigned long long)ncand);
+    atomicAdd(&counters[kCtrModeScanned], (unsigned long long)nscanned);
+    if (branch == 1) atomicAdd(&counters[kCtrModeSeq], 1ull);
+  }
+  if ((threadIdx.x & 63) == 0 && nref) atomicAdd(&counters[kCtrRefined], (unsigned long long)nref);
+}
+
+void launch_node_lb(hipStream_t st, uint32_t nq, uint32_t nnodes, uint32_t L, const float *Q, const uint2 *nd_seg,
+                    const int16_t *seg_end, const float4 *ind, float *nlb)
+{
+  hipLaunchKernelGGL(k_node_lb, dim3((nnodes + kBlock - 1) / kBlock, nq), dim3(kBlock), L * sizeof(float), st,
+                     nnodes, L, Q, nd_seg, seg_end, ind, nlb);
+}
+
+void launch_mode_keys(hipStream_t st, uint32_t pairs, uint32_t *keys, uint32_t *vals)
+{
+  hipLaunchKernelGGL(k_mode_keys, dim3((pairs + 255) / 256), dim3(256), 0, st, pairs, keys, vals);
+}
+
+size_t mode_search_lds(uint32_t L, uint32_t nnodes, bool heap_in_lds)
+{
+  return ((L * 4 + 15) & ~15u) + (heap_in_lds ? (size_t)(nnodes + 1) * 8 : 0);
+}
+
+void launch_mode_search(hipStream_t st, uint32_t nq, const float *X, const uint8_t *sax, uint32_t L, const float *Q,
+                        const float *paa, const uint32_t *kq, uint32_t nnodes, const int4 *nd_link,
+                        const int4 *nd_split, const float *nlb, uint32_t nleaf, const uint64_t *leaf_pos,
+                        const uint32_t *leaf_size, const int32_t *leaf_node, const uint32_t *ids, const float *lb,
+                        uint64_t *heap_g, const ModeParams &mp, uint32_t cap, uint64_t *cand, uint32_t *cand_cnt,
+                        uint32_t *bound, uint32_t *keys, uint32_t *nwork, uint32_t *info,
+                        unsigned long long *counters)
+{
+  const bool in_lds = (nnodes + 1) <= kModeHeapLds;
+  hipLaunchKernelGGL(k_mode_search, dim3(nq), dim3(kBlock), mode_search_lds(L, nnodes, in_lds), st, X, sax, L, Q,
+                     paa, kq, nnodes, nd_link, nd_split, nlb, nleaf, leaf_pos, leaf_size, leaf_node, ids, lb, heap_g,
+                     in_lds ? 1 : 0, mp, cap, cand, cand_cnt, bound, keys, nwork, info, counters);
+}
+
 }  // namespace hkg

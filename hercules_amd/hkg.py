@@ -38,7 +38,26 @@ class Options(C.Structure):
 
 class QueryOpts(C.Structure):
     _fields_ = [("path", C.c_int), ("batch", C.c_uint32), ("profile", C.c_uint32), ("cand_cap", C.c_uint32),
-                ("bf16_bound_pass", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("bf16_bound_pass", C.c_uint32), ("epsilon", C.c_float), ("delta", C.c_float),
+                ("nprobes", C.c_uint32), ("approx_leaves", C.c_uint32), ("eapca_threshold", C.c_float),
+                ("sax_threshold", C.c_float), ("reserved", C.c_uint32 * 2)]
+
+
+def query_opts(path: str = "auto", batch: int = 0, profile: bool = False, cand_cap: int = 0,
+               bf16_bound_pass: int = 0, **modes) -> QueryOpts:
+    """hkg_default_query_opts, then the given settings.  ``modes``:
+    epsilon, delta, nprobes, approx_leaves, eapca_threshold, sax_threshold
+    (the reference's --epsilon, --delta, --nprobes, --approx-stop-condition,
+    --eapca-threshold, --sax-threshold)."""
+    qo = QueryOpts()
+    lib().hkg_default_query_opts(C.byref(qo))
+    qo.path, qo.batch, qo.profile, qo.cand_cap, qo.bf16_bound_pass = (PATHS[path], batch, int(profile), cand_cap,
+                                                                       bf16_bound_pass)
+    for key, v in modes.items():
+        if key not in ("epsilon", "delta", "nprobes", "approx_leaves", "eapca_threshold", "sax_threshold"):
+            raise TypeError(f"unknown query option {key!r}")
+        setattr(qo, key, v)
+    return qo
 
 
 STAGES = ["prep", "eapca", "select_leaf", "approx", "collect", "refine", "topk", "scan", "bound"]
@@ -50,7 +69,9 @@ class QueryStats(C.Structure):
                 ("series_refined", C.c_uint64), ("bytes_algorithmic", C.c_uint64), ("device_ms", C.c_double),
                 ("overflow_retries", C.c_uint32), ("launches", C.c_uint32), ("bf16_pairs", C.c_uint64),
                 ("stage_ms", C.c_double * len(STAGES)),
-                ("stage_bytes", C.c_uint64 * len(STAGES)), ("stage_launches", C.c_uint64 * len(STAGES))]
+                ("stage_bytes", C.c_uint64 * len(STAGES)), ("stage_launches", C.c_uint64 * len(STAGES)),
+                ("mode_leaves_first", C.c_uint64), ("mode_candidates", C.c_uint64),
+                ("mode_candidates_scanned", C.c_uint64), ("mode_sequential", C.c_uint64)]
 
     def as_dict(self) -> dict:
         d = {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("stage_")}
@@ -275,8 +296,10 @@ class HerculesIndex:
 
     # ---- queries -------------------------------------------------------------
     def knn(self, Q: np.ndarray, k, path: str = "auto", batch: int = 0, profile: bool = False,
-            cand_cap: int = 0, bf16_bound_pass: int = 0) -> KnnResult:
-        """Exact k-NN of each row of Q.  ``k`` is an int or a per-query array."""
+            cand_cap: int = 0, bf16_bound_pass: int = 0, **modes) -> KnnResult:
+        """k-NN of each row of Q.  ``k`` is an int or a per-query array.
+        Exact unless ``modes`` asks for the reference's epsilon or nprobes
+        search (see query_opts)."""
         Q = np.ascontiguousarray(Q, np.float32)
         if Q.ndim == 1:
             Q = Q[None, :]
@@ -289,7 +312,7 @@ class HerculesIndex:
             kmax = int(kv.max()) if nq else 1
         ids = np.empty((nq, kmax), np.uint64)
         d2 = np.empty((nq, kmax), np.float32)
-        qo = QueryOpts(PATHS[path], batch, int(profile), cand_cap, bf16_bound_pass)
+        qo = query_opts(path, batch, profile, cand_cap, bf16_bound_pass, **modes)
         st = QueryStats()
         _check(lib().hkg_knn(self._h, _ptr(Q), nq, _ptr(kv) if kv is not None else None, kmax, C.byref(qo),
                              _ptr(ids), _ptr(d2), C.byref(st)))
@@ -297,8 +320,8 @@ class HerculesIndex:
 
     def knn_device(self, dQ: DeviceArray, nq: int, dk: DeviceArray, kmax: int, dids: DeviceArray,
                    dd2: DeviceArray, path: str = "auto", batch: int = 0, profile: bool = False,
-                   bf16_bound_pass: int = 0) -> dict:
-        qo = QueryOpts(PATHS[path], batch, int(profile), 0, bf16_bound_pass)
+                   bf16_bound_pass: int = 0, **modes) -> dict:
+        qo = query_opts(path, batch, profile, 0, bf16_bound_pass, **modes)
         st = QueryStats()
         _check(lib().hkg_knn_device(self._h, dQ.ptr, nq, dk.ptr, kmax, C.byref(qo), dids.ptr, dd2.ptr, None,
                                     C.byref(st)))

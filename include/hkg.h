@@ -84,7 +84,23 @@ typedef struct hkg_query_opts {
   uint32_t bf16_bound_pass; /* BF16 path: 0 = only when re-running queries
                                whose survivor list overflowed, 1 = always,
                                2 = never (seeded re-runs only)              */
-  uint32_t reserved[3];
+  /* Approximate modes of the reference's query answering (hercules.c:48-51,
+     :70-72).  Exact search when epsilon == 0 and nprobes == 0; otherwise the
+     answers are the reference's for the same index and settings, and
+     `path` is ignored (the modes run their own stage sequence).  Squared
+     distances throughout, as in the reference. */
+  float epsilon;           /* > 0: epsilon search: leaves are pruned against
+                              kth / (1 + epsilon)                            */
+  float delta;             /* accepted for the reference's interface; it has
+                              no effect there either (r_delta = FLT_MAX,
+                              hercules.c:479)                                */
+  uint32_t nprobes;        /* > 0: no-guarantee search that scans nprobes
+                              leaves (exact_ng_knn_search); epsilon ignored  */
+  uint32_t approx_leaves;  /* epsilon search: leaf budget of its approximate
+                              phase (--approx-stop-condition; default 80)    */
+  float eapca_threshold;   /* epsilon search: candidate-branch thresholds    */
+  float sax_threshold;     /* (--eapca-threshold 0.25, --sax-threshold 0.9)  */
+  uint32_t reserved[2];
 } hkg_query_opts;
 
 /* stages reported in hkg_query_stats.stage_ms / stage_bytes */
@@ -117,6 +133,12 @@ typedef struct hkg_query_stats {
   double stage_ms[HKG_NUM_STAGES];         /* with profile=1: summed stage times */
   uint64_t stage_bytes[HKG_NUM_STAGES];    /* algorithmic HBM bytes per stage    */
   uint64_t stage_launches[HKG_NUM_STAGES]; /* kernel launches of each stage      */
+  uint64_t mode_leaves_first;       /* modes: leaves scanned before the candidates */
+  uint64_t mode_candidates;         /* epsilon search: candidate leaves            */
+  uint64_t mode_candidates_scanned; /* epsilon search: candidate leaves examined   */
+  uint64_t mode_sequential;         /* epsilon search: queries whose candidates
+                                       were scanned in file order (the reference's
+                                       sequential branch)                          */
 } hkg_query_stats;
 
 typedef struct hkg_index_info {
