@@ -54,6 +54,10 @@ def parse() -> argparse.Namespace:
     p.add_argument("--serve-qps", type=float, default=0.0,
                    help="offered load of the serving leg (0: half the measured throughput)")
     p.add_argument("--serve-requests", type=int, default=4096, help="requests of the serving leg (0: skip)")
+    p.add_argument("--mode-epsilon", type=float, default=1.0,
+                   help="epsilon of the approximate-mode leg (0: skip that leg)")
+    p.add_argument("--mode-nprobes", type=int, default=10, help="nprobes of the ng-mode leg (0: skip that leg)")
+    p.add_argument("--mode-queries", type=int, default=256, help="queries of each mode leg (from the first step)")
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
                    help="one device-to-device copy of this size first (reference dispatch for counter runs)")
     return p.parse_args()
@@ -160,6 +164,12 @@ def serving_leg(hkg, ix, args, Qh: np.ndarray, kq: np.ndarray, qps: float) -> di
             "answers_identical_to_direct_queries": same, "sampled": int(len(sample))}
 
 
+def _views(hkg, dQ, dk, step: int, B: int, L: int):
+    """Non-owning views of one step's queries and k values."""
+    return (hkg.DeviceArray.view(dQ.ptr + step * B * L * 4, B * L * 4),
+            hkg.DeviceArray.view(dk.ptr + step * B * 4, B * 4))
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -200,16 +210,9 @@ def main() -> int:
     dd2 = hkg.DeviceArray(B * kmax * 4)
     # per-step views into the resident query array
     def run(step: int, profile: bool) -> dict:
-        qv = hkg.DeviceArray.__new__(hkg.DeviceArray)
-        qv.ptr, qv.nbytes = dQ.ptr + step * B * L * 4, B * L * 4
-        kv = hkg.DeviceArray.__new__(hkg.DeviceArray)
-        kv.ptr, kv.nbytes = dk.ptr + step * B * 4, B * 4
-        try:
-            return ix.knn_device(qv, B, kv, kmax, dids, dd2, path=args.path, batch=B, profile=profile,
-                                 bf16_bound_pass=args.bf16_bound_pass)
-        finally:
-            qv.ptr = None
-            kv.ptr = None
+        qv, kv = _views(hkg, dQ, dk, step, B, L)
+        return ix.knn_device(qv, B, kv, kmax, dids, dd2, path=args.path, batch=B, profile=profile,
+                             bf16_bound_pass=args.bf16_bound_pass)
 
     for s in range(warm):
         st = run(s, True)
@@ -276,6 +279,39 @@ def main() -> int:
         par["rows_found_exactly"] = f"{found}/{len(rows)}"
         log(f"dataset rows found at distance 0: {found}/{len(rows)}")
 
+    # the reference's epsilon and no-guarantee modes on one batch of the same
+    # queries: throughput, and how their answers compare with the exact ones
+    modes = {}
+    legs = ([("epsilon", {"epsilon": args.mode_epsilon})] if args.mode_epsilon > 0 else []) + \
+           ([("ng", {"nprobes": args.mode_nprobes})] if args.mode_nprobes > 0 else [])
+    M = min(args.mode_queries, B)
+    if legs and M:
+        exact = ix.knn(Qh[warm * B:warm * B + M], kq[warm * B:warm * B + M], path=args.path)
+        kv = kq[warm * B:warm * B + M]
+        w = exact.d2.shape[1]
+        ok = np.arange(w)[None, :] < kv[:, None]
+        for name, kw in legs:
+            qv, kvv = _views(hkg, dQ, dk, warm, B, L)
+            hkg.lib().hkg_device_sync()
+            tm = time.perf_counter()
+            st = ix.knn_device(qv, M, kvv, kmax, dids, dd2, batch=M, profile=True, **kw)
+            hkg.lib().hkg_device_sync()
+            ms = (time.perf_counter() - tm) * 1e3
+            d2m = dd2.download(np.empty((M, kmax), np.float32))[:, :w]
+            ex = exact.d2.astype(np.float64)
+            ratio = np.where(ok, d2m.astype(np.float64) / np.maximum(ex, 1e-30), 1.0)
+            leg = {"settings": kw, "queries": M, "queries_per_s": M / (ms * 1e-3), "ms_per_batch": ms,
+                   "queries_with_exact_answer": float(np.mean(np.all(~ok | (d2m == exact.d2), axis=1))),
+                   "max_d2_ratio_to_exact": float(ratio.max()),
+                   "leaves_scanned_first_per_query": st["mode_leaves_first"] / M,
+                   "candidate_leaves_per_query": st["mode_candidates"] / M,
+                   "sequential_branch_queries": st["mode_sequential"],
+                   "stages_ms": {kk: round(v, 3) for kk, v in st["stage_ms"].items() if v > 0}}
+            if name == "epsilon":
+                leg["within_1_plus_epsilon"] = bool(ratio.max() <= (1.0 + kw["epsilon"]) * (1 + 1e-6))
+            modes[name] = leg
+            log(f"mode {name}: {leg}")
+
     serving = None
     if args.serve_requests:
         sq = args.serve_qps if args.serve_qps > 0 else 0.5 * qps
@@ -317,6 +353,7 @@ def main() -> int:
         "series_refined_per_query": refined / (steps * B),
         "series_lb_checked_per_query": checked / (steps * B),
         "parity": par,
+        "modes": modes,
         "serving": serving,
         "cpu_baseline": cpu,
     }

@@ -381,6 +381,7 @@ QueryBuffers::~QueryBuffers()
   dfree(out_d2);
   dfree(nlb);
   dfree(heap);
+  dfree(bfs);
   dfree(info);
 }
 
@@ -532,8 +533,9 @@ hkg_status Index::enqueue_mode_batch(QueryBuffers &b, uint32_t nq, hipStream_t s
   hkg_status rc;
   if (!b.nlb && (rc = dalloc(&b.nlb, (uint64_t)b.B * nn, &b.bytes)) != HKG_OK) return rc;
   if (!b.info && (rc = dalloc(&b.info, (uint64_t)b.B * 4, &b.bytes)) != HKG_OK) return rc;
-  if (nn + 1 > kModeHeapLds && !b.heap && (rc = dalloc(&b.heap, (uint64_t)b.B * (nn + 1), &b.bytes)) != HKG_OK)
+  if (!mode_heap_in_lds(L, nn) && !b.heap && (rc = dalloc(&b.heap, (uint64_t)b.B * (nn + 1), &b.bytes)) != HKG_OK)
     return rc;
+  if (!b.bfs && (rc = dalloc(&b.bfs, (uint64_t)b.B * 2 * (nn + 1), &b.bytes)) != HKG_OK) return rc;
   ModeParams mp = call_settings().mp;
   mp.nleaves = nleaf;
   mp.nseries = dev.n;
@@ -550,7 +552,7 @@ hkg_status Index::enqueue_mode_batch(QueryBuffers &b, uint32_t nq, hipStream_t s
   const uint32_t pairs = nq * nleaf;
   launch_mode_keys(st, pairs, b.work_keys, b.work_vals);
   launch_mode_search(st, nq, dev.X, dev.sax, L, b.Q, b.paa, b.kq, nn, dev.nd_link, dev.nd_split, b.nlb, nleaf,
-                     dev.leaf_pos, dev.leaf_size, dev.leaf_node, dev.ids, b.lb, b.heap, mp, b.cap, b.cand,
+                     dev.leaf_pos, dev.leaf_size, dev.leaf_node, dev.ids, b.lb, b.heap, b.bfs, mp, b.cap, b.cand,
                      b.cand_cnt, b.bound, b.work_keys, nwork, b.info, b.counters);
   if (pf) pf->mark(st, HKG_STAGE_APPROX);
   if (mp.nprobes == 0) {

@@ -103,6 +103,7 @@ struct QueryBuffers {
   // approximate modes
   float *nlb = nullptr;         // B x nnodes exact node bounds
   uint64_t *heap = nullptr;     // B x (nnodes + 1) traversal heaps (when too big for LDS)
+  uint32_t *bfs = nullptr;      // B x 2 (nnodes + 1) level lists of the candidate walk
   uint32_t *info = nullptr;     // B x 4 per-query traversal counts
   bool busy = false;            // held by a query call (guarded by Index::mu_bufs)
   // captured stage sequence of the batched path

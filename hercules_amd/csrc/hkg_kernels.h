@@ -35,7 +35,10 @@ struct ModeParams {
   uint32_t nleaves;   // leaves with rows (the reference's leaves_counter)
   uint64_t nseries;   // rows under the root
 };
-constexpr uint32_t kModeHeapLds = 2048;  // heap entries kept in LDS (else in scratch)
+// the traversal heap lives in LDS when it and the query fit in this many
+// bytes (one block per CU), else in per-query scratch
+constexpr size_t kModeLdsMax = 144 * 1024;
+bool mode_heap_in_lds(uint32_t L, uint32_t nnodes);
 
 hipError_t upload_breakpoints_dev();
 
@@ -76,8 +79,8 @@ void launch_mode_search(hipStream_t st, uint32_t nq, const float *X, const uint8
                         const float *paa, const uint32_t *kq, uint32_t nnodes, const int4 *nd_link,
                         const int4 *nd_split, const float *nlb, uint32_t nleaf, const uint64_t *leaf_pos,
                         const uint32_t *leaf_size, const int32_t *leaf_node, const uint32_t *ids, const float *lb,
-                        uint64_t *heap_g, const ModeParams &mp, uint32_t cap, uint64_t *cand, uint32_t *cand_cnt,
-                        uint32_t *bound, uint32_t *keys, uint32_t *nwork, uint32_t *info,
+                        uint64_t *heap_g, uint32_t *bfs, const ModeParams &mp, uint32_t cap, uint64_t *cand,
+                        uint32_t *cand_cnt, uint32_t *bound, uint32_t *keys, uint32_t *nwork, uint32_t *info,
                         unsigned long long *counters);
 void launch_scan(hipStream_t st, uint32_t grid_x, const float *X, uint64_t n, uint32_t L, const float *Q,
                  uint32_t nq, const uint32_t *kq, const uint32_t *ids, uint32_t cap, uint64_t *cand,

@@ -1215,19 +1215,58 @@ __device__ bool heap_pop(uint64_t *d, uint32_t &size, uint64_t &out)
   return true;
 }
 
-// rows [beg, end) of a leaf into the block's running answer
+// rows [beg, end) of a leaf into the block's running answer: one row per
+// lane, streamed in 128-byte steps and folded in the reference's order
+// (ts.c:83-109), abandoned once the partial sum exceeds the k-th distance
+// (partial sums only grow, so no row within it is lost).  Passing rows are
+// staged and merged only when the staging area would overflow, and once at
+// the end: the answer after the call is the same, with far fewer merges.
 __device__ void scan_rows(const float *__restrict__ X, uint32_t L, const float *qs, uint64_t pos, uint32_t beg,
-                          uint32_t end, const uint32_t *__restrict__ ids, int k, TopK &tk, RefineShared &rs,
-                          uint64_t *rows, uint32_t *scan, uint32_t &nref)
+                          uint32_t end, const uint32_t *__restrict__ ids, int k, TopK &tk, uint32_t *scan,
+                          uint32_t &nref)
 {
+  __shared__ uint32_t s_pass;
+  const int nch = (int)(L >> 3);
+  const float4 *q4 = reinterpret_cast<const float4 *>(qs);
   for (uint32_t t0 = beg; t0 < end; t0 += kBlock) {
-    const uint32_t nr = end - t0 < (uint32_t)kBlock ? end - t0 : (uint32_t)kBlock;
-    if (threadIdx.x < nr) rows[threadIdx.x] = pos + t0 + threadIdx.x;
+    if (threadIdx.x == 0) s_pass = 0;
     __syncthreads();
-    refine_rows(X, L, qs, rows, (int)nr, ids, tk.kth_bits, tk, rs, nref);
+    const uint32_t bnd = tk.kth_bits;
+    const uint32_t r = t0 + threadIdx.x;
+    bool pass = false;
+    float acc = 0.0f;
+    uint64_t row = 0;
+    if (r < end) {
+      row = pos + r;
+      const float4 *xp = reinterpret_cast<const float4 *>(X + row * L);
+      bool alive = true;
+      int c = 0;
+      for (; c + 4 <= nch && alive; c += 4) {
+        float4 xv[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) xv[i] = ld_stream(reinterpret_cast<const float *>(xp + 2 * c + i));
+#pragma unroll
+        for (int sub = 0; sub < 4; ++sub)
+          acc = acc + ed_chunk(q4[2 * (c + sub)], q4[2 * (c + sub) + 1], xv[2 * sub], xv[2 * sub + 1]);
+        alive = __float_as_uint(acc) <= bnd;
+      }
+      for (; c < nch && alive; ++c) {
+        const float4 xa = ld_stream(reinterpret_cast<const float *>(xp + 2 * c));
+        const float4 xb = ld_stream(reinterpret_cast<const float *>(xp + 2 * c + 1));
+        acc = acc + ed_chunk(q4[2 * c], q4[2 * c + 1], xa, xb);
+        alive = __float_as_uint(acc) <= bnd;
+      }
+      pass = alive;
+      ++nref;
+    }
+    const uint64_t bal = __ballot(pass);
+    if ((threadIdx.x & 63) == 0 && bal) atomicAdd(&s_pass, (uint32_t)__popcll(bal));
     __syncthreads();
-    topk_merge(tk, scan, k);
+    if (tk.nstage + (int)s_pass > kBlock) topk_merge(tk, scan, k);  // uniform
+    if (pass) topk_stage(tk, make_key(acc, ids ? ids[row] : (uint32_t)row));
+    __syncthreads();
   }
+  topk_merge(tk, scan, k);
 }
 
 // one query per block.  ng (nprobes > 0): the leaf the split policies route
@@ -1246,7 +1285,8 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
     const int4 *__restrict__ nd_link, const int4 *__restrict__ nd_split, const float *__restrict__ nlb,
     uint32_t nleaf, const uint64_t *__restrict__ leaf_pos, const uint32_t *__restrict__ leaf_size,
     const int32_t *__restrict__ leaf_node, const uint32_t *__restrict__ ids, const float *__restrict__ lb,
-    uint64_t *__restrict__ heap_g, int heap_in_lds, ModeParams mp, uint32_t cap, uint64_t *__restrict__ cand,
+    uint64_t *__restrict__ heap_g, int heap_in_lds, uint32_t *__restrict__ bfs, ModeParams mp, uint32_t cap,
+    uint64_t *__restrict__ cand,
     uint32_t *__restrict__ cand_cnt, uint32_t *__restrict__ bound, uint32_t *__restrict__ keys,
     uint32_t *__restrict__ nwork, uint32_t *__restrict__ info, unsigned long long *__restrict__ counters)
 {
@@ -1256,8 +1296,6 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
   uint64_t *hp = heap_in_lds ? reinterpret_cast<uint64_t *>(mode_lds + ((L * 4 + 15) & ~15u))
                              : heap_g + (uint64_t)q * (nnodes + 1);
   __shared__ TopK tk;
-  __shared__ RefineShared rs;
-  __shared__ uint64_t rows[kBlock];
   __shared__ uint32_t scan[kBlock];
   __shared__ float bp[256];
   __shared__ float spaa[kPaaSegments];
@@ -1295,7 +1333,7 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
     __syncthreads();
     const int32_t lf = nd_link[s_node].z;
     if (lf >= 0) {
-      scan_rows(X, L, qs, leaf_pos[lf], 0, leaf_size[lf], ids, k, tk, rs, rows, scan, nref);
+      scan_rows(X, L, qs, leaf_pos[lf], 0, leaf_size[lf], ids, k, tk, scan, nref);
       first_rows += leaf_size[lf];
     }
     if (threadIdx.x == 0) {
@@ -1341,7 +1379,7 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
     if (node < 0) break;
     const int32_t lf = nd_link[node].z;
     if (lf >= 0) {
-      scan_rows(X, L, qs, leaf_pos[lf], 0, leaf_size[lf], ids, k, tk, rs, rows, scan, nref);
+      scan_rows(X, L, qs, leaf_pos[lf], 0, leaf_size[lf], ids, k, tk, scan, nref);
       first_rows += leaf_size[lf];
     }
     if (threadIdx.x == 0) ++nvis;
@@ -1351,36 +1389,59 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
   if (!ng) {
     __shared__ float s_kth0;
     __shared__ int s_branch;
+    __shared__ uint32_t s_hsize, s_ncand, s_nempty, s_nnext;
     if (threadIdx.x == 0) {
-      // candidate leaves: the same queue against kth0 / (1 + epsilon)
-      const float kth0 = __uint_as_float(tk.kth_bits);
-      const float thr = kth0 / (1.0f + mp.epsilon);
-      uint64_t it;
-      while (heap_pop(hp, hsize, it)) {
-        const float pri = __uint_as_float(hpri(it));
-        if (pri > thr) break;
-        const uint32_t node = (uint32_t)it;
-        const int4 ln = nd_link[node];
-        if (ln.w) {
-          ++ncand;
-          if (ln.z >= 0) qkeys[ln.z] = __float_as_uint(lb[(uint64_t)q * nleaf + ln.z]);
-          else ++nempty;
-          continue;
-        }
-        for (int c = 0; c < 2; ++c) {
-          const int32_t ch = c ? ln.y : ln.x;
-          const float b = qnlb[ch];
-          if (b < thr) heap_push(hp, hsize, b, (uint32_t)ch);
-        }
+      s_kth0 = __uint_as_float(tk.kth_bits);
+      s_hsize = hsize;
+      s_ncand = 0;
+      s_nempty = 0;
+      s_nnext = 0;
+      s_lab = 0;
+    }
+    __syncthreads();
+    // candidate leaves (:836-891): every queued node within kth0 / (1 +
+    // epsilon) and all it reaches through children below that bound.  The
+    // bound no longer moves, so the queue's order decides only when a leaf
+    // is found, not which: the block walks the tree level by level.
+    const float thr = s_kth0 / (1.0f + mp.epsilon);
+    uint32_t *cur = bfs + (uint64_t)q * 2 * (nnodes + 1), *nxt = cur + (nnodes + 1);
+    auto visit = [&](uint32_t node) __attribute__((always_inline)) {
+      const int4 ln = nd_link[node];
+      if (ln.w) {
+        atomicAdd(&s_ncand, 1u);
+        if (ln.z >= 0) qkeys[ln.z] = __float_as_uint(lb[(uint64_t)q * nleaf + ln.z]);
+        else atomicAdd(&s_nempty, 1u);
+      } else {
+        if (qnlb[ln.x] < thr) nxt[atomicAdd(&s_nnext, 1u)] = (uint32_t)ln.x;
+        if (qnlb[ln.y] < thr) nxt[atomicAdd(&s_nnext, 1u)] = (uint32_t)ln.y;
       }
-      s_kth0 = kth0;
+    };
+    for (uint32_t i = 1 + threadIdx.x; i < s_hsize; i += kBlock) {
+      const uint64_t e = hp[i];
+      if (__uint_as_float(hpri(e)) <= thr) visit((uint32_t)e);
+    }
+    __syncthreads();
+    for (;;) {
+      const uint32_t n = s_nnext;
+      __syncthreads();
+      if (n == 0) break;
+      if (threadIdx.x == 0) s_nnext = 0;
+      uint32_t *t = cur;
+      cur = nxt;
+      nxt = t;
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < n; i += kBlock) visit(cur[i]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      ncand = s_ncand;
+      nempty = s_nempty;
       int br = 0;
       if (ncand) {
         const float flp = (float)(1.0 - (double)ncand / (double)(float)mp.nleaves);
         br = flp < mp.eapca_th ? 1 : 3;  // 3: decided by the iSAX count
       }
       s_branch = br;
-      s_lab = 0;
     }
     __syncthreads();
     const float kth0 = s_kth0;
@@ -1400,9 +1461,17 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
           const uint32_t lf = s_list[j];
           const uint64_t pos = leaf_pos[lf];
           const uint32_t size = leaf_size[lf];
-          for (uint32_t r = threadIdx.x; r < size; r += kBlock) {
-            const uint4 w = *reinterpret_cast<const uint4 *>(sax + (pos + r) * kPaaSegments);
-            mine += lb_sax_word(spaa, w, bp, ratio) <= kth0 ? 1ull : 0ull;
+          // four rows per lane in flight
+          for (uint32_t r = threadIdx.x; r < size; r += 4 * kBlock) {
+            uint4 w[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const uint32_t rr = r + u * kBlock;
+              w[u] = rr < size ? *reinterpret_cast<const uint4 *>(sax + (pos + rr) * kPaaSegments) : make_uint4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (r + u * kBlock < size) mine += lb_sax_word(spaa, w[u], bp, ratio) <= kth0 ? 1ull : 0ull;
           }
         }
         __syncthreads();
@@ -1447,9 +1516,9 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
           const uint32_t size = leaf_size[lf];
           if (s_go && size > 0) {
             const uint64_t pos = leaf_pos[lf];
-            scan_rows(X, L, qs, pos, 0, size - 1, ids, k, tk, rs, rows, scan, nref);
+            scan_rows(X, L, qs, pos, 0, size - 1, ids, k, tk, scan, nref);
             if (threadIdx.x == 0) s_kthb = __uint_as_float(tk.kth_bits);  // read before the last row
-            scan_rows(X, L, qs, pos, size - 1, size, ids, k, tk, rs, rows, scan, nref);
+            scan_rows(X, L, qs, pos, size - 1, size, ids, k, tk, scan, nref);
             ++nscanned;
           }
           __syncthreads();
@@ -1487,23 +1556,35 @@ void launch_mode_keys(hipStream_t st, uint32_t pairs, uint32_t *keys, uint32_t *
   hipLaunchKernelGGL(k_mode_keys, dim3((pairs + 255) / 256), dim3(256), 0, st, pairs, keys, vals);
 }
 
-size_t mode_search_lds(uint32_t L, uint32_t nnodes, bool heap_in_lds)
+// The queue only ever holds roots of disjoint subtrees, so it never has more
+// entries than the tree has leaves ((nnodes + 1) / 2 in a full binary tree).
+static size_t mode_heap_bytes(uint32_t nnodes) { return ((size_t)(nnodes + 1) / 2 + 2) * 8; }
+static size_t mode_qs_bytes(uint32_t L) { return ((size_t)L * 4 + 15) & ~(size_t)15; }
+
+bool mode_heap_in_lds(uint32_t L, uint32_t nnodes)
 {
-  return ((L * 4 + 15) & ~15u) + (heap_in_lds ? (size_t)(nnodes + 1) * 8 : 0);
+  return mode_qs_bytes(L) + mode_heap_bytes(nnodes) <= kModeLdsMax;
 }
 
 void launch_mode_search(hipStream_t st, uint32_t nq, const float *X, const uint8_t *sax, uint32_t L, const float *Q,
                         const float *paa, const uint32_t *kq, uint32_t nnodes, const int4 *nd_link,
                         const int4 *nd_split, const float *nlb, uint32_t nleaf, const uint64_t *leaf_pos,
                         const uint32_t *leaf_size, const int32_t *leaf_node, const uint32_t *ids, const float *lb,
-                        uint64_t *heap_g, const ModeParams &mp, uint32_t cap, uint64_t *cand, uint32_t *cand_cnt,
-                        uint32_t *bound, uint32_t *keys, uint32_t *nwork, uint32_t *info,
+                        uint64_t *heap_g, uint32_t *bfs, const ModeParams &mp, uint32_t cap, uint64_t *cand,
+                        uint32_t *cand_cnt, uint32_t *bound, uint32_t *keys, uint32_t *nwork, uint32_t *info,
                         unsigned long long *counters)
 {
-  const bool in_lds = (nnodes + 1) <= kModeHeapLds;
-  hipLaunchKernelGGL(k_mode_search, dim3(nq), dim3(kBlock), mode_search_lds(L, nnodes, in_lds), st, X, sax, L, Q,
+  const bool in_lds = mode_heap_in_lds(L, nnodes);
+  const size_t lds = mode_qs_bytes(L) + (in_lds ? mode_heap_bytes(nnodes) : 0);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_mode_search),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kModeLdsMax);
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_mode_search, dim3(nq), dim3(kBlock), lds, st, X, sax, L, Q,
                      paa, kq, nnodes, nd_link, nd_split, nlb, nleaf, leaf_pos, leaf_size, leaf_node, ids, lb, heap_g,
-                     in_lds ? 1 : 0, mp, cap, cand, cand_cnt, bound, keys, nwork, info, counters);
+                     in_lds ? 1 : 0, bfs, mp, cap, cand, cand_cnt, bound, keys, nwork, info, counters);
 }
 
 }  // namespace hkg

@@ -200,10 +200,17 @@ class DeviceArray:
         _check(lib().hkg_memcpy(_ptr(a), self.ptr, a.nbytes, 2))
         return a
 
+    @classmethod
+    def view(cls, ptr: int, nbytes: int) -> "DeviceArray":
+        """A view of device memory that this object never frees."""
+        v = cls.__new__(cls)
+        v.ptr, v.nbytes, v.owned = ptr, int(nbytes), False
+        return v
+
     def close(self) -> None:
-        if self.ptr:
+        if self.ptr and getattr(self, "owned", True):
             lib().hkg_device_free(self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
