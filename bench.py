@@ -58,6 +58,9 @@ def parse() -> argparse.Namespace:
                    help="epsilon of the approximate-mode leg (0: skip that leg)")
     p.add_argument("--mode-nprobes", type=int, default=10, help="nprobes of the ng-mode leg (0: skip that leg)")
     p.add_argument("--mode-queries", type=int, default=256, help="queries of each mode leg (from the first step)")
+    p.add_argument("--real-queries", type=int, default=16,
+                   help="real-data leg: Seismic query files on the index; this many checked against the full "
+                        "scan (0: skip the leg)")
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
                    help="one device-to-device copy of this size first (reference dispatch for counter runs)")
     return p.parse_args()
@@ -312,6 +315,31 @@ def main() -> int:
             modes[name] = leg
             log(f"mode {name}: {leg}")
 
+    # real-data queries (the reference's Seismic query files, len 256) on the
+    # same index: throughput of one batch, and the answers of a few against
+    # the full scan
+    real = {}
+    if args.real_queries and L == 256:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import golden_data as gd
+        Qr = gd.real_queries("seismic")
+        kr = np.full(Qr.shape[0], 10, np.uint32)
+        ix.knn(Qr[:8], 10, path=args.path)  # warm the shape
+        hkg.lib().hkg_device_sync()
+        tr = time.perf_counter()
+        rr_ = ix.knn(Qr, kr, path=args.path, batch=Qr.shape[0])
+        hkg.lib().hkg_device_sync()
+        ms = (time.perf_counter() - tr) * 1e3
+        sc = ix.knn(Qr[:args.real_queries], 10, path="scan")
+        same = bool(np.array_equal(rr_.ids[:args.real_queries], sc.ids) and
+                    np.array_equal(rr_.d2[:args.real_queries].view(np.uint32), sc.d2.view(np.uint32)))
+        real = {"queries": "Seismic hard1p + hard10p + size100 (300, len 256), k = 10",
+                "queries_per_s_host_to_host": Qr.shape[0] / (ms * 1e-3), "ms": ms,
+                "device_ms": rr_.stats["device_ms"],
+                "series_refined_per_query": rr_.stats["series_refined"] / Qr.shape[0],
+                "identical_to_full_scan": same, "checked": int(args.real_queries)}
+        log(f"real queries: {real}")
+
     serving = None
     if args.serve_requests:
         sq = args.serve_qps if args.serve_qps > 0 else 0.5 * qps
@@ -354,6 +382,7 @@ def main() -> int:
         "series_lb_checked_per_query": checked / (steps * B),
         "parity": par,
         "modes": modes,
+        "real_queries": real,
         "serving": serving,
         "cpu_baseline": cpu,
     }

@@ -6,7 +6,7 @@ FP32 squared distances per rank (printed in hex by oracle/ref_interpose.c)
 and the oracle's brute-force ids and distances.  Datasets are not stored:
 they are regenerated from the seeds by the generator specification.
 
-    python tests/golden/make_golden.py [tiny|c1|emb|all|modes]
+    python tests/golden/make_golden.py [tiny|c1|emb|all|modes|real]
 """
 from __future__ import annotations
 
@@ -154,9 +154,73 @@ def make_modes(name: str) -> None:
         shutil.rmtree(work, ignore_errors=True)
 
 
+# real-data query files (tests/golden/real_queries) on the synthetic
+# datasets of their length: the committed tiny (len 256) and embedding
+# (len 96) indexes, and a 20K-series random-walk set of len 128
+REAL_SETS = {
+    "seismic": dict(index="tiny", files=["queries_hard1p_seismic_len256_znorm.bin",
+                                         "queries_hard10p_seismic_len256_znorm.bin",
+                                         "queries_size100_seismic_len256_znorm.bin"]),
+    "deep1b": dict(index="emb", files=["queries_hard1p_deep1b_len96_znorm.bin",
+                                       "queries_orig100_deep1b_len96_znorm.bin"]),
+    "sald": dict(index=None, kind=0, N=20000, L=128, seed=31, leaf=500,
+                 files=["queries_hard1p_sald_len128_znorm.bin"]),
+}
+REAL_MODES = [dict(k=10, epsilon=2.0, budget=1, eapca=0.25, sax=0.9), dict(k=10, nprobes=5)]
+
+
+def real_queries(name: str) -> np.ndarray:
+    import golden_data as gd
+    return gd.real_queries(name)
+
+
+def make_real(name: str) -> None:
+    """The reference CLI's exact answers (k = 10), and on the committed
+    indexes its epsilon / nprobes answers, for a real query set."""
+    import golden_data as gd
+    c = REAL_SETS[name]
+    o = Oracle()
+    Q = real_queries(name)
+    work = tempfile.mkdtemp(prefix=f"golden_real_{name}_", dir=os.environ.get("TMPDIR", "/tmp"))
+    try:
+        idx = os.path.join(work, "idx")
+        if c["index"]:
+            m = gd.meta(c["index"])
+            raw, _ = gd.reference_index(o, c["index"], idx)
+            X = o.series(m["kind"], m["seed"], 0, m["N"], m["L"])
+            L = m["L"]
+        else:
+            L = c["L"]
+            X = o.series(c["kind"], c["seed"], 0, c["N"], L)
+            rr.write_f32(os.path.join(work, "data.bin"), X)
+            rr.build_index(os.path.join(work, "data.bin"), c["N"], L, c["leaf"], idx, threads=8)
+        rr.write_f32(os.path.join(work, "q.bin"), Q)
+        out = {}
+        rd2, _, _, _ = rr.query(idx, os.path.join(work, "q.bin"), Q.shape[0], L, 10, threads=8, sims=True)
+        ids, d2 = o.knn(X, Q, 10, threads=8)
+        if not np.array_equal(rd2.view(np.uint32), d2.view(np.uint32)):
+            raise RuntimeError(f"{name}: reference SIMS differs from brute force")
+        out["exact_d2"], out["exact_ids"] = d2, ids
+        if c["index"]:
+            for i, mc in enumerate(REAL_MODES):
+                md2, _, _, _ = rr.query(idx, os.path.join(work, "q.bin"), Q.shape[0], L, mc["k"], threads=8,
+                                        sims=True, eapca_threshold=mc.get("eapca", 0.25),
+                                        sax_threshold=mc.get("sax", 0.9), extra=mode_flags(mc))
+                out[f"m{i}"] = md2
+        np.savez_compressed(os.path.join(HERE, f"real_{name}.npz"), **out)
+        with open(os.path.join(HERE, f"real_{name}.json"), "w") as fh:
+            json.dump(dict(c, modes=REAL_MODES if c["index"] else []), fh, indent=1)
+        print("real", name, Q.shape)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 if __name__ == "__main__":
     which = sys.argv[1] if len(sys.argv) > 1 else "all"
-    if which == "modes":
+    if which == "real":
+        for n in REAL_SETS:
+            make_real(n)
+    elif which == "modes":
         for n in MODE_CONFIGS:
             make_modes(n)
     else:

@@ -57,3 +57,28 @@ def reference_index(gen, name: str, out_dir: str):
     raw.tofile(os.path.join(out_dir, "leaves_raw.idx"))
     sims.tofile(os.path.join(out_dir, "leaves_sims.idx"))
     return raw, sims
+
+
+REAL_DIR = os.path.join(GOLDEN, "real_queries")
+
+
+def real_set(name: str) -> dict:
+    with open(os.path.join(GOLDEN, f"real_{name}.json")) as f:
+        return json.load(f)
+
+
+def real_answers(name: str):
+    return np.load(os.path.join(GOLDEN, f"real_{name}.npz"))
+
+
+def real_queries(name: str) -> np.ndarray:
+    """The real query files of a set, concatenated (tests/golden/real_queries)."""
+    files = {"seismic": ["queries_hard1p_seismic_len256_znorm.bin", "queries_hard10p_seismic_len256_znorm.bin",
+                         "queries_size100_seismic_len256_znorm.bin"],
+             "deep1b": ["queries_hard1p_deep1b_len96_znorm.bin", "queries_orig100_deep1b_len96_znorm.bin"],
+             "sald": ["queries_hard1p_sald_len128_znorm.bin"]}[name]
+    out = []
+    for f in files:
+        L = int(f.split("_len")[1].split("_")[0])
+        out.append(np.fromfile(os.path.join(REAL_DIR, f), np.float32).reshape(-1, L))
+    return np.ascontiguousarray(np.concatenate(out))
