@@ -1390,6 +1390,7 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
     __shared__ float s_kth0;
     __shared__ int s_branch;
     __shared__ uint32_t s_hsize, s_ncand, s_nempty, s_nnext;
+    __shared__ unsigned long long s_crows;
     if (threadIdx.x == 0) {
       s_kth0 = __uint_as_float(tk.kth_bits);
       s_hsize = hsize;
@@ -1397,6 +1398,7 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
       s_nempty = 0;
       s_nnext = 0;
       s_lab = 0;
+      s_crows = 0;
     }
     __syncthreads();
     // candidate leaves (:836-891): every queued node within kth0 / (1 +
@@ -1409,8 +1411,12 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
       const int4 ln = nd_link[node];
       if (ln.w) {
         atomicAdd(&s_ncand, 1u);
-        if (ln.z >= 0) qkeys[ln.z] = __float_as_uint(lb[(uint64_t)q * nleaf + ln.z]);
-        else atomicAdd(&s_nempty, 1u);
+        if (ln.z >= 0) {
+          qkeys[ln.z] = __float_as_uint(lb[(uint64_t)q * nleaf + ln.z]);
+          atomicAdd(&s_crows, (unsigned long long)leaf_size[ln.z]);
+        } else {
+          atomicAdd(&s_nempty, 1u);
+        }
       } else {
         if (qnlb[ln.x] < thr) nxt[atomicAdd(&s_nnext, 1u)] = (uint32_t)ln.x;
         if (qnlb[ln.y] < thr) nxt[atomicAdd(&s_nnext, 1u)] = (uint32_t)ln.y;
@@ -1440,6 +1446,10 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
       if (ncand) {
         const float flp = (float)(1.0 - (double)ncand / (double)(float)mp.nleaves);
         br = flp < mp.eapca_th ? 1 : 3;  // 3: decided by the iSAX count
+        // the count is at most every candidate row, and 1 - count / n only
+        // falls as the count grows (each rounding is monotone): when even
+        // all of them keep it at or above the threshold, so does the count
+        if (br == 3 && !((float)(1.0 - (double)s_crows / (double)(float)mp.nseries) < mp.sax_th)) br = 2;
       }
       s_branch = br;
     }
