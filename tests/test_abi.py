@@ -62,3 +62,33 @@ def test_inspect_dir_errors(tmp_path):
     with pytest.raises(hkg.HkgError) as e:
         hkg.inspect_dir(str(tmp_path))
     assert e.value.status == 4
+
+
+def test_query_opts_layout_and_defaults(tmp_path):
+    """The ctypes mirror of hkg_query_opts has the C layout (a C program
+    compiled against include/hkg.h prints the offsets), and
+    hkg_default_query_opts sets the reference CLI's defaults
+    (hercules.c:48-51, :70-72): exact search, 80-leaf budget, 0.25 / 0.9."""
+    import ctypes as C
+
+    from hercules_amd import hkg
+    fields = ["path", "batch", "profile", "cand_cap", "bf16_bound_pass", "epsilon", "delta", "nprobes",
+              "approx_leaves", "eapca_threshold", "sax_threshold", "reserved"]
+    sfields = ["device_ms", "bf16_pairs", "stage_ms", "stage_launches", "mode_leaves_first", "mode_sequential"]
+    prog = tmp_path / "layout.c"
+    prog.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "hkg.h"\nint main(void){\n'
+                    + "".join(f'printf("%zu\\n", offsetof(hkg_query_opts, {f}));\n' for f in fields)
+                    + 'printf("%zu\\n", sizeof(hkg_query_opts));\n'
+                    + "".join(f'printf("%zu\\n", offsetof(hkg_query_stats, {f}));\n' for f in sfields)
+                    + 'printf("%zu\\n", sizeof(hkg_query_stats));\nreturn 0;}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), str(prog), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [getattr(hkg.QueryOpts, f).offset for f in fields] + [C.sizeof(hkg.QueryOpts)]
+    want += [getattr(hkg.QueryStats, f).offset for f in sfields] + [C.sizeof(hkg.QueryStats)]
+    assert got == want
+    qo = hkg.query_opts()
+    assert qo.epsilon == 0.0 and qo.nprobes == 0 and qo.delta == 1.0 and qo.approx_leaves == 80
+    assert abs(qo.eapca_threshold - 0.25) < 1e-7 and abs(qo.sax_threshold - 0.9) < 1e-7
+    with pytest.raises(TypeError):
+        hkg.query_opts(epsilonn=1.0)

@@ -71,6 +71,29 @@ def test_modes_reproduce_reference_cli(hkg, oracle, name, tmp_path):
                 assert r.stats["overflow_retries"] > 0
 
 
+def test_modes_with_per_query_k(hkg, oracle, tmp_path):
+    """k differs per query: each query's answer is the oracle's for its k."""
+    d = tmp_path / "idx"
+    raw, sims = gd.reference_index(oracle, "tiny", str(d))
+    tabs = node_tables(parse_root_idx(str(d / "root.idx")))
+    Q = gd.queries(oracle, gd.meta("tiny"))
+    kv = np.array([1, 3, 10, 64, 100, 7][: 6] * (Q.shape[0] // 6), np.uint32)
+    with hkg.HerculesIndex.open(str(d)) as ix:
+        for kw, okw in [({"epsilon": 3.0, "approx_leaves": 1}, {"epsilon": 3.0, "approx_budget": 1}),
+                        ({"epsilon": 0.5, "approx_leaves": 2, "eapca_threshold": 0.99},
+                         {"epsilon": 0.5, "approx_budget": 2, "eapca_th": 0.99}),
+                        ({"nprobes": 4}, {"nprobes": 4})]:
+            print(f"modes per-query k {kw}", flush=True)
+            r = ix.knn(Q, kv, **kw)
+            for k in np.unique(kv):
+                sel = np.nonzero(kv == k)[0]
+                ids, d2, _ = oracle.knn_mode(tabs, raw, sims, np.ascontiguousarray(Q[sel]), int(k), **okw)
+                assert np.array_equal(r.d2[sel, :k].view(np.uint32), d2.view(np.uint32)), (kw, k)
+                assert np.array_equal(r.ids[sel, :k], ids), (kw, k)
+                if k < r.d2.shape[1]:
+                    assert (r.ids[sel, k:] == np.uint64(2**64 - 1)).all()
+
+
 def test_modes_on_device_built_index(hkg, oracle, tmp_path):
     n, L = 60000, 256
     X = oracle.series(0, 41, 0, n, L)
