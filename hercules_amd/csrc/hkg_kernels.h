@@ -11,6 +11,7 @@ constexpr int kScanQ = 4;                // queries per block in the full scan
 inline uint32_t scan_group_size(uint32_t nq) { return nq <= 2 ? (nq ? nq : 1) : (uint32_t)kScanQ; }
 constexpr size_t kEapcaLdsBytes = 48 * 1024;
 constexpr int kWorkChunk = 8;            // work items taken per atomic
+constexpr int kFilterRows = 1024;        // rows per iSAX filter step of k_filter_refine
 constexpr int kBestLeaves = 4;           // leaves per query scanned in the approximate stage
 constexpr int kBf16Bins = 32;            // threshold histogram bins per query (BF16 path)
 

@@ -589,7 +589,7 @@ __global__ void __launch_bounds__(kBlock) k_filter_refine(
   extern __shared__ float qs[];
   __shared__ TopK tk;
   __shared__ RefineShared rs;
-  __shared__ uint64_t rows[kBlock];
+  __shared__ uint64_t rows[kFilterRows];
   __shared__ uint32_t scan[kBlock];
   __shared__ float bp[256];
   __shared__ float spaa[kPaaSegments];
@@ -640,7 +640,9 @@ __global__ void __launch_bounds__(kBlock) k_filter_refine(
     const uint64_t pos = leaf_pos[leaf];
     const uint32_t size = leaf_size[leaf];
     ++nleaves;
-    for (uint32_t t0 = 0; t0 < size; t0 += kBlock) {
+    // kFilterRows rows per step: each lane's iSAX words are loaded together,
+    // the survivors refined kBlock at a time
+    for (uint32_t t0 = 0; t0 < size; t0 += kFilterRows) {
       if (threadIdx.x == 0) {
         uint32_t gb = __atomic_load_n(&bound[q], __ATOMIC_RELAXED);
         s_bound = gb < tk.kth_bits ? gb : tk.kth_bits;
@@ -650,27 +652,34 @@ __global__ void __launch_bounds__(kBlock) k_filter_refine(
       __syncthreads();
       if (s_stop) break;
       const uint32_t bnd = s_bound;
-      const uint32_t r = t0 + threadIdx.x;
-      bool keep = false;
-      if (r < size) {
-        uint4 word = *reinterpret_cast<const uint4 *>(sax + (pos + r) * kPaaSegments);
-        keep = lb_keep(lb_sax_word(spaa, word, bp, ratio), bnd);
+      uint4 word[kFilterRows / kBlock];
+#pragma unroll
+      for (int u = 0; u < kFilterRows / kBlock; ++u) {
+        const uint32_t r = t0 + u * kBlock + threadIdx.x;
+        word[u] = r < size ? *reinterpret_cast<const uint4 *>(sax + (pos + r) * kPaaSegments) : make_uint4(0, 0, 0, 0);
       }
-      nchecked += (threadIdx.x == 0) ? (unsigned long long)min((uint32_t)kBlock, size - t0) : 0ull;
-      uint64_t mask = __ballot(keep);
-      int lane = threadIdx.x & 63;
-      int leader = __ffsll((long long)mask) - 1;
-      uint32_t wbase = 0;
-      if (mask) {
-        if (lane == leader) wbase = atomicAdd(&s_nsurv, (uint32_t)__popcll(mask));
-        wbase = __shfl(wbase, leader);
+      const int lane = threadIdx.x & 63;
+#pragma unroll
+      for (int u = 0; u < kFilterRows / kBlock; ++u) {
+        const uint32_t r = t0 + u * kBlock + threadIdx.x;
+        const bool keep = r < size && lb_keep(lb_sax_word(spaa, word[u], bp, ratio), bnd);
+        const uint64_t mask = __ballot(keep);
+        const int leader = __ffsll((long long)mask) - 1;
+        uint32_t wbase = 0;
+        if (mask) {
+          if (lane == leader) wbase = atomicAdd(&s_nsurv, (uint32_t)__popcll(mask));
+          wbase = __shfl(wbase, leader);
+        }
+        if (keep) rows[wbase + __popcll(mask & ((1ull << lane) - 1))] = pos + r;
       }
-      if (keep) rows[wbase + __popcll(mask & ((1ull << lane) - 1))] = pos + r;
+      nchecked += (threadIdx.x == 0) ? (unsigned long long)min((uint32_t)kFilterRows, size - t0) : 0ull;
       __syncthreads();
       const int ns = (int)s_nsurv;
-      refine_rows(X, L, qs, rows, ns, ids, bnd, tk, rs, nref);
-      __syncthreads();
-      topk_merge(tk, scan, k);
+      for (int g = 0; g < ns; g += kBlock) {
+        refine_rows(X, L, qs, rows + g, ns - g < kBlock ? ns - g : kBlock, ids, bnd, tk, rs, nref);
+        __syncthreads();
+        topk_merge(tk, scan, k);
+      }
     }
     flush_topk(tk, k, q, cap, cand, cand_cnt, bound);
   }
