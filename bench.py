@@ -61,6 +61,8 @@ def parse() -> argparse.Namespace:
     p.add_argument("--real-queries", type=int, default=16,
                    help="real-data leg: Seismic query files on the index; this many checked against the full "
                         "scan (0: skip the leg)")
+    p.add_argument("--loader-series", type=int, default=0,
+                   help="loader leg: write an index of this many series to TMPDIR and time hkg_open_dir (0: skip)")
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
                    help="one device-to-device copy of this size first (reference dispatch for counter runs)")
     return p.parse_args()
@@ -165,6 +167,38 @@ def serving_leg(hkg, ix, args, Qh: np.ndarray, kq: np.ndarray, qps: float) -> di
             "p99_ms": round(float(np.percentile(lat, 99)) / 1e3, 3),
             "max_batch": args.batch, "inflight": 2, "path": args.path,
             "answers_identical_to_direct_queries": same, "sampled": int(len(sample))}
+
+
+def loader_leg(hkg, args, L: int, Qh: np.ndarray) -> dict:
+    """An index of loader_series series built on the device, written in the
+    reference's format (root.idx, leaves_raw.idx, leaves_sims.idx) and read
+    back by hkg_open_dir; the files are fresh, so they come from the page
+    cache: the figure is the loader's pinned-staging upload, not a disk's.
+    Both indexes answer the same queries identically."""
+    import shutil
+    n = args.loader_series
+    dX = hkg.DeviceArray(n * L * 4)
+    hkg.gen_series_device(0, 7, 0, n, L, dX)
+    d = tempfile.mkdtemp(prefix="hkg_loader_", dir=os.environ.get("TMPDIR", "/tmp"))
+    try:
+        built = hkg.HerculesIndex.build_device(dX, n, L, leaf_size=args.leaf)
+        del dX
+        tw = time.perf_counter()
+        built.write(d)
+        write_s = time.perf_counter() - tw
+        Qs = Qh[:16]
+        a = built.knn(Qs, 10, path="batched")
+        built.close()
+        t0 = time.perf_counter()
+        ix2 = hkg.HerculesIndex.open(d)
+        open_s = time.perf_counter() - t0
+        b = ix2.knn(Qs, 10, path="batched")
+        ix2.close()
+        nbytes = n * L * 4 + n * 16
+        return {"series": n, "bytes": nbytes, "open_s": open_s, "open_gbs": nbytes / open_s / 1e9,
+                "write_s": write_s, "answers_identical": bool(np.array_equal(a.d2.view(np.uint32), b.d2.view(np.uint32)))}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
 
 
 def _views(hkg, dQ, dk, step: int, B: int, L: int):
@@ -340,6 +374,11 @@ def main() -> int:
                 "identical_to_full_scan": same, "checked": int(args.real_queries)}
         log(f"real queries: {real}")
 
+    loader = {}
+    if args.loader_series:
+        loader = loader_leg(hkg, args, L, Qh)
+        log(f"loader: {loader}")
+
     serving = None
     if args.serve_requests:
         sq = args.serve_qps if args.serve_qps > 0 else 0.5 * qps
@@ -383,6 +422,7 @@ def main() -> int:
         "parity": par,
         "modes": modes,
         "real_queries": real,
+        "loader": loader,
         "serving": serving,
         "cpu_baseline": cpu,
     }
