@@ -1466,40 +1466,58 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
     const float kth0 = s_kth0;
     int br = s_branch;
     if (br == 3) {
-      // rows of the candidate leaves whose iSAX bound is within kth0 (:2171-2205)
+      // rows of the candidate leaves whose iSAX bound is within kth0
+      // (:2171-2205).  The branch test is monotone in the count, so the
+      // count stops once it is decided: the count so far already flips it,
+      // or even every row still to count could not.
       const float ratio = (float)(L / kPaaSegments);
-      unsigned long long mine = 0;
+      const double nser = (double)(float)mp.nseries;
+      __shared__ int s_dec;
+      if (threadIdx.x == 0) s_dec = 0;
+      unsigned long long seen = 0;  // candidate rows passed so far (every lane keeps the same)
       for (uint32_t l0 = 0; l0 < nleaf; l0 += kBlock) {
         if (threadIdx.x == 0) s_nlist = 0;
         __syncthreads();
+        if (s_dec) break;
         const uint32_t l = l0 + threadIdx.x;
-        if (l < nleaf && qkeys[l] != 0xFFFFFFFFu && qnlb[leaf_node[l]] <= kth0) s_list[atomicAdd(&s_nlist, 1)] = l;
+        if (l < nleaf && qkeys[l] != 0xFFFFFFFFu) s_list[atomicAdd(&s_nlist, 1)] = l;
         __syncthreads();
         const int nl = s_nlist;
         for (int j = 0; j < nl; ++j) {
           const uint32_t lf = s_list[j];
           const uint64_t pos = leaf_pos[lf];
           const uint32_t size = leaf_size[lf];
-          // four rows per lane in flight
-          for (uint32_t r = threadIdx.x; r < size; r += 4 * kBlock) {
-            uint4 w[4];
+          unsigned long long mine = 0;
+          if (qnlb[leaf_node[lf]] <= kth0) {
+            // eight rows per lane in flight
+            for (uint32_t r = threadIdx.x; r < size; r += 8 * kBlock) {
+              uint4 w[8];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const uint32_t rr = r + u * kBlock;
-              w[u] = rr < size ? *reinterpret_cast<const uint4 *>(sax + (pos + rr) * kPaaSegments) : make_uint4(0, 0, 0, 0);
+              for (int u = 0; u < 8; ++u) {
+                const uint32_t rr = r + u * kBlock;
+                w[u] = rr < size ? *reinterpret_cast<const uint4 *>(sax + (pos + rr) * kPaaSegments)
+                                 : make_uint4(0, 0, 0, 0);
+              }
+#pragma unroll
+              for (int u = 0; u < 8; ++u)
+                if (r + u * kBlock < size) mine += lb_sax_word(spaa, w[u], bp, ratio) <= kth0 ? 1ull : 0ull;
             }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-              if (r + u * kBlock < size) mine += lb_sax_word(spaa, w[u], bp, ratio) <= kth0 ? 1ull : 0ull;
           }
+          seen += size;
+          if (mine) atomicAdd(&s_lab, mine);
+          __syncthreads();
+          if (threadIdx.x == 0) {
+            const unsigned long long lab = s_lab;
+            if ((float)(1.0 - (double)lab / nser) < mp.sax_th) s_dec = 1;
+            else if (!((float)(1.0 - (double)(lab + (s_crows - seen)) / nser) < mp.sax_th)) s_dec = 2;
+          }
+          __syncthreads();
+          if (s_dec) break;
         }
-        __syncthreads();
       }
-      atomicAdd(&s_lab, mine);
       __syncthreads();
       if (threadIdx.x == 0) {
-        const float slp = (float)(1.0 - (double)s_lab / (double)(float)mp.nseries);
-        s_branch = slp < mp.sax_th ? 1 : 2;
+        s_branch = s_dec ? s_dec : ((float)(1.0 - (double)s_lab / nser) < mp.sax_th ? 1 : 2);
       }
       __syncthreads();
       br = s_branch;
