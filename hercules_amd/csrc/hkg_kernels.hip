@@ -1473,37 +1473,47 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
       const float ratio = (float)(L / kPaaSegments);
       const double nser = (double)(float)mp.nseries;
       __shared__ int s_dec;
+      __shared__ uint64_t s_lpos[kBlock];
+      __shared__ uint32_t s_lsize[kBlock];
+      __shared__ uint8_t s_lok[kBlock];
       if (threadIdx.x == 0) s_dec = 0;
+      const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
       unsigned long long seen = 0;  // candidate rows passed so far (every lane keeps the same)
       for (uint32_t l0 = 0; l0 < nleaf; l0 += kBlock) {
         if (threadIdx.x == 0) s_nlist = 0;
         __syncthreads();
         if (s_dec) break;
         const uint32_t l = l0 + threadIdx.x;
-        if (l < nleaf && qkeys[l] != 0xFFFFFFFFu) s_list[atomicAdd(&s_nlist, 1)] = l;
+        if (l < nleaf && qkeys[l] != 0xFFFFFFFFu) {
+          const int j = atomicAdd(&s_nlist, 1);
+          s_lpos[j] = leaf_pos[l];
+          s_lsize[j] = leaf_size[l];
+          s_lok[j] = qnlb[leaf_node[l]] <= kth0 ? 1 : 0;
+        }
         __syncthreads();
         const int nl = s_nlist;
-        for (int j = 0; j < nl; ++j) {
-          const uint32_t lf = s_list[j];
-          const uint64_t pos = leaf_pos[lf];
-          const uint32_t size = leaf_size[lf];
+        // sixteen leaves per round, one wave per leaf at a time
+        for (int jb = 0; jb < nl; jb += 16) {
+          const int je = jb + 16 < nl ? jb + 16 : nl;
           unsigned long long mine = 0;
-          if (qnlb[leaf_node[lf]] <= kth0) {
-            // eight rows per lane in flight
-            for (uint32_t r = threadIdx.x; r < size; r += 8 * kBlock) {
+          for (int j = jb + wave; j < je; j += kBlock / 64) {
+            if (!s_lok[j]) continue;
+            const uint64_t pos = s_lpos[j];
+            const uint32_t size = s_lsize[j];
+            for (uint32_t r = lane; r < size; r += 8 * 64) {
               uint4 w[8];
 #pragma unroll
               for (int u = 0; u < 8; ++u) {
-                const uint32_t rr = r + u * kBlock;
+                const uint32_t rr = r + u * 64;
                 w[u] = rr < size ? *reinterpret_cast<const uint4 *>(sax + (pos + rr) * kPaaSegments)
                                  : make_uint4(0, 0, 0, 0);
               }
 #pragma unroll
               for (int u = 0; u < 8; ++u)
-                if (r + u * kBlock < size) mine += lb_sax_word(spaa, w[u], bp, ratio) <= kth0 ? 1ull : 0ull;
+                if (r + u * 64 < size) mine += lb_sax_word(spaa, w[u], bp, ratio) <= kth0 ? 1ull : 0ull;
             }
           }
-          seen += size;
+          for (int j = jb; j < je; ++j) seen += s_lsize[j];
           if (mine) atomicAdd(&s_lab, mine);
           __syncthreads();
           if (threadIdx.x == 0) {

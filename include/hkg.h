@@ -97,7 +97,9 @@ typedef struct hkg_query_opts {
   uint32_t nprobes;        /* > 0: no-guarantee search that scans nprobes
                               leaves (exact_ng_knn_search); epsilon ignored  */
   uint32_t approx_leaves;  /* epsilon search: leaf budget of its approximate
-                              phase (--approx-stop-condition; default 80)    */
+                              phase (--approx-stop-condition; default 80;
+                              the first leaf is always scanned, so 0 acts
+                              as 1, as in the reference)                     */
   float eapca_threshold;   /* epsilon search: candidate-branch thresholds    */
   float sax_threshold;     /* (--eapca-threshold 0.25, --sax-threshold 0.9)  */
   uint32_t reserved[2];
