@@ -142,6 +142,33 @@ def test_modes_on_device_built_index(hkg, oracle, tmp_path):
         ix.close()
 
 
+def test_modes_edge_settings(hkg, oracle, tmp_path):
+    """Short series (L = 64), k = 256, more probes and a larger budget than
+    the tree has leaves, a tiny epsilon, duplicated rows (ties)."""
+    n, L = 6000, 64
+    X = oracle.series(1, 51, 0, n, L)
+    X[100:110] = X[5]  # ten copies of one row: equal distances, smallest id wins
+    Q = np.ascontiguousarray(np.concatenate([X[5:6], oracle.series(1, 52, 0, 11, L)]))
+    dX = hkg.DeviceArray(X.nbytes)
+    dX.upload(X)
+    d = tmp_path / "edge"
+    with hkg.HerculesIndex.build_device(dX, n, L, leaf_size=150) as built:
+        built.write(str(d))
+    tabs = node_tables(parse_root_idx(str(d / "root.idx")))
+    raw = np.fromfile(d / "leaves_raw.idx", np.float32).reshape(n, L)
+    sims = np.fromfile(d / "leaves_sims.idx", np.uint8).reshape(n, 16)
+    nleaf = int((tabs["nodes"][:, 0] == 1).sum())
+    with hkg.HerculesIndex.open(str(d)) as ix:
+        for c in [dict(k=256, nprobes=nleaf + 50), dict(k=256, epsilon=1e-6, budget=100000, eapca=0.25, sax=0.9),
+                  dict(k=1, epsilon=0.3, budget=0, eapca=0.25, sax=0.9), dict(k=33, nprobes=1)]:
+            print(f"modes edge {c}", flush=True)
+            r = ix.knn(Q, c["k"], **_mode_kw(c))
+            ids, d2, info = oracle.knn_mode(tabs, raw, sims, Q, c["k"], **_oracle_kw(c))
+            assert np.array_equal(r.d2.view(np.uint32), d2.view(np.uint32)), c
+            assert np.array_equal(r.ids, ids), c
+            _check_stats(r.stats, info, c)
+
+
 def test_modes_reject_bad_settings(hkg, oracle):
     X = oracle.series(0, 5, 0, 2000, 64)
     with hkg.HerculesIndex.open_raw(X) as ix:
