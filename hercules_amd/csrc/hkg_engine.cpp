@@ -482,6 +482,19 @@ CallSettings &call_settings()
 // ---------------------------------------------------------------------------
 // one batch through a path (device pointers; results left in b->out_*)
 // ---------------------------------------------------------------------------
+// persistent grid of k_filter_refine: as many blocks as fit the CUs at once
+static uint32_t filter_grid()
+{
+  static uint32_t g = 0;
+  if (!g) {
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    g = (uint32_t)std::max(cus, 1) * kFilterBlocksPerCU;
+  }
+  return g;
+}
+
 static uint32_t persistent_grid()
 {
   static uint32_t g = 0;
@@ -516,7 +529,7 @@ void Index::enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Pr
   launch_collect(st, nq, nleaf, b.lb, b.best, b.bound, b.work_keys, b.work_vals, nwork);
   (void)launch_sort_work(st, b.sort_tmp, b.sort_bytes, b.work_keys, keys2, b.work_vals, vals2, pairs);
   if (pf) pf->mark(st, HKG_STAGE_COLLECT);
-  launch_filter_refine(st, persistent_grid(), dev.X, dev.sax, L, b.Q, b.paa, b.kq, dev.leaf_pos, dev.leaf_size,
+  launch_filter_refine(st, filter_grid(), dev.X, dev.sax, L, b.Q, b.paa, b.kq, dev.leaf_pos, dev.leaf_size,
                        dev.ids, nleaf, b.lb, vals2, nwork, work_next, b.cap, b.cand, b.cand_cnt, b.bound,
                        b.counters);
   if (pf) pf->mark(st, HKG_STAGE_REFINE);
@@ -559,7 +572,7 @@ hkg_status Index::enqueue_mode_batch(QueryBuffers &b, uint32_t nq, hipStream_t s
     uint32_t *keys2 = b.work_keys + b.work_cap, *vals2 = b.work_vals + b.work_cap;
     HKG_HIP_TRY(launch_sort_work(st, b.sort_tmp, b.sort_bytes, b.work_keys, keys2, b.work_vals, vals2, pairs));
     if (pf) pf->mark(st, HKG_STAGE_COLLECT);
-    launch_filter_refine(st, persistent_grid(), dev.X, dev.sax, L, b.Q, b.paa, b.kq, dev.leaf_pos, dev.leaf_size,
+    launch_filter_refine(st, filter_grid(), dev.X, dev.sax, L, b.Q, b.paa, b.kq, dev.leaf_pos, dev.leaf_size,
                          dev.ids, nleaf, b.lb, vals2, nwork, work_next, b.cap, b.cand, b.cand_cnt, b.bound,
                          b.counters);
     if (pf) pf->mark(st, HKG_STAGE_REFINE);

@@ -193,19 +193,22 @@ __device__ void topk_merge(TopK &s, uint32_t *scan, int k)
 // in order (the reference's sequential accumulation).  Rows are staged as
 // candidates when d2 <= bound (d2 equal to the bound is kept for the
 // smallest-id rule).
-struct RefineShared {
-  float csum[kBlock / 64][kRefineTile];  // per wave
+template <int TILE>
+struct RefineSharedT {
+  float csum[kBlock / 64][TILE];  // per wave
 };
+using RefineShared = RefineSharedT<kRefineTile>;
 
+template <int TILE>
 __device__ void refine_rows(const float *__restrict__ X, uint32_t L, const float *qs,
                             const uint64_t *rows, int nrows, const uint32_t *__restrict__ ids,
-                            uint32_t bound_bits, TopK &tk, RefineShared &rs, uint32_t &nrefined)
+                            uint32_t bound_bits, TopK &tk, RefineSharedT<TILE> &rs, uint32_t &nrefined)
 {
   const int cpr = (int)(L >> 3);
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  // rows per wave group: the padded chunk tile must fit kRefineTile floats
-  int g = kRefineTile / (cpr + 1);
+  // rows per wave group: the padded chunk tile must fit TILE floats
+  int g = TILE / (cpr + 1);
   if (g > 64) g = 64;
   const int per_block = g * (kBlock / 64);
   float *cs = rs.csum[wave];
@@ -588,7 +591,7 @@ __global__ void __launch_bounds__(kBlock) k_filter_refine(
 {
   extern __shared__ float qs[];
   __shared__ TopK tk;
-  __shared__ RefineShared rs;
+  __shared__ RefineSharedT<kRefineTileFilter> rs;  // smaller: more blocks per CU
   __shared__ uint64_t rows[kFilterRows];
   __shared__ uint32_t scan[kBlock];
   __shared__ float bp[256];
