@@ -163,7 +163,8 @@ __device__ __forceinline__ void load_lds4(const void *g, lds_void *lds)
 }
 
 // Filter.  Block = W waves x 64 queries (two 32-query tiles per wave) of
-// block group g (grid.y), the queries qperm[64 W g ..]; it loops over the
+// block group g (grid.y), wave w taking the queries qperm[64 (w G + g) ..]
+// (G groups); it loops over the
 // row tiles of the batch's tile list (tiles overlapping a leaf whose EAPCA
 // bound is within the k-th distance bound of one of the batch's queries).
 // Tiles of kRowTile series go through a ring of kBuf LDS buffers, loaded
@@ -207,7 +208,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   // vector position, so 16 distinct positions keep them conflict-free
   constexpr int SW = (VPR % 16 == 0) ? 15 : (VPR % 8 == 0) ? 7 : 3;
   constexpr int NCH = kRowTile * VPR / 64;         // 1 KB chunks per tile
-  constexpr int kWaves = W, kQPerBlock = W * kQPerWave;
+  constexpr int kWaves = W;
   // vector memory ops every wave issues per tile, at least: its share of
   // the series chunks, and with four waves one |x|^2 or slack-term load each
   constexpr int kWaitOps = NCH / W + (W == 4 ? 1 : 0);
@@ -242,7 +243,11 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   bf16x8 qop[2][KS];
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
-    const uint32_t slot = g * kQPerBlock + wave * kQPerWave + t * 32 + l32;
+    // the batch is ordered by best leaf, so a wave holds neighbouring
+    // queries (their survivors fall in the same tiles); waves are dealt to
+    // the block groups round-robin, so each group gets a share of every
+    // region and the groups' survivor loads even out
+    const uint32_t slot = (wave * gridDim.y + g) * kQPerWave + t * 32 + l32;
     const uint32_t q = slot < nq ? qperm[slot] : 0xFFFFFFFFu;
     qid[t] = q;
     const uint32_t qs = q < nq ? q : 0;
