@@ -58,6 +58,8 @@ def parse() -> argparse.Namespace:
                    help="epsilon of the approximate-mode leg (0: skip that leg)")
     p.add_argument("--mode-nprobes", type=int, default=10, help="nprobes of the ng-mode leg (0: skip that leg)")
     p.add_argument("--mode-queries", type=int, default=256, help="queries of each mode leg (from the first step)")
+    p.add_argument("--mode-cpu-queries", type=int, default=32,
+                   help="modes beside the reference CLI on a 1M-series index: queries (0: skip)")
     p.add_argument("--real-queries", type=int, default=16,
                    help="real-data leg: Seismic query files on the index; this many checked against the full "
                         "scan (0: skip the leg)")
@@ -99,6 +101,58 @@ def cpu_baseline(args, Q: np.ndarray, kq: np.ndarray) -> dict:
             "sample": f"reference PSCAN ({threads - 1} scan workers + 1 reader) over the first {m} rows of the "
                       f"same dataset, {nq} of the benchmark's queries (k={k}), {wall:.2f} s; queries/s scaled "
                       f"by {m}/{args.n}"}
+
+
+def mode_cpu_leg(hkg, args, Q: np.ndarray) -> dict:
+    """The epsilon and nprobes modes beside the reference CLI on the same
+    index: a 1M-series index (the first 1M rows of the dataset, the
+    reference's CPU-sized case) built on the device and written in the
+    reference's format; the reference (compiled from its sources into
+    oracle/_ref) and the device answer the same queries with the same
+    settings, and the answers are compared bit for bit."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    try:
+        import ref_runner as rr
+    except Exception as e:  # pragma: no cover
+        return {"error": f"reference runner unavailable: {e}"}
+    if not rr.available():
+        return {"error": "reference binaries not built (oracle/_ref)"}
+    import shutil
+    threads = max(2, min(16, os.cpu_count() or 2))
+    n, L, nq, k = min(1_000_000, args.n), args.length, args.mode_cpu_queries, 10
+    out = {"index": f"first {n} rows, leaf {args.leaf}, built on the device", "queries": nq, "k": k,
+           "reference_threads": threads}
+    d = tempfile.mkdtemp(prefix="hkg_mode_cpu_", dir=os.environ.get("TMPDIR", "/tmp"))
+    try:
+        dX = hkg.DeviceArray(n * L * 4)
+        hkg.gen_series_device(0, 1, 0, n, L, dX)
+        with hkg.HerculesIndex.build_device(dX, n, L, leaf_size=args.leaf) as built:
+            built.write(os.path.join(d, "idx"))
+        rr.write_f32(os.path.join(d, "q.bin"), Q[:nq])
+        with hkg.HerculesIndex.open(os.path.join(d, "idx")) as ix:
+            legs = ([("epsilon", {"epsilon": args.mode_epsilon},
+                      ["--epsilon", args.mode_epsilon, "--approx-stop-condition", 80])] if args.mode_epsilon > 0 else []) + \
+                   ([("ng", {"nprobes": args.mode_nprobes}, ["--nprobes", args.mode_nprobes])]
+                    if args.mode_nprobes > 0 else [])
+            for name, kw, flags in legs:
+                ix.knn(Q[:2], k, **kw)  # warm the shape
+                hkg.lib().hkg_device_sync()
+                t = time.perf_counter()
+                r = ix.knn(Q[:nq], k, batch=nq, **kw)
+                gpu_s = time.perf_counter() - t
+                rd2, _, wall, _ = rr.query(os.path.join(d, "idx"), os.path.join(d, "q.bin"), nq, L, k,
+                                           threads=threads, sims=True, extra=flags)
+                # the same run with one query: its wall time is the CLI's start-up
+                # and index load, which the per-query rate leaves out
+                _, _, wall1, _ = rr.query(os.path.join(d, "idx"), os.path.join(d, "q.bin"), 1, L, k,
+                                          threads=threads, sims=True, extra=flags)
+                per = max(wall - wall1, 1e-9) / (nq - 1) if nq > 1 else wall
+                out[name] = {"settings": kw, "gpu_queries_per_s": nq / gpu_s, "reference_queries_per_s": 1.0 / per,
+                             "reference_wall_s": wall, "reference_wall_1_query_s": wall1,
+                             "identical_to_reference": bool(np.array_equal(r.d2.view(np.uint32), rd2.view(np.uint32)))}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return out
 
 
 def traffic_bytes(kernel: str):
@@ -352,6 +406,10 @@ def main() -> int:
     # real-data queries (the reference's Seismic query files, len 256) on the
     # same index: throughput of one batch, and the answers of a few against
     # the full scan
+    if legs and args.mode_cpu_queries and not args.no_cpu_baseline:
+        modes["beside_reference_1M"] = mode_cpu_leg(hkg, args, Qh)
+        log(f"modes beside the reference: {modes['beside_reference_1M']}")
+
     real = {}
     if args.real_queries and L == 256:
         sys.path.insert(0, os.path.join(REPO, "tests"))
