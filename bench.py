@@ -104,12 +104,12 @@ def cpu_baseline(args, Q: np.ndarray, kq: np.ndarray) -> dict:
 
 
 def mode_cpu_leg(hkg, args, Q: np.ndarray) -> dict:
-    """The epsilon and nprobes modes beside the reference CLI on the same
-    index: a 1M-series index (the first 1M rows of the dataset, the
-    reference's CPU-sized case) built on the device and written in the
-    reference's format; the reference (compiled from its sources into
-    oracle/_ref) and the device answer the same queries with the same
-    settings, and the answers are compared bit for bit."""
+    """The exact search and the epsilon and nprobes modes beside the
+    reference CLI on the same index: a 1M-series index (the first 1M rows of
+    the dataset, the reference's CPU-sized case) built on the device and
+    written in the reference's format; the reference (compiled from its
+    sources into oracle/_ref) and the device answer the same queries with
+    the same settings, and the answers are compared bit for bit."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     try:
         import ref_runner as rr
@@ -134,8 +134,11 @@ def mode_cpu_leg(hkg, args, Q: np.ndarray) -> dict:
                       ["--epsilon", args.mode_epsilon, "--approx-stop-condition", 80])] if args.mode_epsilon > 0 else []) + \
                    ([("ng", {"nprobes": args.mode_nprobes}, ["--nprobes", args.mode_nprobes])]
                     if args.mode_nprobes > 0 else [])
+            # the exact search too: the reference's SIMS path against the
+            # batched index path
+            legs = [("exact", {"path": "batched"}, [])] + legs
             for name, kw, flags in legs:
-                ix.knn(Q[:2], k, **kw)  # warm the shape
+                ix.knn(Q[:2], k, **kw)  # warm the code path
                 hkg.lib().hkg_device_sync()
                 t = time.perf_counter()
                 r = ix.knn(Q[:nq], k, batch=nq, **kw)
