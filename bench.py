@@ -268,7 +268,7 @@ def main() -> int:
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(how the path would shard is described in DESIGN.md)", file=sys.stderr)
+              "(the index would shard by leaf, one contiguous range of whole leaves per GPU)", file=sys.stderr)
         return 2
     from hercules_amd import hkg
 
