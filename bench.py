@@ -400,7 +400,10 @@ def main() -> int:
                    "leaves_scanned_first_per_query": st["mode_leaves_first"] / M,
                    "candidate_leaves_per_query": st["mode_candidates"] / M,
                    "sequential_branch_queries": st["mode_sequential"],
-                   "stages_ms": {kk: round(v, 3) for kk, v in st["stage_ms"].items() if v > 0}}
+                   "stages_ms": {kk: round(v, 3) for kk, v in st["stage_ms"].items() if v > 0},
+                   # k_mode_search: the rows of every leaf it scans, 4 L bytes each
+                   "mode_search_algorithmic_gbs": (st["stage_bytes"]["approx"] / (st["stage_ms"]["approx"] * 1e6)
+                                                   if st["stage_ms"]["approx"] > 0 else None)}
             if name == "epsilon":
                 leg["within_1_plus_epsilon"] = bool(ratio.max() <= (1.0 + kw["epsilon"]) * (1 + 1e-6))
             modes[name] = leg
