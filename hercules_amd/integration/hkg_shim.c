@@ -1,0 +1,68 @@
+/* hkg_shim.c -- exact_de_knn_search_psq_lf_sims_adaptive on the MI355X engine.
+ *
+ * Compiled with the reference's headers and linked into the reference's own
+ * main (hercules.c) in place of its definition of this function; every other
+ * reference function stays as it is.  The index directory is the one the
+ * reference loaded (index->settings->root_directory, written with --serial 1
+ * --sims); the engine uploads it to HBM once, on the first query.
+ * Replaces hercules_query_engine.c:706-1083 (declared in
+ * include/hercules_query_engine.h:284-289). */
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include "hercules_query_engine.h"   /* reference: query_result, query_settings, stats calls */
+#include "hkg.h"
+
+static hkg_index *g_hkg = NULL;
+
+static hkg_index *hkg_for(struct hercules_index *index)
+{
+  if (!g_hkg) {
+    hkg_options o;
+    hkg_default_options(&o);
+    if (hkg_open_dir(index->settings->root_directory, &o, &g_hkg) != HKG_OK) {
+      fprintf(stderr, "hkg: %s\n", hkg_last_error());
+      exit(-1);                      /* the reference's convention for fatal errors */
+    }
+  }
+  return g_hkg;
+}
+
+void exact_de_knn_search_psq_lf_sims_adaptive(ts_type *query_ts, ts_type *query_ts_reordered, int *query_order,
+                                              unsigned int offset, ts_type *query_paa,
+                                              struct hercules_index *index, ts_type minimum_distance,
+                                              ts_type epsilon, double delta, unsigned int k, unsigned int q_id,
+                                              char *qfilename, query_settings q_settings)
+{
+  (void)query_ts_reordered; (void)query_order; (void)offset; (void)query_paa; (void)minimum_distance;
+  uint64_t *ids = malloc(sizeof(uint64_t) * k);
+  float *d2 = malloc(sizeof(float) * k);
+  hkg_query_opts qo;
+  hkg_default_query_opts(&qo);
+  qo.path = HKG_PATH_INDEX;          /* one query: the per-query index path */
+  qo.epsilon = epsilon;              /* 0: exact; > 0: the reference's epsilon search */
+  qo.delta = (float)delta;           /* accepted; the reference ignores it too */
+  qo.approx_leaves = (uint32_t)q_settings.approx_stop_condition;
+  qo.eapca_threshold = q_settings.eapca_threshold;
+  qo.sax_threshold = q_settings.sax_threshold;
+  if (hkg_knn(hkg_for(index), query_ts, 1, NULL, k, &qo, ids, d2, NULL) != HKG_OK) {
+    fprintf(stderr, "hkg: %s\n", hkg_last_error());
+    exit(-1);
+  }
+  /* report rank by rank as the reference does (hercules_query_engine.c:1062-1073); the reference's
+   * record prints sqrtf(distance), so pass the squared distance as it keeps it */
+  for (unsigned int pos = 0; pos < k; ++pos) {
+    struct query_result r;
+    r.distance = d2[pos];            /* FLT_MAX for ranks past the number of distinct distances */
+    r.node = NULL;
+    r.max_distance = FLT_MAX;
+    r.pqueue_position = 0;
+    update_query_stats(index, q_id, pos + 1, r);
+    get_query_stats(index, pos + 1);
+    print_query_stats(index, q_id, pos + 1, qfilename);
+  }
+  fflush(stdout);
+  free(ids);
+  free(d2);
+}

@@ -48,15 +48,19 @@ _D2 = re.compile(r"^Query_exact_d2\t(\S+)\t(\d+)\t(\d+)$", re.M)
 _DIST = re.compile(r"^Query_exact_distance\t(\S+)\t\S+\t(\d+)\t(\d+)$", re.M)
 
 
+DROPIN = os.path.join(REF_DIR, "hercules_gpu")  # the reference's main with the engine's shim
+
+
 def query(index_dir: str, queries_path: str, nq: int, L: int, k: int, threads: int = 8, sims: bool = True,
-          sax_threshold: float = 0.9, eapca_threshold: float = 0.25, timeout: float = 900, extra=()):
+          sax_threshold: float = 0.9, eapca_threshold: float = 0.25, timeout: float = 900, extra=(),
+          binary: str = HERCULES):
     """Run the reference query path; returns (d2 (nq,k) float32 from the
     exact hex print, printed sqrt distances (nq,k), wall seconds, stdout).
     sims=False with threads=1 runs the serial path exact_de_knn_search.
     extra: further options of the table (--epsilon, --nprobes,
     --approx-stop-condition)."""
     index_dir = os.path.abspath(index_dir) + "/"
-    cmd = [HERCULES, "--mode", "1", "--index-path", index_dir, "--queries", os.path.abspath(queries_path),
+    cmd = [binary, "--mode", "1", "--index-path", index_dir, "--queries", os.path.abspath(queries_path),
            "--queries-size", str(nq), "--timeseries-size", str(L), "--k", str(k), "--serial", "1",
            "--num-threads", str(threads), "--num-read-threads", "1",
            "--eapca-threshold", str(eapca_threshold), "--sax-threshold", str(sax_threshold)]

@@ -1,0 +1,38 @@
+"""The drop-in on an MI355X: the reference's own main (hercules.c), linked
+with hercules_amd/integration/hkg_shim.c in place of its
+exact_de_knn_search_psq_lf_sims_adaptive, reads the reference-built tiny
+index, answers the query file through libhkg, and prints the reference's
+record stream -- with the reference CLI's answers, exact and epsilon."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+import golden_data as gd
+import ref_runner as rr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.skipif(not os.path.exists(rr.DROPIN), reason="oracle/_ref/hercules_gpu not built (make -C oracle dropin)")
+def test_reference_main_through_the_shim(oracle, tmp_path):
+    d = tmp_path / "idx"
+    gd.reference_index(oracle, "tiny", str(d))
+    m = gd.meta("tiny")
+    Q = gd.queries(oracle, m)
+    qf = str(tmp_path / "q.bin")
+    rr.write_f32(qf, Q)
+    a = gd.answers("tiny")
+    print("drop-in: exact, k = 10", flush=True)
+    d2, dist, _, out = rr.query(str(d), qf, Q.shape[0], m["L"], 10, threads=4, binary=rr.DROPIN, timeout=300)
+    assert np.array_equal(d2.view(np.uint32), a["ref_sims_d2_k10"].view(np.uint32))
+    # the printed records: sqrtf(d2) with six decimals
+    assert np.allclose(dist, np.sqrt(a["ref_sims_d2_k10"].astype(np.float64)), rtol=1e-6, atol=1e-6)
+    c = gd.mode_configs("tiny")[0]
+    print(f"drop-in: {c}", flush=True)
+    d2, _, _, _ = rr.query(str(d), qf, Q.shape[0], m["L"], c["k"], threads=4, binary=rr.DROPIN, timeout=300,
+                           eapca_threshold=c["eapca"], sax_threshold=c["sax"],
+                           extra=["--epsilon", c["epsilon"], "--approx-stop-condition", c["budget"]])
+    assert np.array_equal(d2.view(np.uint32), gd.mode_answers("tiny")["m0"].view(np.uint32))
