@@ -1,12 +1,13 @@
-/* hkg_shim.c -- exact_de_knn_search_psq_lf_sims_adaptive on the MI355X engine.
+/* hkg_shim.c -- the reference's query functions on the MI355X engine:
+ * exact_de_knn_search_psq_lf_sims_adaptive (exact and epsilon search) and
+ * exact_ng_knn_search (nprobes).
  *
  * Compiled with the reference's headers and linked into the reference's own
- * main (hercules.c) in place of its definition of this function; every other
- * reference function stays as it is.  The index directory is the one the
- * reference loaded (index->settings->root_directory, written with --serial 1
- * --sims); the engine uploads it to HBM once, on the first query.
- * Replaces hercules_query_engine.c:706-1083 (declared in
- * include/hercules_query_engine.h:284-289). */
+ * main (hercules.c) in place of the library's definitions of these two
+ * functions; every other reference function stays as it is.  The index
+ * directory is the one the reference loaded (index->settings->root_directory,
+ * written with --serial 1 --sims); the engine uploads it to HBM once, on the
+ * first query. */
 #include <float.h>
 #include <math.h>
 #include <stdio.h>
@@ -29,29 +30,18 @@ static hkg_index *hkg_for(struct hercules_index *index)
   return g_hkg;
 }
 
-void exact_de_knn_search_psq_lf_sims_adaptive(ts_type *query_ts, ts_type *query_ts_reordered, int *query_order,
-                                              unsigned int offset, ts_type *query_paa,
-                                              struct hercules_index *index, ts_type minimum_distance,
-                                              ts_type epsilon, double delta, unsigned int k, unsigned int q_id,
-                                              char *qfilename, query_settings q_settings)
+/* one query through the engine, then the reference's records rank by rank
+ * (hercules_query_engine.c:1062-1073, :2786-2797); the records print
+ * sqrtf(distance), so the squared distance goes in as the reference keeps it */
+static void hkg_answer(struct hercules_index *index, ts_type *query_ts, unsigned int k, unsigned int q_id,
+                       char *qfilename, const hkg_query_opts *qo)
 {
-  (void)query_ts_reordered; (void)query_order; (void)offset; (void)query_paa; (void)minimum_distance;
   uint64_t *ids = malloc(sizeof(uint64_t) * k);
   float *d2 = malloc(sizeof(float) * k);
-  hkg_query_opts qo;
-  hkg_default_query_opts(&qo);
-  qo.path = HKG_PATH_INDEX;          /* one query: the per-query index path */
-  qo.epsilon = epsilon;              /* 0: exact; > 0: the reference's epsilon search */
-  qo.delta = (float)delta;           /* accepted; the reference ignores it too */
-  qo.approx_leaves = (uint32_t)q_settings.approx_stop_condition;
-  qo.eapca_threshold = q_settings.eapca_threshold;
-  qo.sax_threshold = q_settings.sax_threshold;
-  if (hkg_knn(hkg_for(index), query_ts, 1, NULL, k, &qo, ids, d2, NULL) != HKG_OK) {
+  if (hkg_knn(hkg_for(index), query_ts, 1, NULL, k, qo, ids, d2, NULL) != HKG_OK) {
     fprintf(stderr, "hkg: %s\n", hkg_last_error());
     exit(-1);
   }
-  /* report rank by rank as the reference does (hercules_query_engine.c:1062-1073); the reference's
-   * record prints sqrtf(distance), so pass the squared distance as it keeps it */
   for (unsigned int pos = 0; pos < k; ++pos) {
     struct query_result r;
     r.distance = d2[pos];            /* FLT_MAX for ranks past the number of distinct distances */
@@ -65,4 +55,37 @@ void exact_de_knn_search_psq_lf_sims_adaptive(ts_type *query_ts, ts_type *query_
   fflush(stdout);
   free(ids);
   free(d2);
+}
+
+/* hercules_query_engine.c:706-1083 (include/hercules_query_engine.h:284-289):
+   exact search, or the epsilon search when epsilon > 0 */
+void exact_de_knn_search_psq_lf_sims_adaptive(ts_type *query_ts, ts_type *query_ts_reordered, int *query_order,
+                                              unsigned int offset, ts_type *query_paa,
+                                              struct hercules_index *index, ts_type minimum_distance,
+                                              ts_type epsilon, double delta, unsigned int k, unsigned int q_id,
+                                              char *qfilename, query_settings q_settings)
+{
+  (void)query_ts_reordered; (void)query_order; (void)offset; (void)query_paa; (void)minimum_distance;
+  hkg_query_opts qo;
+  hkg_default_query_opts(&qo);
+  qo.path = HKG_PATH_INDEX;          /* one query: the per-query index path */
+  qo.epsilon = epsilon;              /* 0: exact; > 0: the reference's epsilon search */
+  qo.delta = (float)delta;           /* accepted; the reference ignores it too */
+  qo.approx_leaves = (uint32_t)q_settings.approx_stop_condition;
+  qo.eapca_threshold = q_settings.eapca_threshold;
+  qo.sax_threshold = q_settings.sax_threshold;
+  hkg_answer(index, query_ts, k, q_id, qfilename, &qo);
+}
+
+/* hercules_query_engine.c:2623-2799 (include/hercules_query_engine.h:209-212):
+   the no-guarantee search over nprobes leaves */
+void exact_ng_knn_search(ts_type *query_ts, ts_type *query_ts_reordered, int *query_order, unsigned int offset,
+                         struct hercules_index *index, ts_type minimum_distance, unsigned int k, unsigned int q_id,
+                         char *qfilename, unsigned int nprobes, query_settings q_settings)
+{
+  (void)query_ts_reordered; (void)query_order; (void)offset; (void)minimum_distance; (void)q_settings;
+  hkg_query_opts qo;
+  hkg_default_query_opts(&qo);
+  qo.nprobes = nprobes;
+  hkg_answer(index, query_ts, k, q_id, qfilename, &qo);
 }

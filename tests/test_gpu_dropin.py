@@ -1,8 +1,9 @@
 """The drop-in on an MI355X: the reference's own main (hercules.c), linked
 with hercules_amd/integration/hkg_shim.c in place of its
-exact_de_knn_search_psq_lf_sims_adaptive, reads the reference-built tiny
-index, answers the query file through libhkg, and prints the reference's
-record stream -- with the reference CLI's answers, exact and epsilon."""
+exact_de_knn_search_psq_lf_sims_adaptive and exact_ng_knn_search, reads the
+reference-built tiny index, answers the query file through libhkg, and
+prints the reference's record stream -- with the reference CLI's answers,
+exact, epsilon and nprobes."""
 from __future__ import annotations
 
 import os
@@ -36,3 +37,8 @@ def test_reference_main_through_the_shim(oracle, tmp_path):
                            eapca_threshold=c["eapca"], sax_threshold=c["sax"],
                            extra=["--epsilon", c["epsilon"], "--approx-stop-condition", c["budget"]])
     assert np.array_equal(d2.view(np.uint32), gd.mode_answers("tiny")["m0"].view(np.uint32))
+    i, c = next((i, c) for i, c in enumerate(gd.mode_configs("tiny")) if c.get("nprobes") == 3)
+    print(f"drop-in: {c}", flush=True)
+    d2, _, _, _ = rr.query(str(d), qf, Q.shape[0], m["L"], c["k"], threads=4, binary=rr.DROPIN, timeout=300,
+                           extra=["--nprobes", c["nprobes"]])
+    assert np.array_equal(d2.view(np.uint32), gd.mode_answers("tiny")[f"m{i}"].view(np.uint32))
