@@ -383,6 +383,9 @@ QueryBuffers::~QueryBuffers()
   dfree(heap);
   dfree(bfs);
   dfree(info);
+  dfree(mstate);
+  dfree(vtop);
+  dfree(lab);
 }
 
 hkg_status Index::get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuffers **out)
@@ -549,6 +552,9 @@ hkg_status Index::enqueue_mode_batch(QueryBuffers &b, uint32_t nq, hipStream_t s
   if (!mode_heap_in_lds(L, nn) && !b.heap && (rc = dalloc(&b.heap, (uint64_t)b.B * (nn + 1), &b.bytes)) != HKG_OK)
     return rc;
   if (!b.bfs && (rc = dalloc(&b.bfs, (uint64_t)b.B * 2 * (nn + 1), &b.bytes)) != HKG_OK) return rc;
+  if (!b.mstate && (rc = dalloc(&b.mstate, (uint64_t)b.B * kModeState, &b.bytes)) != HKG_OK) return rc;
+  if (!b.vtop && (rc = dalloc(&b.vtop, (uint64_t)b.B * b.kmax, &b.bytes)) != HKG_OK) return rc;
+  if (!b.lab && (rc = dalloc(&b.lab, (uint64_t)b.B, &b.bytes)) != HKG_OK) return rc;
   ModeParams mp = call_settings().mp;
   mp.nleaves = nleaf;
   mp.nseries = dev.n;
@@ -566,7 +572,15 @@ hkg_status Index::enqueue_mode_batch(QueryBuffers &b, uint32_t nq, hipStream_t s
   launch_mode_keys(st, pairs, b.work_keys, b.work_vals);
   launch_mode_search(st, nq, dev.X, dev.sax, L, b.Q, b.paa, b.kq, nn, dev.nd_link, dev.nd_split, b.nlb, nleaf,
                      dev.leaf_pos, dev.leaf_size, dev.leaf_node, dev.ids, b.lb, b.heap, b.bfs, mp, b.cap, b.cand,
-                     b.cand_cnt, b.bound, b.work_keys, nwork, b.info, b.counters);
+                     b.cand_cnt, b.bound, b.work_keys, nwork, b.info, b.counters, b.mstate, b.vtop, b.kmax);
+  if (mp.nprobes == 0) {
+    launch_fill(st, reinterpret_cast<uint32_t *>(b.lab), 2 * nq, 0);
+    launch_mode_count(st, nq, dev.sax, L, b.paa, nn, b.nlb, nleaf, dev.leaf_pos, dev.leaf_size, dev.leaf_node,
+                      b.work_keys, b.mstate, b.lab);
+    launch_mode_finish(st, nq, dev.X, L, b.Q, b.kq, nn, b.nlb, nleaf, dev.leaf_pos, dev.leaf_size, dev.leaf_node,
+                       dev.ids, mp, b.cap, b.cand, b.cand_cnt, b.bound, b.work_keys, nwork, b.info, b.counters,
+                       b.mstate, b.vtop, b.kmax, b.lab);
+  }
   if (pf) pf->mark(st, HKG_STAGE_APPROX);
   if (mp.nprobes == 0) {
     uint32_t *keys2 = b.work_keys + b.work_cap, *vals2 = b.work_vals + b.work_cap;

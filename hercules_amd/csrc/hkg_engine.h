@@ -105,6 +105,9 @@ struct QueryBuffers {
   uint64_t *heap = nullptr;     // B x (nnodes + 1) traversal heaps (when too big for LDS)
   uint32_t *bfs = nullptr;      // B x 2 (nnodes + 1) level lists of the candidate walk
   uint32_t *info = nullptr;     // B x 4 per-query traversal counts
+  uint32_t *mstate = nullptr;   // B x kModeState: queries k_mode_search hands to k_mode_count / k_mode_finish
+  uint64_t *vtop = nullptr;     // B x kmax: their answers after the approximate phase
+  unsigned long long *lab = nullptr;  // B: their iSAX counts
   bool busy = false;            // held by a query call (guarded by Index::mu_bufs)
   // captured stage sequence of the batched path
   hipGraphExec_t graph_exec = nullptr;

@@ -84,7 +84,20 @@ void launch_mode_search(hipStream_t st, uint32_t nq, const float *X, const uint8
                         const uint32_t *leaf_size, const int32_t *leaf_node, const uint32_t *ids, const float *lb,
                         uint64_t *heap_g, uint32_t *bfs, const ModeParams &mp, uint32_t cap, uint64_t *cand,
                         uint32_t *cand_cnt, uint32_t *bound, uint32_t *keys, uint32_t *nwork, uint32_t *info,
-                        unsigned long long *counters);
+                        unsigned long long *counters, uint32_t *mstate, uint64_t *vtop, uint32_t vcap);
+// the epsilon search's queries whose candidate branch needs the iSAX count
+// or is sequential: k_mode_search hands them over (kModeState words per
+// query, and its answer so far), k_mode_count counts, k_mode_finish ends them
+constexpr int kModeState = 8;
+void launch_mode_count(hipStream_t st, uint32_t nq, const uint8_t *sax, uint32_t L, const float *paa, uint32_t nnodes,
+                       const float *nlb, uint32_t nleaf, const uint64_t *leaf_pos, const uint32_t *leaf_size,
+                       const int32_t *leaf_node, const uint32_t *keys, const uint32_t *mstate, unsigned long long *lab);
+void launch_mode_finish(hipStream_t st, uint32_t nq, const float *X, uint32_t L, const float *Q, const uint32_t *kq,
+                        uint32_t nnodes, const float *nlb, uint32_t nleaf, const uint64_t *leaf_pos,
+                        const uint32_t *leaf_size, const int32_t *leaf_node, const uint32_t *ids, const ModeParams &mp,
+                        uint32_t cap, uint64_t *cand, uint32_t *cand_cnt, uint32_t *bound, uint32_t *keys,
+                        uint32_t *nwork, uint32_t *info, unsigned long long *counters, const uint32_t *mstate,
+                        const uint64_t *vtop, uint32_t vcap, const unsigned long long *lab);
 void launch_scan(hipStream_t st, uint32_t grid_x, const float *X, uint64_t n, uint32_t L, const float *Q,
                  uint32_t nq, const uint32_t *kq, const uint32_t *ids, uint32_t cap, uint64_t *cand,
                  uint32_t *cand_cnt, uint32_t *bound);

@@ -1300,7 +1300,8 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
     uint64_t *__restrict__ heap_g, int heap_in_lds, uint32_t *__restrict__ bfs, ModeParams mp, uint32_t cap,
     uint64_t *__restrict__ cand,
     uint32_t *__restrict__ cand_cnt, uint32_t *__restrict__ bound, uint32_t *__restrict__ keys,
-    uint32_t *__restrict__ nwork, uint32_t *__restrict__ info, unsigned long long *__restrict__ counters)
+    uint32_t *__restrict__ nwork, uint32_t *__restrict__ info, unsigned long long *__restrict__ counters,
+    uint32_t *__restrict__ mstate, uint64_t *__restrict__ vtop, uint32_t vcap)
 {
   extern __shared__ __align__(16) unsigned char mode_lds[];
   float *qs = reinterpret_cast<float *>(mode_lds);
@@ -1312,10 +1313,6 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
   __shared__ float bp[256];
   __shared__ float spaa[kPaaSegments];
   __shared__ int32_t s_node;
-  __shared__ int s_go;
-  __shared__ uint32_t s_list[kBlock];
-  __shared__ int s_nlist;
-  __shared__ unsigned long long s_lab;
   const int k = (int)kq[q];
   const bool ng = mp.nprobes > 0;
   const float *qnlb = nlb + (uint64_t)q * nnodes;
@@ -1409,7 +1406,6 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
       s_ncand = 0;
       s_nempty = 0;
       s_nnext = 0;
-      s_lab = 0;
       s_crows = 0;
     }
     __syncthreads();
@@ -1466,120 +1462,31 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
       s_branch = br;
     }
     __syncthreads();
-    const float kth0 = s_kth0;
-    int br = s_branch;
-    if (br == 3) {
-      // rows of the candidate leaves whose iSAX bound is within kth0
-      // (:2171-2205).  The branch test is monotone in the count, so the
-      // count stops once it is decided: the count so far already flips it,
-      // or even every row still to count could not.
-      const float ratio = (float)(L / kPaaSegments);
-      const double nser = (double)(float)mp.nseries;
-      __shared__ int s_dec;
-      __shared__ uint64_t s_lpos[kBlock];
-      __shared__ uint32_t s_lsize[kBlock];
-      __shared__ uint8_t s_lok[kBlock];
-      if (threadIdx.x == 0) s_dec = 0;
-      const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-      unsigned long long seen = 0;  // candidate rows passed so far (every lane keeps the same)
-      for (uint32_t l0 = 0; l0 < nleaf; l0 += kBlock) {
-        if (threadIdx.x == 0) s_nlist = 0;
-        __syncthreads();
-        if (s_dec) break;
-        const uint32_t l = l0 + threadIdx.x;
-        if (l < nleaf && qkeys[l] != 0xFFFFFFFFu) {
-          const int j = atomicAdd(&s_nlist, 1);
-          s_lpos[j] = leaf_pos[l];
-          s_lsize[j] = leaf_size[l];
-          s_lok[j] = qnlb[leaf_node[l]] <= kth0 ? 1 : 0;
-        }
-        __syncthreads();
-        const int nl = s_nlist;
-        // sixteen leaves per round, one wave per leaf at a time
-        for (int jb = 0; jb < nl; jb += 16) {
-          const int je = jb + 16 < nl ? jb + 16 : nl;
-          unsigned long long mine = 0;
-          for (int j = jb + wave; j < je; j += kBlock / 64) {
-            if (!s_lok[j]) continue;
-            const uint64_t pos = s_lpos[j];
-            const uint32_t size = s_lsize[j];
-            for (uint32_t r = lane; r < size; r += 8 * 64) {
-              uint4 w[8];
-#pragma unroll
-              for (int u = 0; u < 8; ++u) {
-                const uint32_t rr = r + u * 64;
-                w[u] = rr < size ? *reinterpret_cast<const uint4 *>(sax + (pos + rr) * kPaaSegments)
-                                 : make_uint4(0, 0, 0, 0);
-              }
-#pragma unroll
-              for (int u = 0; u < 8; ++u)
-                if (r + u * 64 < size) mine += lb_sax_word(spaa, w[u], bp, ratio) <= kth0 ? 1ull : 0ull;
-            }
-          }
-          for (int j = jb; j < je; ++j) seen += s_lsize[j];
-          if (mine) atomicAdd(&s_lab, mine);
-          __syncthreads();
-          if (threadIdx.x == 0) {
-            const unsigned long long lab = s_lab;
-            if ((float)(1.0 - (double)lab / nser) < mp.sax_th) s_dec = 1;
-            else if (!((float)(1.0 - (double)(lab + (s_crows - seen)) / nser) < mp.sax_th)) s_dec = 2;
-          }
-          __syncthreads();
-          if (s_dec) break;
-        }
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        s_branch = s_dec ? s_dec : ((float)(1.0 - (double)s_lab / nser) < mp.sax_th ? 1 : 2);
-      }
-      __syncthreads();
-      br = s_branch;
-    }
+    const int br = s_branch;
     branch = (uint32_t)br;
-    if (br == 1) {
-      // candidates in file (leaf) order, each against the lagging k-th distance
-      __shared__ float s_kthb;
-      if (threadIdx.x == 0) s_kthb = __uint_as_float(tk.kth_bits);
-      for (uint32_t l0 = 0; l0 < nleaf; l0 += kBlock) {
-        const uint32_t l = l0 + threadIdx.x;
-        const bool is_c = l < nleaf && qkeys[l] != 0xFFFFFFFFu;
-        __syncthreads();
-        // ordered compaction of this chunk's candidates
-        scan[threadIdx.x] = is_c ? 1u : 0u;
-        __syncthreads();
-        for (int off = 1; off < kBlock; off <<= 1) {
-          uint32_t v = threadIdx.x >= (uint32_t)off ? scan[threadIdx.x - off] : 0u;
-          __syncthreads();
-          scan[threadIdx.x] += v;
-          __syncthreads();
-        }
-        if (is_c) {
-          s_list[scan[threadIdx.x] - 1] = l;
-          qkeys[l] = 0xFFFFFFFFu;  // handled here, not by the work list
-        }
-        const int nl = (int)scan[kBlock - 1];
-        __syncthreads();
-        for (int j = 0; j < nl; ++j) {
-          const uint32_t lf = s_list[j];
-          if (threadIdx.x == 0) s_go = qnlb[leaf_node[lf]] <= s_kthb / (1.0f + mp.epsilon);
-          __syncthreads();
-          const uint32_t size = leaf_size[lf];
-          if (s_go && size > 0) {
-            const uint64_t pos = leaf_pos[lf];
-            scan_rows(X, L, qs, pos, 0, size - 1, ids, k, tk, scan, nref);
-            if (threadIdx.x == 0) s_kthb = __uint_as_float(tk.kth_bits);  // read before the last row
-            scan_rows(X, L, qs, pos, size - 1, size, ids, k, tk, scan, nref);
-            ++nscanned;
-          }
-          __syncthreads();
-        }
-      }
-    } else if (br == 2) {
+    if (br == 2) {
       nscanned = ncand - nempty;
       if (threadIdx.x == 0 && nscanned) atomicAdd(nwork, nscanned);
     }
   }
-  flush_topk(tk, k, q, cap, cand, cand_cnt, bound);
+  // the sequential branch, and the iSAX count that may choose it, run in
+  // k_mode_count / k_mode_finish: hand them this query's state and answer
+  const bool defer = branch == 1 || branch == 3;
+  if (threadIdx.x == 0) {
+    uint32_t *ms = mstate + (uint64_t)q * kModeState;
+    ms[3] = defer ? branch : 0u;
+    if (defer) {
+      ms[0] = tk.kth_bits;  // kth0: the k-th distance after the approximate phase
+      ms[1] = ncand;
+      ms[2] = nempty;
+      ms[4] = (uint32_t)tk.cnt;
+    }
+  }
+  if (defer) {
+    for (int i = threadIdx.x; i < tk.cnt; i += kBlock) vtop[(uint64_t)q * vcap + i] = tk.keys[i];
+  } else {
+    flush_topk(tk, k, q, cap, cand, cand_cnt, bound);
+  }
   if (threadIdx.x == 0) {
     info[4 * q + 0] = nvis;
     info[4 * q + 1] = ncand;
@@ -1588,10 +1495,181 @@ __global__ void __launch_bounds__(kBlock) k_mode_search(
     atomicAdd(&counters[kCtrApprox], first_rows);
     atomicAdd(&counters[kCtrModeFirst], (unsigned long long)nvis);
     atomicAdd(&counters[kCtrModeCand], (unsigned long long)ncand);
-    atomicAdd(&counters[kCtrModeScanned], (unsigned long long)nscanned);
-    if (branch == 1) atomicAdd(&counters[kCtrModeSeq], 1ull);
+    if (!defer && nscanned) atomicAdd(&counters[kCtrModeScanned], (unsigned long long)nscanned);
   }
   if ((threadIdx.x & 63) == 0 && nref) atomicAdd(&counters[kCtrRefined], (unsigned long long)nref);
+}
+
+// The iSAX count of the epsilon search's queries whose branch it decides
+// (:2171-2205): the rows of their candidate leaves whose bound is within
+// kth0, every (query, 256-leaf chunk) a block, one wave per leaf.
+__global__ void __launch_bounds__(kBlock) k_mode_count(
+    const uint8_t *__restrict__ sax, uint32_t L, const float *__restrict__ paa, uint32_t nnodes,
+    const float *__restrict__ nlb, uint32_t nleaf, const uint64_t *__restrict__ leaf_pos,
+    const uint32_t *__restrict__ leaf_size, const int32_t *__restrict__ leaf_node,
+    const uint32_t *__restrict__ keys, const uint32_t *__restrict__ mstate, unsigned long long *__restrict__ lab)
+{
+  const uint32_t q = blockIdx.y;
+  const uint32_t *ms = mstate + (uint64_t)q * kModeState;
+  if (ms[3] != 3u) return;  // uniform
+  __shared__ float bp[256];
+  __shared__ float spaa[kPaaSegments];
+  __shared__ uint64_t s_lpos[kBlock];
+  __shared__ uint32_t s_lsize[kBlock];
+  __shared__ int s_n;
+  __shared__ unsigned long long s_cnt;
+  const float kth0 = __uint_as_float(ms[0]);
+  for (int i = threadIdx.x; i < 256; i += kBlock) bp[i] = c_bp[i];
+  if (threadIdx.x < kPaaSegments) spaa[threadIdx.x] = paa[q * kPaaSegments + threadIdx.x];
+  if (threadIdx.x == 0) {
+    s_n = 0;
+    s_cnt = 0;
+  }
+  __syncthreads();
+  const uint32_t l = blockIdx.x * kBlock + threadIdx.x;
+  if (l < nleaf && keys[(uint64_t)q * nleaf + l] != 0xFFFFFFFFu &&
+      nlb[(uint64_t)q * nnodes + leaf_node[l]] <= kth0) {
+    const int j = atomicAdd(&s_n, 1);
+    s_lpos[j] = leaf_pos[l];
+    s_lsize[j] = leaf_size[l];
+  }
+  __syncthreads();
+  const int n = s_n;
+  const float ratio = (float)(L / kPaaSegments);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  unsigned long long mine = 0;
+  for (int j = wave; j < n; j += kBlock / 64) {
+    const uint64_t pos = s_lpos[j];
+    const uint32_t size = s_lsize[j];
+    for (uint32_t r = lane; r < size; r += 8 * 64) {
+      uint4 w[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t rr = r + u * 64;
+        w[u] = rr < size ? *reinterpret_cast<const uint4 *>(sax + (pos + rr) * kPaaSegments) : make_uint4(0, 0, 0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (r + u * 64 < size) mine += lb_sax_word(spaa, w[u], bp, ratio) <= kth0 ? 1ull : 0ull;
+    }
+  }
+  if (mine) atomicAdd(&s_cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_cnt) atomicAdd(&lab[q], s_cnt);
+}
+
+// The epsilon search's deferred queries: the branch the iSAX count chose
+// (:967-978), then either every candidate to the work list, or the
+// candidates scanned in file order against the k-th distance the
+// reference's loop variable holds (exact_de_knn_search_noapprox,
+// :2450-2507); the answer so far comes back from k_mode_search's hand-over.
+__global__ void __launch_bounds__(kBlock) k_mode_finish(
+    const float *__restrict__ X, uint32_t L, const float *__restrict__ Q, const uint32_t *__restrict__ kq,
+    uint32_t nnodes, const float *__restrict__ nlb, uint32_t nleaf, const uint64_t *__restrict__ leaf_pos,
+    const uint32_t *__restrict__ leaf_size, const int32_t *__restrict__ leaf_node,
+    const uint32_t *__restrict__ ids, ModeParams mp, uint32_t cap, uint64_t *__restrict__ cand,
+    uint32_t *__restrict__ cand_cnt, uint32_t *__restrict__ bound, uint32_t *__restrict__ keys,
+    uint32_t *__restrict__ nwork, uint32_t *__restrict__ info, unsigned long long *__restrict__ counters,
+    const uint32_t *__restrict__ mstate, const uint64_t *__restrict__ vtop, uint32_t vcap,
+    const unsigned long long *__restrict__ lab)
+{
+  const uint32_t q = blockIdx.x;
+  const uint32_t *ms = mstate + (uint64_t)q * kModeState;
+  uint32_t br = ms[3];
+  if (br == 0u) return;  // uniform: answered by k_mode_search
+  extern __shared__ float qs[];
+  __shared__ TopK tk;
+  __shared__ uint32_t scan[kBlock];
+  __shared__ uint32_t s_list[kBlock];
+  __shared__ float s_kthb;
+  __shared__ int s_go;
+  const int k = (int)kq[q];
+  const uint32_t ncand = ms[1], nempty = ms[2];
+  if (br == 3u) {
+    const float slp = (float)(1.0 - (double)lab[q] / (double)(float)mp.nseries);
+    br = slp < mp.sax_th ? 1u : 2u;
+  }
+  load_query(Q, q, L, qs);
+  // the answer after the approximate phase, as k_mode_search left it
+  const int cnt = (int)ms[4];
+  for (int i = threadIdx.x; i < cnt; i += kBlock) tk.keys[i] = vtop[(uint64_t)q * vcap + i];
+  if (threadIdx.x == 0) {
+    tk.cnt = cnt;
+    tk.nstage = 0;
+    tk.kth_bits = cnt == k ? (uint32_t)(tk.keys[k - 1] >> 32) : kInfBits;
+  }
+  __syncthreads();
+  uint32_t nref = 0, nscanned = 0;
+  uint32_t *qkeys = keys + (uint64_t)q * nleaf;
+  const float *qnlb = nlb + (uint64_t)q * nnodes;
+  if (br == 2u) {
+    nscanned = ncand - nempty;
+    if (threadIdx.x == 0 && nscanned) atomicAdd(nwork, nscanned);
+  } else {
+    if (threadIdx.x == 0) s_kthb = __uint_as_float(tk.kth_bits);
+    for (uint32_t l0 = 0; l0 < nleaf; l0 += kBlock) {
+      const uint32_t l = l0 + threadIdx.x;
+      const bool is_c = l < nleaf && qkeys[l] != 0xFFFFFFFFu;
+      __syncthreads();
+      // ordered compaction of this chunk's candidates
+      scan[threadIdx.x] = is_c ? 1u : 0u;
+      __syncthreads();
+      for (int off = 1; off < kBlock; off <<= 1) {
+        uint32_t v = threadIdx.x >= (uint32_t)off ? scan[threadIdx.x - off] : 0u;
+        __syncthreads();
+        scan[threadIdx.x] += v;
+        __syncthreads();
+      }
+      if (is_c) {
+        s_list[scan[threadIdx.x] - 1] = l;
+        qkeys[l] = 0xFFFFFFFFu;  // handled here, not by the work list
+      }
+      const int nl = (int)scan[kBlock - 1];
+      __syncthreads();
+      for (int j = 0; j < nl; ++j) {
+        const uint32_t lf = s_list[j];
+        if (threadIdx.x == 0) s_go = qnlb[leaf_node[lf]] <= s_kthb / (1.0f + mp.epsilon);
+        __syncthreads();
+        const uint32_t size = leaf_size[lf];
+        if (s_go && size > 0) {
+          const uint64_t pos = leaf_pos[lf];
+          scan_rows(X, L, qs, pos, 0, size - 1, ids, k, tk, scan, nref);
+          if (threadIdx.x == 0) s_kthb = __uint_as_float(tk.kth_bits);  // read before the last row
+          scan_rows(X, L, qs, pos, size - 1, size, ids, k, tk, scan, nref);
+          ++nscanned;
+        }
+        __syncthreads();
+      }
+    }
+  }
+  flush_topk(tk, k, q, cap, cand, cand_cnt, bound);
+  if (threadIdx.x == 0) {
+    info[4 * q + 2] = br;
+    info[4 * q + 3] = nscanned;
+    if (nscanned) atomicAdd(&counters[kCtrModeScanned], (unsigned long long)nscanned);
+    if (br == 1u) atomicAdd(&counters[kCtrModeSeq], 1ull);
+  }
+  if ((threadIdx.x & 63) == 0 && nref) atomicAdd(&counters[kCtrRefined], (unsigned long long)nref);
+}
+
+void launch_mode_count(hipStream_t st, uint32_t nq, const uint8_t *sax, uint32_t L, const float *paa, uint32_t nnodes,
+                       const float *nlb, uint32_t nleaf, const uint64_t *leaf_pos, const uint32_t *leaf_size,
+                       const int32_t *leaf_node, const uint32_t *keys, const uint32_t *mstate, unsigned long long *lab)
+{
+  hipLaunchKernelGGL(k_mode_count, dim3((nleaf + kBlock - 1) / kBlock, nq), dim3(kBlock), 0, st, sax, L, paa, nnodes,
+                     nlb, nleaf, leaf_pos, leaf_size, leaf_node, keys, mstate, lab);
+}
+
+void launch_mode_finish(hipStream_t st, uint32_t nq, const float *X, uint32_t L, const float *Q, const uint32_t *kq,
+                        uint32_t nnodes, const float *nlb, uint32_t nleaf, const uint64_t *leaf_pos,
+                        const uint32_t *leaf_size, const int32_t *leaf_node, const uint32_t *ids, const ModeParams &mp,
+                        uint32_t cap, uint64_t *cand, uint32_t *cand_cnt, uint32_t *bound, uint32_t *keys,
+                        uint32_t *nwork, uint32_t *info, unsigned long long *counters, const uint32_t *mstate,
+                        const uint64_t *vtop, uint32_t vcap, const unsigned long long *lab)
+{
+  hipLaunchKernelGGL(k_mode_finish, dim3(nq), dim3(kBlock), L * sizeof(float), st, X, L, Q, kq, nnodes, nlb, nleaf,
+                     leaf_pos, leaf_size, leaf_node, ids, mp, cap, cand, cand_cnt, bound, keys, nwork, info, counters,
+                     mstate, vtop, vcap, lab);
 }
 
 void launch_node_lb(hipStream_t st, uint32_t nq, uint32_t nnodes, uint32_t L, const float *Q, const uint2 *nd_seg,
@@ -1622,7 +1700,7 @@ void launch_mode_search(hipStream_t st, uint32_t nq, const float *X, const uint8
                         const uint32_t *leaf_size, const int32_t *leaf_node, const uint32_t *ids, const float *lb,
                         uint64_t *heap_g, uint32_t *bfs, const ModeParams &mp, uint32_t cap, uint64_t *cand,
                         uint32_t *cand_cnt, uint32_t *bound, uint32_t *keys, uint32_t *nwork, uint32_t *info,
-                        unsigned long long *counters)
+                        unsigned long long *counters, uint32_t *mstate, uint64_t *vtop, uint32_t vcap)
 {
   const bool in_lds = mode_heap_in_lds(L, nnodes);
   const size_t lds = mode_qs_bytes(L) + (in_lds ? mode_heap_bytes(nnodes) : 0);
@@ -1634,7 +1712,8 @@ void launch_mode_search(hipStream_t st, uint32_t nq, const float *X, const uint8
   }
   hipLaunchKernelGGL(k_mode_search, dim3(nq), dim3(kBlock), lds, st, X, sax, L, Q,
                      paa, kq, nnodes, nd_link, nd_split, nlb, nleaf, leaf_pos, leaf_size, leaf_node, ids, lb, heap_g,
-                     in_lds ? 1 : 0, bfs, mp, cap, cand, cand_cnt, bound, keys, nwork, info, counters);
+                     in_lds ? 1 : 0, bfs, mp, cap, cand, cand_cnt, bound, keys, nwork, info, counters, mstate, vtop,
+                     vcap);
 }
 
 }  // namespace hkg
