@@ -485,17 +485,27 @@ CallSettings &call_settings()
 // ---------------------------------------------------------------------------
 // one batch through a path (device pointers; results left in b->out_*)
 // ---------------------------------------------------------------------------
-// persistent grid of k_filter_refine: as many blocks as fit the CUs at once
-static uint32_t filter_grid()
+static uint32_t cu_count()
 {
-  static uint32_t g = 0;
-  if (!g) {
-    int dev = 0, cus = 0;
+  static uint32_t cus = 0;
+  if (!cus) {
+    int dev = 0, n = 0;
     (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    g = (uint32_t)std::max(cus, 1) * kFilterBlocksPerCU;
+    (void)hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+    cus = (uint32_t)std::max(n, 1);
   }
-  return g;
+  return cus;
+}
+
+// persistent grid of k_filter_refine: blocks per CU from the call, else the
+// tuned shape, else kFilterBlocksPerCU (any grid is correct: the blocks
+// take work items from one queue)
+uint32_t Index::filter_grid() const
+{
+  uint32_t per = call_settings().fblocks;
+  if (!per) per = tuned_fblocks.load(std::memory_order_relaxed);
+  if (!per) per = kFilterBlocksPerCU;
+  return cu_count() * per;
 }
 
 static uint32_t persistent_grid()
@@ -663,7 +673,7 @@ hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, 
     rc = enqueue_mode_batch(*b, nq, st, pf, seed_host != nullptr);
     if (rc != HKG_OK) return rc;
   } else if (use_graph && !scan && !pf) {
-    if (!b->graph_exec || b->graph_stream != st || b->graph_nq != nq) {
+    if (!b->graph_exec || b->graph_stream != st || b->graph_nq != nq || b->graph_fgrid != filter_grid()) {
       if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
       b->graph_exec = nullptr;
       hipGraph_t g;
@@ -675,6 +685,7 @@ hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, 
       HKG_HIP_TRY(e);
       b->graph_stream = st;
       b->graph_nq = nq;
+      b->graph_fgrid = filter_grid();
     }
     HKG_HIP_TRY(hipGraphLaunch(b->graph_exec, st));
   } else if (scan) {
@@ -866,6 +877,7 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
   }
   if (!dev.has_tree && path != HKG_PATH_SCAN) path = HKG_PATH_SCAN;
   uint32_t batch = o.batch;
+  if (batch == 0) batch = tuned_batch[path].load(std::memory_order_relaxed);
   if (batch == 0) batch = path == HKG_PATH_INDEX ? 1 : (path == HKG_PATH_SCAN ? 64 : 1024);
   if (path == HKG_PATH_INDEX) batch = 1;
   if (batch > 16384) batch = 16384;  // grid.y of the per-query launches
@@ -1040,6 +1052,106 @@ hkg_status Index::knn_host(const float *Q, uint32_t nq, const uint32_t *k, uint3
   return rc;
 }
 
+// Launch-shape tuner: each candidate (queries per batch x filter grid) runs
+// the sample twice and keeps its faster device time; the default shape runs
+// first, after one untimed warm-up call (BF16 copy, buffer sets, code
+// objects).  Answers must not depend on the shape: any difference is an
+// engine fault, reported instead of tuning.
+hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint32_t kmax, uint32_t path_in,
+                           hkg_tuning *out)
+{
+  if (!Q || !out || nq == 0 || kmax == 0 || path_in > HKG_PATH_BF16) {
+    set_error("autotune: null argument, no queries, kmax 0 or unknown path");
+    return HKG_EINVAL;
+  }
+  hkg_path path = path_in == HKG_PATH_AUTO ? HKG_PATH_BATCHED : (hkg_path)path_in;
+  if (!dev.has_tree) path = HKG_PATH_SCAN;
+  std::vector<uint32_t> batches, grids;
+  uint32_t def_batch = 1024, def_grid = 0;
+  switch (path) {
+    case HKG_PATH_INDEX:
+      batches = {1};
+      grids = {2, 4, 8};
+      def_batch = 1;
+      def_grid = kFilterBlocksPerCU;
+      break;
+    case HKG_PATH_BATCHED:
+      batches = {256, 512, 1024, 2048};
+      grids = {2, 4, 8};
+      def_grid = kFilterBlocksPerCU;
+      break;
+    case HKG_PATH_BF16:
+      batches = {512, 1024, 2048, 4096};
+      grids = {0};
+      break;
+    default:
+      batches = {16, 64, 256};
+      grids = {0};
+      def_batch = 64;
+      break;
+  }
+  // batches beyond the sample time the same as the sample-sized one
+  std::vector<uint32_t> bs;
+  for (uint32_t b : batches)
+    if (b <= nq || bs.empty()) bs.push_back(b);
+  struct Restore {
+    CallSettings saved;
+    ~Restore() { call_settings() = saved; }
+  } restore{call_settings()};
+  hkg_query_opts qo;
+  hkg_default_query_opts(&qo);
+  qo.path = path;
+  const size_t m = (size_t)nq * kmax;
+  std::vector<uint64_t> ids0(m), ids(m);
+  std::vector<float> d20(m), d2(m);
+  auto run = [&](uint32_t batch, uint32_t grid, uint64_t *I, float *D, double *ms) -> hkg_status {
+    qo.batch = batch;
+    call_settings().fblocks = grid;
+    *ms = 1e300;
+    for (int rep = 0; rep < 2; ++rep) {
+      hkg_query_stats st;
+      hkg_status rc = knn_host(Q, nq, k, kmax, &qo, I, D, &st);
+      if (rc != HKG_OK) return rc;
+      *ms = std::min(*ms, st.device_ms);
+    }
+    return HKG_OK;
+  };
+  double ms_def = 0.0, ms;
+  hkg_status rc = run(def_batch, def_grid, ids0.data(), d20.data(), &ms);  // warm-up
+  if (rc != HKG_OK) return rc;
+  if ((rc = run(def_batch, def_grid, ids0.data(), d20.data(), &ms_def)) != HKG_OK) return rc;
+  uint32_t best_b = def_batch, best_g = def_grid, ncand = 1;
+  double best = ms_def;
+  for (uint32_t b : bs) {
+    for (uint32_t g : grids) {
+      if (b == def_batch && g == def_grid) continue;
+      if ((rc = run(b, g, ids.data(), d2.data(), &ms)) != HKG_OK) return rc;
+      ++ncand;
+      if (memcmp(ids.data(), ids0.data(), m * 8) != 0 || memcmp(d2.data(), d20.data(), m * 4) != 0) {
+        set_error("autotune: a launch shape changed the answers (batch " + std::to_string(b) + ", grid " +
+                  std::to_string(g) + " blocks per CU)");
+        return HKG_EINTERNAL;
+      }
+      if (ms < best) {
+        best = ms;
+        best_b = b;
+        best_g = g;
+      }
+    }
+  }
+  tuned_batch[path].store(best_b, std::memory_order_relaxed);
+  if (best_g) tuned_fblocks.store(best_g, std::memory_order_relaxed);
+  memset(out, 0, sizeof(*out));
+  out->path = path;
+  out->length = dev.L;
+  out->batch = best_b;
+  out->filter_blocks_per_cu = best_g;
+  out->candidates = ncand;
+  out->ms_per_query = best / nq;
+  out->ms_per_query_default = ms_def / nq;
+  return HKG_OK;
+}
+
 }  // namespace hkg
 
 // ---------------------------------------------------------------------------
@@ -1081,6 +1193,7 @@ const char *hkg_status_str(hkg_status s)
   case HKG_EDEVICE: return "device error";
   case HKG_EBUSY: return "busy";
   case HKG_ENOTREADY: return "not ready";
+  case HKG_EINTERNAL: return "internal check failed";
   }
   return "unknown status";
 }
@@ -1175,6 +1288,13 @@ hkg_status hkg_knn(hkg_index *ix, const float *Q, uint32_t nq, const uint32_t *k
 {
   if (!ix) return HKG_EINVAL;
   return reinterpret_cast<Index *>(ix)->knn_host(Q, nq, k, kmax, qo, ids, d2, stats);
+}
+
+hkg_status hkg_autotune(hkg_index *ix, const float *Q, uint32_t nq, const uint32_t *k, uint32_t kmax,
+                        uint32_t path, hkg_tuning *out)
+{
+  if (!ix) return HKG_EINVAL;
+  return reinterpret_cast<Index *>(ix)->autotune(Q, nq, k, kmax, path, out);
 }
 
 hkg_status hkg_knn_device(hkg_index *ix, const float *dQ, uint32_t nq, const uint32_t *dk, uint32_t kmax,

@@ -1,6 +1,7 @@
 // Engine objects behind the C ABI handles.
 #pragma once
 
+#include <atomic>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -113,6 +114,7 @@ struct QueryBuffers {
   hipGraphExec_t graph_exec = nullptr;
   hipStream_t graph_stream = nullptr;
   uint32_t graph_nq = 0;
+  uint32_t graph_fgrid = 0;     // filter + refine grid the graph was captured with
   ~QueryBuffers();
 };
 
@@ -126,6 +128,10 @@ struct Index {
   std::mutex mu_bufs, mu_bf16;
   bool bf16_ready = false;      // BF16 copy, norms and tile tables built (guarded by mu_bf16)
   std::multimap<uint64_t, std::unique_ptr<QueryBuffers>> bufs;
+  // launch shapes chosen by autotune (0: the defaults): queries per batch
+  // of each path, and k_filter_refine blocks per CU
+  std::atomic<uint32_t> tuned_batch[HKG_PATH_BF16 + 1] = {};
+  std::atomic<uint32_t> tuned_fblocks{0};
 
   ~Index();
   hkg_status init_device(int device);
@@ -148,6 +154,10 @@ struct Index {
                             bool bound_pass);
   uint64_t total_scan_bytes(uint32_t nq, uint32_t batch) const;
   uint64_t total_segs() const;
+  // persistent grid of k_filter_refine for the running call
+  uint32_t filter_grid() const;
+  hkg_status autotune(const float *Q, uint32_t nq, const uint32_t *k, uint32_t kmax, uint32_t path,
+                      hkg_tuning *out);
   hkg_status fix_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax, hipStream_t st, uint32_t *retries);
   hkg_status fix_bf16_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax, hipStream_t st, uint32_t *retries);
   hkg_status knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, uint32_t kmax, const hkg_query_opts *qo,
@@ -163,6 +173,7 @@ struct CallSettings {
   uint32_t bound_pass = 0;    // hkg_query_opts.bf16_bound_pass
   uint32_t min_buffer_batch = 0;  // size buffer sets for at least this many queries (server workers)
   bool mode = false;          // an approximate mode (epsilon > 0 or nprobes > 0) is on
+  uint32_t fblocks = 0;       // k_filter_refine blocks per CU (0: the index's tuned or default)
   ModeParams mp{};            // its parameters (nleaves / nseries filled per index)
 };
 CallSettings &call_settings();

@@ -17,11 +17,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhkg.so")
 
 HKG_PATH_AUTO, HKG_PATH_INDEX, HKG_PATH_BATCHED, HKG_PATH_SCAN, HKG_PATH_BF16 = range(5)
-HKG_OK, HKG_EINVAL, HKG_ENOMEM, HKG_EIO, HKG_EFORMAT, HKG_EDEVICE, HKG_EBUSY, HKG_ENOTREADY = range(8)
+HKG_OK, HKG_EINVAL, HKG_ENOMEM, HKG_EIO, HKG_EFORMAT, HKG_EDEVICE, HKG_EBUSY, HKG_ENOTREADY, HKG_EINTERNAL = range(9)
 PATHS = {"auto": HKG_PATH_AUTO, "index": HKG_PATH_INDEX, "batched": HKG_PATH_BATCHED,
          "scan": HKG_PATH_SCAN, "bf16": HKG_PATH_BF16}
 STATUS = {0: "ok", 1: "invalid argument", 2: "out of memory", 3: "i/o error", 4: "malformed index",
-          5: "device error", 6: "busy", 7: "not ready"}
+          5: "device error", 6: "busy", 7: "not ready", 8: "internal check failed"}
 FLT_MAX = np.float32(np.finfo(np.float32).max)
 NO_ID = np.uint64(0xFFFFFFFFFFFFFFFF)
 
@@ -80,6 +80,12 @@ class QueryStats(C.Structure):
         return d
 
 
+class Tuning(C.Structure):
+    _fields_ = [("path", C.c_uint32), ("length", C.c_uint32), ("batch", C.c_uint32),
+                ("filter_blocks_per_cu", C.c_uint32), ("candidates", C.c_uint32), ("reserved", C.c_uint32),
+                ("ms_per_query", C.c_double), ("ms_per_query_default", C.c_double)]
+
+
 class IndexInfo(C.Structure):
     _fields_ = [("n_series", C.c_uint64), ("length", C.c_uint32), ("paa_segments", C.c_uint32),
                 ("sax_bit_cardinality", C.c_uint32), ("max_leaf_size", C.c_uint32), ("n_leaves", C.c_uint64),
@@ -111,6 +117,7 @@ SIGNATURES = {
     "hkg_close": (None, [_P]),
     "hkg_knn": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P]),
     "hkg_knn_device": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P]),
+    "hkg_autotune": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, _P]),
     "hkg_default_build_opts": (None, [_P]),
     "hkg_build": (C.c_int, [_P, C.c_uint64, C.c_uint32, _P, C.POINTER(_P)]),
     "hkg_build_device": (C.c_int, [_P, C.c_uint64, C.c_uint32, _P, C.POINTER(_P)]),
@@ -324,6 +331,27 @@ class HerculesIndex:
         _check(lib().hkg_knn(self._h, _ptr(Q), nq, _ptr(kv) if kv is not None else None, kmax, C.byref(qo),
                              _ptr(ids), _ptr(d2), C.byref(st)))
         return KnnResult(ids, d2, st.as_dict())
+
+    def autotune(self, Q: np.ndarray, k, path: str = "batched") -> dict:
+        """Time the launch shapes of ``path`` on the sample queries Q (k an
+        int or per-query array) and keep the fastest for later calls that
+        leave ``batch`` at 0."""
+        Q = np.ascontiguousarray(Q, np.float32)
+        if Q.ndim == 1:
+            Q = Q[None, :]
+        nq = Q.shape[0]
+        if np.isscalar(k):
+            kmax, kv = int(k), None
+        else:
+            kv = np.ascontiguousarray(k, np.uint32)
+            kmax = int(kv.max()) if nq else 1
+        t = Tuning()
+        _check(lib().hkg_autotune(self._h, _ptr(Q), nq, _ptr(kv) if kv is not None else None, kmax, PATHS[path],
+                                  C.byref(t)))
+        names = {v: n for n, v in PATHS.items()}
+        return {"path": names[t.path], "length": t.length, "batch": t.batch,
+                "filter_blocks_per_cu": t.filter_blocks_per_cu, "candidates": t.candidates,
+                "ms_per_query": t.ms_per_query, "ms_per_query_default": t.ms_per_query_default}
 
     def knn_device(self, dQ: DeviceArray, nq: int, dk: DeviceArray, kmax: int, dids: DeviceArray,
                    dd2: DeviceArray, path: str = "auto", batch: int = 0, profile: bool = False,

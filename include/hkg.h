@@ -22,6 +22,8 @@
  *                     1694-1791): a GPU bulk loader producing the same
  *                     on-disk format (root.idx, leaves_raw.idx,
  *                     leaves_sims.idx)
+ *   hkg_autotune      no counterpart: the reference's query-time shape is
+ *                     its thread counts, set by hand on the command line
  *   hkg_server_*      no single counterpart: the query file loop
  *                     hercules_knn_query_binary_file
  *                     (code/hercules/src/hercules_file_loaders.c:99-294) as a
@@ -53,7 +55,8 @@ typedef enum hkg_status {
   HKG_EFORMAT = 4,  /* index file malformed or truncated */
   HKG_EDEVICE = 5,  /* HIP runtime error */
   HKG_EBUSY = 6,    /* serving loop full (admission refused) */
-  HKG_ENOTREADY = 7 /* serving ticket not finished yet */
+  HKG_ENOTREADY = 7, /* serving ticket not finished yet */
+  HKG_EINTERNAL = 8  /* an engine self-check failed */
 } hkg_status;
 
 typedef struct hkg_index hkg_index;
@@ -192,6 +195,34 @@ hkg_status hkg_knn(hkg_index *ix, const float *Q, uint32_t nq, const uint32_t *k
 hkg_status hkg_knn_device(hkg_index *ix, const float *dQ, uint32_t nq, const uint32_t *dk,
                           uint32_t kmax, const hkg_query_opts *qo, uint64_t *dids, float *dd2,
                           void *stream, hkg_query_stats *stats);
+
+/* ---- launch-shape autotuner --------------------------------------------- */
+/* The reference has no tuner: its query-time shape is the thread counts set
+   by hand (--num-threads / --num-read-threads, hercules.c:104-105,227;
+   PSCAN's --num-query-threads / --dbsize, parallel_scan.c:64-65).  Here the
+   shapes are timed on the index itself, for its series length: the queries
+   per device batch of a path and, on the FP32 index paths, the persistent
+   grid of the iSAX filter + refine kernel (blocks per CU).  Every candidate
+   must return the same answers, bit for bit (HKG_EINTERNAL otherwise). */
+typedef struct hkg_tuning {
+  uint32_t path;                 /* the path tuned (hkg_path; AUTO tunes BATCHED) */
+  uint32_t length;               /* series length of the index                    */
+  uint32_t batch;                /* queries per device batch chosen               */
+  uint32_t filter_blocks_per_cu; /* chosen grid of the FP32 filter + refine
+                                    (0: the path does not launch it)              */
+  uint32_t candidates;           /* shapes timed                                  */
+  uint32_t reserved;
+  double ms_per_query;           /* device time per query, the chosen shape       */
+  double ms_per_query_default;   /* device time per query, the default shape      */
+} hkg_tuning;
+
+/* Time the candidate shapes of a path on nq sample queries (host rows, k as
+   in hkg_knn), keep the fastest in the handle and return it.  Later calls
+   on that path with hkg_query_opts.batch = 0 use the tuned batch; the FP32
+   index paths and the modes use the tuned grid.  Not to be called while
+   other calls on the same handle run. */
+hkg_status hkg_autotune(hkg_index *ix, const float *Q, uint32_t nq, const uint32_t *k, uint32_t kmax,
+                        uint32_t path, hkg_tuning *out);
 
 /* ---- index build (GPU bulk loader, same on-disk format) ---------------- */
 typedef struct hkg_build_opts {

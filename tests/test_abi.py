@@ -22,7 +22,7 @@ def declared() -> list[str]:
 def test_header_declares_entry_points():
     names = declared()
     for must in ["hkg_open_dir", "hkg_knn", "hkg_knn_device", "hkg_close", "hkg_build", "hkg_write_dir",
-                 "hkg_server_start", "hkg_submit", "hkg_poll", "hkg_server_stop", "hkg_inspect_dir"]:
+                 "hkg_server_start", "hkg_submit", "hkg_poll", "hkg_server_stop", "hkg_inspect_dir", "hkg_autotune"]:
         assert must in names
 
 
@@ -48,7 +48,7 @@ def test_no_torch_or_hip_types_in_header():
 
 def test_status_strings():
     from hercules_amd import hkg
-    for s in range(8):
+    for s in range(9):
         assert hkg.lib().hkg_status_str(s).decode() == hkg.STATUS[s]
 
 
@@ -75,17 +75,21 @@ def test_query_opts_layout_and_defaults(tmp_path):
     fields = ["path", "batch", "profile", "cand_cap", "bf16_bound_pass", "epsilon", "delta", "nprobes",
               "approx_leaves", "eapca_threshold", "sax_threshold", "reserved"]
     sfields = ["device_ms", "bf16_pairs", "stage_ms", "stage_launches", "mode_leaves_first", "mode_sequential"]
+    tfields = [f for f, _ in hkg.Tuning._fields_]
     prog = tmp_path / "layout.c"
     prog.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "hkg.h"\nint main(void){\n'
                     + "".join(f'printf("%zu\\n", offsetof(hkg_query_opts, {f}));\n' for f in fields)
                     + 'printf("%zu\\n", sizeof(hkg_query_opts));\n'
                     + "".join(f'printf("%zu\\n", offsetof(hkg_query_stats, {f}));\n' for f in sfields)
-                    + 'printf("%zu\\n", sizeof(hkg_query_stats));\nreturn 0;}\n')
+                    + 'printf("%zu\\n", sizeof(hkg_query_stats));\n'
+                    + "".join(f'printf("%zu\\n", offsetof(hkg_tuning, {f}));\n' for f in tfields)
+                    + 'printf("%zu\\n", sizeof(hkg_tuning));\nreturn 0;}\n')
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), str(prog), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [getattr(hkg.QueryOpts, f).offset for f in fields] + [C.sizeof(hkg.QueryOpts)]
     want += [getattr(hkg.QueryStats, f).offset for f in sfields] + [C.sizeof(hkg.QueryStats)]
+    want += [getattr(hkg.Tuning, f).offset for f in tfields] + [C.sizeof(hkg.Tuning)]
     assert got == want
     qo = hkg.query_opts()
     assert qo.epsilon == 0.0 and qo.nprobes == 0 and qo.delta == 1.0 and qo.approx_leaves == 80

@@ -1,0 +1,71 @@
+"""The launch-shape autotuner on an MI355X: for each path and series length
+it returns one of its candidate shapes, the answers after tuning equal
+those before it bit for bit, and later calls that leave the batch at 0 run
+in the tuned batch size."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CANDIDATE_BATCHES = {"batched": {256, 512, 1024, 2048}, "bf16": {512, 1024, 2048, 4096}, "scan": {16, 64, 256},
+                     "index": {1}}
+
+
+@pytest.fixture(scope="module")
+def hkg():
+    from hercules_amd import hkg as m
+    m.lib()
+    return m
+
+
+@pytest.mark.parametrize("kind,L", [(0, 256), (1, 96)])
+def test_autotune_paths(hkg, oracle, kind, L):
+    n = 300_000
+    dX = hkg.DeviceArray(n * L * 4)
+    hkg.gen_series_device(kind, 11, 0, n, L, dX)
+    rng = np.random.default_rng(12)
+    nq = 1200
+    sig = rng.uniform(0.0, 0.2, nq).astype(np.float32)
+    Q, _ = hkg.gen_noisy_queries(kind, 11, n, 13, sig, L)
+    kv = rng.integers(1, 65, nq).astype(np.uint32)
+    with hkg.HerculesIndex.build_device(dX, n, L, leaf_size=3000) as ix:
+        for path, m in [("batched", nq), ("bf16", nq), ("scan", 256), ("index", 48)]:
+            print(f"autotune L={L} path={path} on {m} queries", flush=True)
+            before = ix.knn(Q[:m], kv[:m], path=path)
+            t = ix.autotune(Q[:m], kv[:m], path=path)
+            print(f"  -> {t}", flush=True)
+            assert t["path"] == path and t["length"] == L
+            assert t["batch"] in CANDIDATE_BATCHES[path]
+            if path in ("batched", "index"):
+                assert t["filter_blocks_per_cu"] in (2, 4, 8)
+            else:
+                assert t["filter_blocks_per_cu"] == 0
+            assert t["candidates"] >= 1 and t["ms_per_query"] > 0 and t["ms_per_query_default"] > 0
+            assert t["ms_per_query"] <= t["ms_per_query_default"]
+            after = ix.knn(Q[:m], kv[:m], path=path)
+            assert np.array_equal(after.d2.view(np.uint32), before.d2.view(np.uint32)), path
+            assert np.array_equal(after.ids, before.ids), path
+            assert after.stats["launches"] == -(-m // t["batch"]), (path, after.stats["launches"])
+        # the exact answers, checked by the oracle on a few queries
+        sel = np.arange(0, nq, 150)
+        r = ix.knn(Q[sel], 10, path="batched")
+        for j, qi in enumerate(sel):
+            for rank in range(10):
+                row = oracle.series(kind, 11, int(r.ids[j, rank]), 1, L, threads=1)[0]
+                assert oracle.ed2(Q[qi], row) == r.d2[j, rank]
+
+
+def test_autotune_rejects_bad_arguments(hkg):
+    X = np.ascontiguousarray(np.random.default_rng(0).standard_normal((4000, 64)), np.float32)
+    dX = hkg.DeviceArray(X.nbytes)
+    dX.upload(X)
+    with hkg.HerculesIndex.build_device(dX, 4000, 64, leaf_size=500) as ix:
+        t = hkg.Tuning()
+        assert hkg.lib().hkg_autotune(ix._h, X.ctypes.data, 0, None, 5, 2, C.byref(t)) == hkg.HKG_EINVAL
+        assert hkg.lib().hkg_autotune(ix._h, X.ctypes.data, 4, None, 5, 9, C.byref(t)) == hkg.HKG_EINVAL
+        assert hkg.lib().hkg_autotune(ix._h, X.ctypes.data, 4, None, 0, 2, C.byref(t)) == hkg.HKG_EINVAL
+        assert hkg.lib().hkg_autotune(ix._h, X.ctypes.data, 4, None, 5, 2, None) == hkg.HKG_EINVAL
