@@ -41,7 +41,9 @@ def parse() -> argparse.Namespace:
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--n", type=int, default=100_000_000, help="series in the index")
     p.add_argument("--length", type=int, default=256)
-    p.add_argument("--batch", type=int, default=1024, help="queries per step")
+    p.add_argument("--batch", type=int, default=0,
+                   help="queries per step (0: the batch size hkg_autotune picks for --path on this index, timed "
+                        "before the timed steps)")
     p.add_argument("--leaf", type=int, default=10000)
     p.add_argument("--path", default="bf16", choices=["batched", "index", "scan", "bf16"])
     p.add_argument("--sigma-max", type=float, default=0.2)
@@ -65,6 +67,9 @@ def parse() -> argparse.Namespace:
                         "scan (0: skip the leg)")
     p.add_argument("--loader-series", type=int, default=0,
                    help="loader leg: write an index of this many series to TMPDIR and time hkg_open_dir (0: skip)")
+    p.add_argument("--tune-queries", type=int, default=4096,
+                   help="with --batch 0: sample queries (seed 6, same noise and k) the autotuner times the "
+                        "candidate batch sizes of --path on")
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
                    help="one device-to-device copy of this size first (reference dispatch for counter runs)")
     return p.parse_args()
@@ -158,17 +163,20 @@ def mode_cpu_leg(hkg, args, Q: np.ndarray) -> dict:
     return out
 
 
-def traffic_bytes(kernel: str):
-    """HBM bytes per launch of a kernel at this workload, from the counter
-    runs summarised in profiles/ (FETCH_SIZE + WRITE_SIZE, scaled by the
-    same counters on a copy of known size); None when not measured."""
+def traffic_bytes(kernel: str, n: int, L: int, batch: int):
+    """HBM bytes per launch of a kernel at this workload (series, length,
+    queries per launch), from the counter runs summarised in profiles/
+    (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a copy of known
+    size); None when that workload was not measured."""
     path = os.path.join(REPO, "profiles", "traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
         parts = line.split()
-        if len(parts) >= 2 and parts[0] == kernel:
-            return float(parts[1])
+        if line.startswith("#") or len(parts) < 5:
+            continue
+        if parts[0] == kernel and (int(parts[1]), int(parts[2]), int(parts[3])) == (n, L, batch):
+            return float(parts[4])
     return None
 
 
@@ -288,6 +296,23 @@ def main() -> int:
     build_s = time.perf_counter() - t1
     info = ix.info()
     log(f"built index: {info['n_leaves']} leaves, {build_s:.1f} s, {info['device_bytes'] / 1e9:.1f} GB")
+
+    # launch shape: the autotuner's choice for the path on this index, from
+    # sample queries of their own (not the timed ones)
+    tuning = {}
+    if B == 0:
+        rt = np.random.default_rng(6)
+        m = args.tune_queries
+        sigt = rt.uniform(0.0, args.sigma_max, m).astype(np.float32)
+        kt = rt.integers(1, args.kmax + 1, m).astype(np.uint32)
+        Qt, _ = hkg.gen_noisy_queries(0, 1, N, 6, sigt, L)
+        t2 = time.perf_counter()
+        tuning = ix.autotune(Qt, kt, path=args.path)
+        tuning["queries"] = m
+        tuning["seconds"] = round(time.perf_counter() - t2, 2)
+        B = tuning["batch"]
+        del Qt
+        log(f"autotuned {args.path} on {m} sample queries: {tuning}")
 
     steps, warm = args.steps, args.warmup
     total_q = (steps + warm) * B
@@ -472,13 +497,13 @@ def main() -> int:
                    "build_s": round(build_s, 2)},
         "roofline": ({"bound": "mfma", "kernel": kernel_names.get(dom, dom), "achieved": achieved_tf,
                       "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved_tf / MFMA_BF16_PEAK_TFLOPS,
-                      "traffic": traffic_bytes(kernel_names.get(dom, dom)), "avg_launch_ms": avg_ms,
+                      "traffic": traffic_bytes(kernel_names.get(dom, dom), N, L, B), "avg_launch_ms": avg_ms,
                       "flops_per_launch": flops_per_launch, "tile_bytes_per_launch": bytes_per_launch,
                       "tile_bytes_gbs": achieved}
                      if mfma else
                      {"bound": "hbm", "kernel": kernel_names.get(dom, dom), "achieved": achieved,
                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                      "traffic": traffic_bytes(kernel_names.get(dom, dom)), "avg_launch_ms": avg_ms,
+                      "traffic": traffic_bytes(kernel_names.get(dom, dom), N, L, B), "avg_launch_ms": avg_ms,
                       "bytes_per_launch": bytes_per_launch}),
         "stages_ms": {k: round(v / steps, 3) for k, v in stage_ms.items() if v > 0},
         "series_refined_per_query": refined / (steps * B),
@@ -487,6 +512,7 @@ def main() -> int:
         "modes": modes,
         "real_queries": real,
         "loader": loader,
+        "tuning": tuning,
         "serving": serving,
         "cpu_baseline": cpu,
     }
