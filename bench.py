@@ -385,9 +385,20 @@ def main() -> int:
         a = ix.knn(Qs, ks, path=args.path)
         b = ix.knn(Qs, ks, path="scan")
         same = bool(np.array_equal(a.ids, b.ids) and np.array_equal(a.d2.view(np.uint32), b.d2.view(np.uint32)))
-        zero = [int(i) for i in range(npar) if sig[i] == 0.0]
-        par = {"index_vs_full_scan_identical": same, "queries_checked": npar}
-        log(f"parity index vs scan on {npar} queries: {same}")
+        # error of every returned d2 against the FP64 distance to the row with
+        # that id (the row regenerated on the device from (seed, id))
+        worst, nres = 0.0, 0
+        for i in range(npar):
+            q64 = Qs[i].astype(np.float64)
+            for r in range(int(ks[i])):
+                x64 = hkg.gen_series(0, 1, int(a.ids[i, r]), 1, L)[0].astype(np.float64)
+                e64 = float(np.sum((q64 - x64) ** 2))
+                worst = max(worst, abs(float(a.d2[i, r]) - e64) / max(e64, 1e-30))
+                nres += 1
+        par = {"index_vs_full_scan_identical": same, "queries_checked": npar,
+               "fp64_max_rel_error": worst, "results_checked_fp64": nres}
+        log(f"parity index vs scan on {npar} queries: {same}; worst FP64 relative error of {nres} results: "
+            f"{worst:.3g}")
     if args.row_probes:
         # every probed dataset row is in the index: found at distance 0 under its own id
         rows = np.unique(np.concatenate([np.linspace(0, N - 1, args.row_probes).astype(np.int64),

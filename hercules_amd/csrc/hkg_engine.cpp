@@ -1053,10 +1053,14 @@ hkg_status Index::knn_host(const float *Q, uint32_t nq, const uint32_t *k, uint3
 }
 
 // Launch-shape tuner: each candidate (queries per batch x filter grid) runs
-// the sample twice and keeps its faster device time; the default shape runs
-// first, after one untimed warm-up call (BF16 copy, buffer sets, code
-// objects).  Answers must not depend on the shape: any difference is an
+// the sample three times and keeps its fastest device time; the default
+// shape runs first, after one untimed warm-up call (BF16 copy, buffer sets,
+// code objects).  Among shapes within kTuneMargin of the fastest, the one
+// with the largest batch is kept: repeated runs of one shape spread by about
+// that much, and a larger batch spreads its per-batch work over more
+// queries.  Answers must not depend on the shape: any difference is an
 // engine fault, reported instead of tuning.
+constexpr double kTuneMargin = 0.01;
 hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint32_t kmax, uint32_t path_in,
                            hkg_tuning *out)
 {
@@ -1108,7 +1112,7 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
     qo.batch = batch;
     call_settings().fblocks = grid;
     *ms = 1e300;
-    for (int rep = 0; rep < 2; ++rep) {
+    for (int rep = 0; rep < 3; ++rep) {
       hkg_query_stats st;
       hkg_status rc = knn_host(Q, nq, k, kmax, &qo, I, D, &st);
       if (rc != HKG_OK) return rc;
@@ -1120,8 +1124,12 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
   hkg_status rc = run(def_batch, def_grid, ids0.data(), d20.data(), &ms);  // warm-up
   if (rc != HKG_OK) return rc;
   if ((rc = run(def_batch, def_grid, ids0.data(), d20.data(), &ms_def)) != HKG_OK) return rc;
-  uint32_t best_b = def_batch, best_g = def_grid, ncand = 1;
-  double best = ms_def;
+  struct Timed {
+    uint32_t b, g;
+    double ms;
+  };
+  std::vector<Timed> timed{{def_batch, def_grid, ms_def}};
+  uint32_t ncand = 1;
   for (uint32_t b : bs) {
     for (uint32_t g : grids) {
       if (b == def_batch && g == def_grid) continue;
@@ -1132,11 +1140,19 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
                   std::to_string(g) + " blocks per CU)");
         return HKG_EINTERNAL;
       }
-      if (ms < best) {
-        best = ms;
-        best_b = b;
-        best_g = g;
-      }
+      timed.push_back({b, g, ms});
+    }
+  }
+  double fastest = ms_def;
+  for (const Timed &t : timed) fastest = std::min(fastest, t.ms);
+  uint32_t best_b = 0, best_g = 0;
+  double best = 0.0;
+  for (const Timed &t : timed) {
+    if (t.ms > fastest * (1.0 + kTuneMargin)) continue;
+    if (best_b == 0 || t.b > best_b || (t.b == best_b && t.ms < best)) {
+      best_b = t.b;
+      best_g = t.g;
+      best = t.ms;
     }
   }
   tuned_batch[path].store(best_b, std::memory_order_relaxed);

@@ -217,7 +217,8 @@ typedef struct hkg_tuning {
 } hkg_tuning;
 
 /* Time the candidate shapes of a path on nq sample queries (host rows, k as
-   in hkg_knn), keep the fastest in the handle and return it.  Later calls
+   in hkg_knn), keep the fastest in the handle (of shapes within 1% of the
+   fastest, the one with the largest batch) and return it.  Later calls
    on that path with hkg_query_opts.batch = 0 use the tuned batch; the FP32
    index paths and the modes use the tuned grid.  Not to be called while
    other calls on the same handle run. */

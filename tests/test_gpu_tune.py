@@ -45,7 +45,8 @@ def test_autotune_paths(hkg, oracle, kind, L):
             else:
                 assert t["filter_blocks_per_cu"] == 0
             assert t["candidates"] >= 1 and t["ms_per_query"] > 0 and t["ms_per_query_default"] > 0
-            assert t["ms_per_query"] <= t["ms_per_query_default"]
+            # the largest batch within 1% of the fastest shape is kept
+            assert t["ms_per_query"] <= t["ms_per_query_default"] * 1.01
             after = ix.knn(Q[:m], kv[:m], path=path)
             assert np.array_equal(after.d2.view(np.uint32), before.d2.view(np.uint32)), path
             assert np.array_equal(after.ids, before.ids), path
