@@ -488,6 +488,9 @@ def main() -> int:
     cpu = None if args.no_cpu_baseline else cpu_baseline(args, Qh, kq)
     if cpu:
         log(f"cpu baseline: {cpu}")
+    tb = traffic_bytes(kernel_names.get(dom, dom), N, L, B)
+    # measured HBM rate of the dominant kernel: counter bytes per launch / launch time
+    traffic_gbs = tb / (avg_ms * 1e6) if tb and avg_ms else None
     out = {
         "metric": "exact k-NN queries/s at 100M\u00d7len256; achieved HBM GB/s; p50/p99 latency",
         "value": qps,
@@ -508,13 +511,13 @@ def main() -> int:
                    "build_s": round(build_s, 2)},
         "roofline": ({"bound": "mfma", "kernel": kernel_names.get(dom, dom), "achieved": achieved_tf,
                       "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved_tf / MFMA_BF16_PEAK_TFLOPS,
-                      "traffic": traffic_bytes(kernel_names.get(dom, dom), N, L, B), "avg_launch_ms": avg_ms,
+                      "traffic": tb, "traffic_gbs": traffic_gbs, "avg_launch_ms": avg_ms,
                       "flops_per_launch": flops_per_launch, "tile_bytes_per_launch": bytes_per_launch,
                       "tile_bytes_gbs": achieved}
                      if mfma else
                      {"bound": "hbm", "kernel": kernel_names.get(dom, dom), "achieved": achieved,
                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                      "traffic": traffic_bytes(kernel_names.get(dom, dom), N, L, B), "avg_launch_ms": avg_ms,
+                      "traffic": tb, "traffic_gbs": traffic_gbs, "avg_launch_ms": avg_ms,
                       "bytes_per_launch": bytes_per_launch}),
         "stages_ms": {k: round(v / steps, 3) for k, v in stage_ms.items() if v > 0},
         "series_refined_per_query": refined / (steps * B),
