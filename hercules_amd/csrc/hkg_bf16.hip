@@ -774,23 +774,33 @@ __global__ void k_bf16_thr_init(uint32_t nq, const uint32_t *__restrict__ bound,
 
 // leaves some query of a block group still needs: EAPCA bound within the
 // query's current k-th distance bound
+// pairs (nullable): also count every (query, leaf) pair of the group whose
+// bound is within the query's k-th bound (then no early exit per leaf)
 __global__ void k_group_leaf_need(uint32_t nq, uint32_t qpb, uint32_t nleaf, const float *__restrict__ lb,
                                   const uint32_t *__restrict__ bound, const uint32_t *__restrict__ qperm,
-                                  uint8_t *__restrict__ need)
+                                  uint8_t *__restrict__ need, unsigned long long *__restrict__ pairs)
 {
   const uint32_t leaf = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t g = blockIdx.y;
-  if (leaf >= nleaf) return;
-  uint8_t nd = 0;
-  const uint32_t s0 = g * qpb;
-  const uint32_t s1 = min(nq, s0 + qpb);
-  for (uint32_t sl = s0; sl < s1 && !nd; ++sl) {
-    const uint32_t q = qperm[sl];
-    const uint32_t bb = bound[q];
-    const float v = lb[(uint64_t)q * nleaf + leaf];
-    nd = (bb >= kInfBits || v <= __uint_as_float(bb) * (1.0f + kLbRel) + kLbAbs) ? 1 : 0;
+  uint32_t cnt = 0;
+  if (leaf < nleaf) {
+    uint8_t nd = 0;
+    const uint32_t s0 = g * qpb;
+    const uint32_t s1 = min(nq, s0 + qpb);
+    for (uint32_t sl = s0; sl < s1 && (pairs || !nd); ++sl) {
+      const uint32_t q = qperm[sl];
+      const uint32_t bb = bound[q];
+      const float v = lb[(uint64_t)q * nleaf + leaf];
+      const bool in = bb >= kInfBits || v <= __uint_as_float(bb) * (1.0f + kLbRel) + kLbAbs;
+      nd |= in ? 1 : 0;
+      cnt += in ? 1u : 0u;
+    }
+    need[(uint64_t)g * nleaf + leaf] = nd;
   }
-  need[(uint64_t)g * nleaf + leaf] = nd;
+  if (pairs) {
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off);
+    if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(pairs, (unsigned long long)cnt);
+  }
 }
 
 // row tiles of each block group: tiles overlapping a needed leaf
@@ -953,6 +963,7 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
   if (seed_host) HKG_HIP_TRY(hipMemcpyAsync(b->seed, seed_host, (size_t)nq * 4, hipMemcpyHostToDevice, st));
   launch_reset(st, nq, b->bound, b->cand_cnt, seed_host ? b->seed : nullptr);
   launch_fill(st, b->misc, 8, 0);
+  launch_check_k(st, nq, kmax, b->kq, b->misc + 3);
   launch_fill(st, reinterpret_cast<uint32_t *>(b->counters), 2 * kNumCounters, 0);
   launch_fill(st, b->bf_cnt, nq, 0);
   if (pf) pf->mark(st, -1);
@@ -977,15 +988,16 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
   float *thr0 = reinterpret_cast<float *>(b->thr);
   constexpr bool kSharedList = true;
   const uint32_t lists = kSharedList ? 1 : groups, list_qpb = kSharedList ? nq : qpb;
-  auto plan_tiles = [&]() {
+  // the last plan also counts the (query, leaf) pairs it keeps
+  auto plan_tiles = [&](bool count) {
     launch_fill(st, b->ntiles_g, lists, 0);
     hipLaunchKernelGGL(k_group_leaf_need, dim3((nleaf + 255) / 256, lists), dim3(256), 0, st, nq, list_qpb, nleaf,
-                       b->lb, b->bound, b->qperm, b->need);
+                       b->lb, b->bound, b->qperm, b->need, count ? b->counters + kCtrLeafPairs : nullptr);
     hipLaunchKernelGGL(k_group_tiles, dim3((dev.ntiles + 255) / 256, lists), dim3(256), 0, st, dev.ntiles, nleaf,
                        dev.tile_leaf0, dev.tile_leaf1, b->need, b->tiles, b->ntiles_g);
     hipLaunchKernelGGL(k_bf16_thr_init, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->bound, thr0);
   };
-  plan_tiles();
+  plan_tiles(!bound_pass);
   if (pf) pf->mark(st, HKG_STAGE_COLLECT);
   std::vector<uint32_t> nt_pass1;
   if (bound_pass) {
@@ -999,7 +1011,7 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
       HKG_HIP_TRY(hipMemcpyAsync(nt_pass1.data(), b->ntiles_g, lists * 4, hipMemcpyDeviceToHost, st));
     }
     hipLaunchKernelGGL(k_bf16_tau, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->kq, b->hist, thr0, b->bound);
-    plan_tiles();
+    plan_tiles(true);
     if (pf) pf->mark(st, HKG_STAGE_BOUND);
   }
   // pass 2: collect the survivors of the certified lower bound
@@ -1013,6 +1025,7 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
   if (ctr) {
     HKG_HIP_TRY(hipMemcpyAsync(ctr->v, b->counters, sizeof(ctr->v), hipMemcpyDeviceToHost, st));
     HKG_HIP_TRY(hipMemcpyAsync(&ctr->overflow, overflow, 4, hipMemcpyDeviceToHost, st));
+    HKG_HIP_TRY(hipMemcpyAsync(&ctr->bad_k, b->misc + 3, 4, hipMemcpyDeviceToHost, st));
     std::vector<uint32_t> nt(lists);
     HKG_HIP_TRY(hipMemcpyAsync(nt.data(), b->ntiles_g, lists * 4, hipMemcpyDeviceToHost, st));
     HKG_HIP_TRY(hipStreamSynchronize(st));

@@ -392,13 +392,14 @@ hkg_status Index::get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuff
 {
   const uint32_t L = dev.L, nleaf = dev.nleaf;
   B = std::max(B, call_settings().min_buffer_batch);
-  const uint64_t key = ((uint64_t)B << 40) ^ ((uint64_t)kmax << 20) ^ cap;
+  const auto key = std::make_tuple(B, kmax, cap);
   std::lock_guard<std::mutex> lock(mu_bufs);
   // a cached set of the same shape that no call holds (an overflow re-run
   // of a whole batch asks for the shape of its caller)
   auto range = bufs.equal_range(key);
   for (auto it = range.first; it != range.second; ++it) {
-    if (!it->second->busy) {
+    const QueryBuffers &c = *it->second;
+    if (!c.busy && c.B == B && c.kmax == kmax && c.cap == cap) {
       it->second->busy = true;
       *out = it->second.get();
       return HKG_OK;
@@ -526,6 +527,7 @@ void Index::enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Pr
   uint32_t *nwork = b.misc + 0, *work_next = b.misc + 1, *overflow = b.misc + 2;
   launch_reset(st, nq, b.bound, b.cand_cnt, seeded ? b.seed : nullptr);
   launch_fill(st, b.misc, 8, 0);
+  launch_check_k(st, nq, b.kmax, b.kq, b.misc + 3);
   launch_fill(st, reinterpret_cast<uint32_t *>(b.counters), 2 * kNumCounters, 0);
   if (pf) pf->mark(st, -1);
   launch_prep(st, b.Q, nq, L, b.paa, b.psum, b.qn2);
@@ -571,6 +573,7 @@ hkg_status Index::enqueue_mode_batch(QueryBuffers &b, uint32_t nq, hipStream_t s
   uint32_t *nwork = b.misc + 0, *work_next = b.misc + 1, *overflow = b.misc + 2;
   launch_reset(st, nq, b.bound, b.cand_cnt, seeded ? b.seed : nullptr);
   launch_fill(st, b.misc, 8, 0);
+  launch_check_k(st, nq, b.kmax, b.kq, b.misc + 3);
   launch_fill(st, reinterpret_cast<uint32_t *>(b.counters), 2 * kNumCounters, 0);
   if (pf) pf->mark(st, -1);
   launch_prep(st, b.Q, nq, L, b.paa, b.psum, b.qn2);
@@ -634,6 +637,7 @@ void Index::enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Pro
   uint32_t *overflow = b.misc + 2;
   launch_reset(st, nq, b.bound, b.cand_cnt, nullptr);
   launch_fill(st, b.misc, 8, 0);
+  launch_check_k(st, nq, b.kmax, b.kq, b.misc + 3);
   launch_fill(st, reinterpret_cast<uint32_t *>(b.counters), 2 * kNumCounters, 0);
   if (pf) pf->mark(st, -1);
   launch_scan(st, scan_grid_x(nq), dev.X, dev.n, dev.L, b.Q, nq, b.kq, dev.ids, b.cap, b.cand, b.cand_cnt,
@@ -698,6 +702,7 @@ hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, 
     HKG_HIP_TRY(hipMemcpyAsync(ctr->v, b->counters, sizeof(ctr->v), hipMemcpyDeviceToHost, st));
     HKG_HIP_TRY(hipMemcpyAsync(&ctr->overflow, b->misc + 2, 4, hipMemcpyDeviceToHost, st));
     HKG_HIP_TRY(hipMemcpyAsync(&ctr->nwork, b->misc + 0, 4, hipMemcpyDeviceToHost, st));
+    HKG_HIP_TRY(hipMemcpyAsync(&ctr->bad_k, b->misc + 3, 4, hipMemcpyDeviceToHost, st));
   }
   *used = b;
   lease.b = nullptr;
@@ -858,6 +863,10 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     return HKG_EINVAL;
   }
   if (nq == 0) return HKG_OK;
+  if ((uint32_t)o.path > (uint32_t)HKG_PATH_BF16) {
+    set_error("unknown query path " + std::to_string((uint32_t)o.path));
+    return HKG_EINVAL;
+  }
   const bool mode = o.nprobes > 0 || o.epsilon != 0.0f;
   if (!(o.epsilon >= 0.0f) || std::isinf(o.epsilon) || std::isnan(o.eapca_threshold) ||
       std::isnan(o.sax_threshold)) {
@@ -924,6 +933,10 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     if (rc != HKG_OK) return rc;
     BufLease lease{this, b};
     HKG_HIP_TRY(hipStreamSynchronize(st));
+    if (c.bad_k) {
+      set_error(std::to_string(c.bad_k) + " per-query k value(s) outside [1, kmax = " + std::to_string(kmax) + "]");
+      return HKG_EINVAL;
+    }
     ++launches;
     if (pf) pf->accumulate(stage_ms, stage_launches);
     std::vector<uint32_t> dbg_cnt, dbg_thr;
@@ -963,7 +976,11 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     stats->queries = nq;
     const bool scan = path == HKG_PATH_SCAN;
     stats->leaves_total = (uint64_t)nq * dev.nleaf;
-    stats->leaves_after_eapca = scan ? 0 : total.v[kCtrLeaves] + nq;
+    // pairs within the k-th bound after the approximate stage, the leaves
+    // that stage scanned included
+    stats->leaves_after_eapca =
+        scan ? 0
+             : (path == HKG_PATH_BF16 ? total.v[kCtrLeafPairs] : total.nwork + total.v[kCtrApproxLeaves]);
     stats->series_total = (uint64_t)nq * dev.n;
     stats->series_lb_checked = scan ? 0 : total.v[kCtrSaxChecked] + total.v[kCtrBf16];
     stats->bf16_pairs = total.v[kCtrBf16];

@@ -6,6 +6,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "hkg_internal.h"
@@ -19,6 +20,7 @@ struct Counters {
   unsigned long long v[kNumCounters] = {};
   uint32_t overflow = 0;
   uint32_t nwork = 0;
+  uint32_t bad_k = 0;        // per-query k outside [1, kmax] (clamped on the device)
   uint64_t bf16_rows = 0;    // rows the BF16 collecting pass read (summed over block groups)
   uint64_t bf16_rows_bound = 0;  // rows the BF16 histogram pass read
 };
@@ -127,7 +129,8 @@ struct Index {
   // streams and nested re-runs never share one.
   std::mutex mu_bufs, mu_bf16;
   bool bf16_ready = false;      // BF16 copy, norms and tile tables built (guarded by mu_bf16)
-  std::multimap<uint64_t, std::unique_ptr<QueryBuffers>> bufs;
+  // key: (queries, kmax, candidate entries per query)
+  std::multimap<std::tuple<uint32_t, uint32_t, uint32_t>, std::unique_ptr<QueryBuffers>> bufs;
   // launch shapes chosen by autotune (0: the defaults): queries per batch
   // of each path, and k_filter_refine blocks per CU
   std::atomic<uint32_t> tuned_batch[HKG_PATH_BF16 + 1] = {};

@@ -28,6 +28,8 @@ enum Counter : int {
   kCtrModeCand = 6,    // approximate modes: candidate leaves
   kCtrModeScanned = 7, // approximate modes: candidate leaves scanned or refined
   kCtrModeSeq = 8,     // approximate modes: queries whose candidates went sequential
+  kCtrApproxLeaves = 9,  // (query, leaf) pairs scanned in the approximate stage
+  kCtrLeafPairs = 10,  // BF16 path: (query, leaf) pairs within the k-th bound after the approximate stage
   kNumCounters = 12
 };
 
@@ -46,6 +48,9 @@ bool mode_heap_in_lds(uint32_t L, uint32_t nnodes);
 hipError_t upload_breakpoints_dev();
 
 void launch_reset(hipStream_t st, uint32_t nq, uint32_t *bound, uint32_t *cand_cnt, const uint32_t *seed);
+// per-query k in place: values outside [1, kmax] are clamped (so no kernel
+// sees them) and counted in *bad, which the host turns into HKG_EINVAL
+void launch_check_k(hipStream_t st, uint32_t nq, uint32_t kmax, uint32_t *kq, uint32_t *bad);
 void launch_fill(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v);
 void launch_prep(hipStream_t st, const float *Q, uint32_t nq, uint32_t L, float *paa, double *psum,
                  double *qn2);

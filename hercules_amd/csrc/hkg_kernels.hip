@@ -431,7 +431,10 @@ __global__ void __launch_bounds__(kBlock) k_scan_best(const float *__restrict__ 
     topk_merge(tk, scan, k);
   }
   flush_topk(tk, k, q, cap, cand, cand_cnt, bound);
-  if (threadIdx.x == 0) atomicAdd(&counters[kCtrApprox], (unsigned long long)size);
+  if (threadIdx.x == 0) {
+    atomicAdd(&counters[kCtrApprox], (unsigned long long)size);
+    atomicAdd(&counters[kCtrApproxLeaves], 1ull);
+  }
 }
 
 // Approximate phase of the BF16 path: in each of the query's best leaves the
@@ -546,7 +549,10 @@ __global__ void __launch_bounds__(kBlock) k_scan_best_bf16(const float *__restri
   __syncthreads();
   topk_merge(tk, scan, k);
   flush_topk(tk, k, q, cap, cand, cand_cnt, bound);
-  if (threadIdx.x == 0) atomicAdd(&counters[kCtrApprox], (unsigned long long)size);
+  if (threadIdx.x == 0) {
+    atomicAdd(&counters[kCtrApprox], (unsigned long long)size);
+    atomicAdd(&counters[kCtrApproxLeaves], 1ull);
+  }
 }
 
 // dense work list of every (query, leaf) pair: key = EAPCA bound bits when
@@ -980,6 +986,15 @@ __global__ void k_reset_queries(uint32_t nq, uint32_t *__restrict__ bound, uint3
   }
 }
 
+__global__ void k_check_k(uint32_t nq, uint32_t kmax, uint32_t *__restrict__ kq, uint32_t *__restrict__ bad)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool out = i < nq && (kq[i] == 0 || kq[i] > kmax);
+  if (out) kq[i] = kq[i] == 0 ? 1u : kmax;
+  const uint64_t m = __ballot(out);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(bad, (uint32_t)__popcll(m));
+}
+
 // small fill kernel (used instead of hipMemsetAsync, which must not be
 // recorded into graphs)
 __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t n, uint32_t v)
@@ -994,6 +1009,11 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t n, uint32_t v)
 void launch_reset(hipStream_t st, uint32_t nq, uint32_t *bound, uint32_t *cand_cnt, const uint32_t *seed)
 {
   hipLaunchKernelGGL(k_reset_queries, dim3((nq + 255) / 256), dim3(256), 0, st, nq, bound, cand_cnt, seed);
+}
+
+void launch_check_k(hipStream_t st, uint32_t nq, uint32_t kmax, uint32_t *kq, uint32_t *bad)
+{
+  hipLaunchKernelGGL(k_check_k, dim3((nq + 255) / 256), dim3(256), 0, st, nq, kmax, kq, bad);
 }
 
 void launch_fill(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v)

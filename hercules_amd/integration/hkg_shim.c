@@ -1,9 +1,11 @@
 /* hkg_shim.c -- the reference's query functions on the MI355X engine:
- * exact_de_knn_search_psq_lf_sims_adaptive (exact and epsilon search) and
- * exact_ng_knn_search (nprobes).
+ * exact_de_knn_search_psq_lf_sims_adaptive (exact and epsilon search),
+ * exact_ng_knn_search (nprobes), and the exact search of the two siblings the
+ * reference dispatches to without --sims (exact_de_knn_search_psq_lf) and
+ * with --num-threads 1 (exact_de_knn_search).
  *
  * Compiled with the reference's headers and linked into the reference's own
- * main (hercules.c) in place of the library's definitions of these two
+ * main (hercules.c) in place of the library's definitions of these four
  * functions; every other reference function stays as it is.  The index
  * directory is the one the reference loaded (index->settings->root_directory,
  * written with --serial 1 --sims); the engine uploads it to HBM once, on the
@@ -87,5 +89,52 @@ void exact_ng_knn_search(ts_type *query_ts, ts_type *query_ts_reordered, int *qu
   hkg_query_opts qo;
   hkg_default_query_opts(&qo);
   qo.nprobes = nprobes;
+  hkg_answer(index, query_ts, k, q_id, qfilename, &qo);
+}
+
+/* The siblings answer the same exact k-NN; their epsilon searches traverse
+   the tree in their own orders (approximate_knn_search, :176-222), which the
+   engine does not replay, so epsilon > 0 is refused rather than answered on
+   the CPU or with another traversal's answers. */
+static void hkg_exact_only(const char *fn, ts_type epsilon)
+{
+  if (epsilon != 0.0f) {
+    fprintf(stderr, "hkg: %s: --epsilon %g is not supported on this path; run with --sims and more than one "
+                    "thread (the epsilon search of exact_de_knn_search_psq_lf_sims_adaptive)\n",
+            fn, (double)epsilon);
+    exit(-1);                        /* the reference's convention for fatal errors */
+  }
+}
+
+/* hercules_query_engine.c:2215-2448 (include/hercules_query_engine.h:203-207):
+   the serial exact search, called for --num-threads 1
+   (hercules_file_loaders.c:239-244) */
+void exact_de_knn_search(ts_type *query_ts, ts_type *query_ts_reordered, int *query_order, unsigned int offset,
+                         struct hercules_index *index, ts_type minimum_distance, ts_type epsilon, ts_type r_delta,
+                         unsigned int k, unsigned int q_id, char *qfilename, query_settings q_settings)
+{
+  (void)query_ts_reordered; (void)query_order; (void)offset; (void)minimum_distance; (void)r_delta;
+  (void)q_settings;
+  hkg_exact_only("exact_de_knn_search", epsilon);
+  hkg_query_opts qo;
+  hkg_default_query_opts(&qo);
+  qo.path = HKG_PATH_INDEX;
+  hkg_answer(index, query_ts, k, q_id, qfilename, &qo);
+}
+
+/* hercules_query_engine.c:1398-1690 (include/hercules_query_engine.h:256-261):
+   the multithreaded exact search without the SIMS cache, called without
+   --sims (hercules_file_loaders.c:258-262) */
+void exact_de_knn_search_psq_lf(ts_type *query_ts, ts_type *query_ts_reordered, int *query_order,
+                                unsigned int offset, struct hercules_index *index, ts_type minimum_distance,
+                                ts_type epsilon, double delta, unsigned int k, unsigned int q_id, char *qfilename,
+                                query_settings q_settings)
+{
+  (void)query_ts_reordered; (void)query_order; (void)offset; (void)minimum_distance; (void)delta;
+  (void)q_settings;
+  hkg_exact_only("exact_de_knn_search_psq_lf", epsilon);
+  hkg_query_opts qo;
+  hkg_default_query_opts(&qo);
+  qo.path = HKG_PATH_INDEX;
   hkg_answer(index, query_ts, k, q_id, qfilename, &qo);
 }

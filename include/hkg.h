@@ -13,7 +13,9 @@
  *                     exact_de_knn_search_psq_lf_sims_adaptive
  *                     (code/hercules/include/hercules_query_engine.h:203-207,
  *                     256-261, 284-289), batched over queries and returning
- *                     the results instead of printing them
+ *                     the results instead of printing them; the drop-in
+ *                     definitions of all three (and of exact_ng_knn_search)
+ *                     are hercules_amd/integration/hkg_shim.c
  *   hkg_open_raw +
  *   hkg_knn(HKG_PATH_SCAN)  PSCAN's parallel_scan / parallel_scan_process_query
  *                     (code/pscan/src/parallel_scan_utils.c:23-347)
@@ -126,7 +128,10 @@ enum {
 typedef struct hkg_query_stats {
   uint64_t queries;
   uint64_t leaves_total;          /* per query, summed over queries            */
-  uint64_t leaves_after_eapca;    /* candidate leaves (1st-level survivors)    */
+  uint64_t leaves_after_eapca;    /* (query, leaf) pairs whose EAPCA bound is
+                                     within the query's k-th distance bound
+                                     after the approximate stage, the leaves
+                                     that stage scanned included             */
   uint64_t series_total;
   uint64_t series_lb_checked;     /* series whose lower bound was evaluated    */
   uint64_t series_refined;        /* series whose exact distance was computed  */
@@ -191,7 +196,9 @@ hkg_status hkg_knn(hkg_index *ix, const float *Q, uint32_t nq, const uint32_t *k
 
 /* Same, with Q, k, ids (uint64) and d2 already in device memory; runs on
    the given HIP stream (cast to void*; NULL = the handle's stream) and
-   synchronises it before returning. */
+   synchronises it before returning.  Every dk[i] must be in [1, kmax]: the
+   device checks them and the call returns HKG_EINVAL otherwise (the
+   outputs are then undefined). */
 hkg_status hkg_knn_device(hkg_index *ix, const float *dQ, uint32_t nq, const uint32_t *dk,
                           uint32_t kmax, const hkg_query_opts *qo, uint64_t *dids, float *dd2,
                           void *stream, hkg_query_stats *stats);

@@ -42,3 +42,31 @@ def test_reference_main_through_the_shim(oracle, tmp_path):
     d2, _, _, _ = rr.query(str(d), qf, Q.shape[0], m["L"], c["k"], threads=4, binary=rr.DROPIN, timeout=300,
                            extra=["--nprobes", c["nprobes"]])
     assert np.array_equal(d2.view(np.uint32), gd.mode_answers("tiny")[f"m{i}"].view(np.uint32))
+
+
+@pytest.mark.skipif(not os.path.exists(rr.DROPIN), reason="oracle/_ref/hercules_gpu not built (make -C oracle dropin)")
+def test_sibling_query_functions_through_the_shim(oracle, tmp_path):
+    """--num-threads 1 dispatches to exact_de_knn_search and a run without
+    --sims to exact_de_knn_search_psq_lf (hercules_file_loaders.c:239-262);
+    both reach the engine through the shim and print the reference's exact
+    answers.  Their epsilon searches are refused, not answered."""
+    d = tmp_path / "idx"
+    gd.reference_index(oracle, "tiny", str(d))
+    m = gd.meta("tiny")
+    Q = gd.queries(oracle, m)
+    qf = str(tmp_path / "q.bin")
+    rr.write_f32(qf, Q)
+    a = gd.answers("tiny")
+    print("drop-in: --num-threads 1, k = 1 and 10", flush=True)
+    d2, _, _, _ = rr.query(str(d), qf, Q.shape[0], m["L"], 1, threads=1, binary=rr.DROPIN, timeout=300)
+    assert np.array_equal(d2.view(np.uint32), a["ref_serial_d2_k1"].view(np.uint32))
+    d2, _, _, _ = rr.query(str(d), qf, Q.shape[0], m["L"], 10, threads=1, binary=rr.DROPIN, timeout=300)
+    assert np.array_equal(d2.view(np.uint32), a["ref_sims_d2_k10"].view(np.uint32))
+    print("drop-in: no --sims, 4 threads, k = 10", flush=True)
+    d2, _, _, _ = rr.query(str(d), qf, Q.shape[0], m["L"], 10, threads=4, sims=False, binary=rr.DROPIN,
+                           timeout=300)
+    assert np.array_equal(d2.view(np.uint32), a["ref_sims_d2_k10"].view(np.uint32))
+    print("drop-in: --num-threads 1 --epsilon 1 is refused", flush=True)
+    with pytest.raises(RuntimeError, match="not supported on this path"):
+        rr.query(str(d), qf, Q.shape[0], m["L"], 10, threads=1, binary=rr.DROPIN, timeout=300,
+                 extra=["--epsilon", 1.0])
