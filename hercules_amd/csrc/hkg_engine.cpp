@@ -154,7 +154,7 @@ hkg_status Index::init_device(int device)
   HKG_HIP_TRY(hipSetDevice(device));
   HKG_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   HKG_HIP_TRY(upload_breakpoints_dev());
-  return HKG_OK;
+  return noalloc_setup(device);
 }
 
 // leaf tables (CSR of segments) from a host tree

@@ -193,6 +193,10 @@ struct BufLease {
   }
 };
 
+// device address of the no-allocation entry point's scratch array, looked
+// up once per device (hkg_noalloc.hip)
+hkg_status noalloc_setup(int device);
+
 hkg_status open_dir(const char *dir, const hkg_options *opt, Index **out);
 hkg_status open_raw(const float *X, uint64_t n, uint32_t L, const hkg_options *opt, Index **out);
 uint32_t scan_grid_x(uint32_t nq);

@@ -117,6 +117,7 @@ SIGNATURES = {
     "hkg_close": (None, [_P]),
     "hkg_knn": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P]),
     "hkg_knn_device": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P]),
+    "hkg_scan_knn_device_noalloc": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, _P, _P]),
     "hkg_autotune": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, _P]),
     "hkg_default_build_opts": (None, [_P]),
     "hkg_build": (C.c_int, [_P, C.c_uint64, C.c_uint32, _P, C.POINTER(_P)]),
@@ -361,6 +362,12 @@ class HerculesIndex:
         _check(lib().hkg_knn_device(self._h, dQ.ptr, nq, dk.ptr, kmax, C.byref(qo), dids.ptr, dd2.ptr, None,
                                     C.byref(st)))
         return st.as_dict()
+
+    def scan_knn_device_noalloc(self, dQ: DeviceArray, nq: int, dk: DeviceArray, kmax: int, dids: DeviceArray,
+                                dd2: DeviceArray) -> None:
+        """The full scan through hkg_scan_knn_device_noalloc: no device
+        allocation (scratch in the library's own device array)."""
+        _check(lib().hkg_scan_knn_device_noalloc(self._h, dQ.ptr, nq, dk.ptr, kmax, dids.ptr, dd2.ptr, None))
 
 
 def gen_series_device(kind: int, seed: int, first: int, n: int, L: int, out: DeviceArray) -> None:

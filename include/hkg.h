@@ -203,6 +203,21 @@ hkg_status hkg_knn_device(hkg_index *ix, const float *dQ, uint32_t nq, const uin
                           uint32_t kmax, const hkg_query_opts *qo, uint64_t *dids, float *dd2,
                           void *stream, hkg_query_stats *stats);
 
+/* Exact k-NN by the full scan (the kernels of hkg_knn_device with path
+   HKG_PATH_SCAN, identical answers) that allocates nothing: no hipMalloc,
+   hipFree or allocating library call.  No counterpart in the reference; for
+   callers that cannot allocate device memory (code between HIP graph
+   captures, tight serving loops, hosts that forbid allocation).  Its scratch
+   is a 16 MiB device array inside the library (one per device, held until
+   the process ends, set up when an index is opened on the device); a call
+   whose scratch would not fit -- beyond 64 queries at kmax 256 on a 256-CU
+   MI355X -- returns HKG_EINVAL.  dQ (nq x L), dk (nq, each in [1, kmax]),
+   dids and dd2 (nq x kmax) are device memory; runs on `stream` (NULL = the
+   handle's stream) and synchronises it before returning.  Calls on one
+   device share the array and are serialised by a lock inside the library. */
+hkg_status hkg_scan_knn_device_noalloc(hkg_index *ix, const float *dQ, uint32_t nq, const uint32_t *dk,
+                                       uint32_t kmax, uint64_t *dids, float *dd2, void *stream);
+
 /* ---- launch-shape autotuner --------------------------------------------- */
 /* The reference has no tuner: its query-time shape is the thread counts set
    by hand (--num-threads / --num-read-threads, hercules.c:104-105,227;
