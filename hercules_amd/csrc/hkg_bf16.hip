@@ -916,7 +916,7 @@ hkg_status Index::make_bf16()
 
 hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax, hipStream_t st,
                                  QueryBuffers **used, Counters *ctr, Prof *pf, const uint32_t *seed_host,
-                                 bool bound_pass)
+                                 bool bound_pass, bool use_graph)
 {
   if (!dev.has_tree) {
     set_error("the BF16 path needs an index (its first bound comes from the best leaf)");
@@ -961,66 +961,79 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
   HKG_HIP_TRY(hipMemcpyAsync(b->kq, dk, (size_t)nq * 4, hipMemcpyDeviceToDevice, st));
   uint32_t *overflow = b->misc + 2;
   if (seed_host) HKG_HIP_TRY(hipMemcpyAsync(b->seed, seed_host, (size_t)nq * 4, hipMemcpyHostToDevice, st));
-  launch_reset(st, nq, b->bound, b->cand_cnt, seed_host ? b->seed : nullptr);
-  launch_fill(st, b->misc, 8, 0);
-  launch_check_k(st, nq, kmax, b->kq, b->misc + 3);
-  launch_fill(st, reinterpret_cast<uint32_t *>(b->counters), 2 * kNumCounters, 0);
-  launch_fill(st, b->bf_cnt, nq, 0);
-  if (pf) pf->mark(st, -1);
-  launch_prep(st, b->Q, nq, L, b->paa, b->psum, b->qn2);
-  launch_query_bf16(st, nq, L, b->Q, b->Qb, b->qst);
-  if (pf) pf->mark(st, HKG_STAGE_PREP);
-  launch_eapca(st, nq, nleaf, L, dev.leaf_seg_off, dev.seg_end, dev.seg_ind, b->psum, b->lb);
-  if (pf) pf->mark(st, HKG_STAGE_EAPCA);
-  launch_best_leaf(st, nq, nleaf, b->lb, b->best);
-  if (pf) pf->mark(st, HKG_STAGE_SELECT_LEAF);
-  launch_scan_best_bf16(st, nq, dev.X, dev.Xb, dev.xn2, dev.max_leaf, L, b->Q, b->kq, b->best, dev.leaf_pos,
-                        dev.leaf_size, dev.ids, b->cap, b->cand, b->cand_cnt, b->bound, b->counters);
-  if (pf) pf->mark(st, HKG_STAGE_APPROX);
   // batch order by first bound, leaves and row tiles each block group needs
   const uint32_t groups = (nq + qpb - 1) / qpb;
-  uint32_t *skeys = b->qperm + b->B, *sidx = b->qperm + 2 * b->B;
-  hipLaunchKernelGGL(k_iota_best, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->best, skeys, sidx);
-  HKG_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(b->bf_sort_tmp, b->bf_sort_bytes, skeys, b->qperm + 3 * b->B, sidx,
-                                                 b->qperm, (int)nq, 0, 32, st));
   // row tiles the batch needs under the current bounds (one list that
   // every block group walks), and the filter threshold of every query
   float *thr0 = reinterpret_cast<float *>(b->thr);
   constexpr bool kSharedList = true;
   const uint32_t lists = kSharedList ? 1 : groups, list_qpb = kSharedList ? nq : qpb;
-  // the last plan also counts the (query, leaf) pairs it keeps
-  auto plan_tiles = [&](bool count) {
-    launch_fill(st, b->ntiles_g, lists, 0);
-    hipLaunchKernelGGL(k_group_leaf_need, dim3((nleaf + 255) / 256, lists), dim3(256), 0, st, nq, list_qpb, nleaf,
-                       b->lb, b->bound, b->qperm, b->need, count ? b->counters + kCtrLeafPairs : nullptr);
-    hipLaunchKernelGGL(k_group_tiles, dim3((dev.ntiles + 255) / 256, lists), dim3(256), 0, st, dev.ntiles, nleaf,
-                       dev.tile_leaf0, dev.tile_leaf1, b->need, b->tiles, b->ntiles_g);
-    hipLaunchKernelGGL(k_bf16_thr_init, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->bound, thr0);
-  };
-  plan_tiles(!bound_pass);
-  if (pf) pf->mark(st, HKG_STAGE_COLLECT);
-  std::vector<uint32_t> nt_pass1;
-  if (bound_pass) {
-    // pass 1: histogram of the BF16 upper bounds -> a k-th distance bound
-    // within about one bin of the k-th smallest upper bound
-    launch_fill(st, b->hist, nq * kBf16Bins, 0);
-    launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, kSharedList, b->bf_cap,
-                       b->bf_surv, b->bf_cnt, thr0, b->hist, b->bf_region, b->bf_region_cnt, b->bf_prog);
-    if (ctr) {
-      nt_pass1.resize(lists);
-      HKG_HIP_TRY(hipMemcpyAsync(nt_pass1.data(), b->ntiles_g, lists * 4, hipMemcpyDeviceToHost, st));
+  // the first pass's tile count is kept on the device (misc[4]) for the
+  // counters read after the batch: one list
+  static_assert(kSharedList, "the pass-1 tile count has one slot");
+  // the whole stage sequence, eager or captured (no host reads inside)
+  auto enqueue = [&](Prof *p) -> hkg_status {
+    launch_reset(st, nq, b->bound, b->cand_cnt, seed_host ? b->seed : nullptr);
+    launch_fill(st, b->misc, 8, 0);
+    launch_check_k(st, nq, kmax, b->kq, b->misc + 3);
+    launch_fill(st, reinterpret_cast<uint32_t *>(b->counters), 2 * kNumCounters, 0);
+    launch_fill(st, b->bf_cnt, nq, 0);
+    if (p) p->mark(st, -1);
+    launch_prep(st, b->Q, nq, L, b->paa, b->psum, b->qn2);
+    launch_query_bf16(st, nq, L, b->Q, b->Qb, b->qst);
+    if (p) p->mark(st, HKG_STAGE_PREP);
+    launch_eapca(st, nq, nleaf, L, dev.leaf_seg_off, dev.seg_end, dev.seg_ind, b->psum, b->lb);
+    if (p) p->mark(st, HKG_STAGE_EAPCA);
+    launch_best_leaf(st, nq, nleaf, b->lb, b->best);
+    if (p) p->mark(st, HKG_STAGE_SELECT_LEAF);
+    launch_scan_best_bf16(st, nq, dev.X, dev.Xb, dev.xn2, dev.max_leaf, L, b->Q, b->kq, b->best, dev.leaf_pos,
+                          dev.leaf_size, dev.ids, b->cap, b->cand, b->cand_cnt, b->bound, b->counters);
+    if (p) p->mark(st, HKG_STAGE_APPROX);
+    uint32_t *skeys = b->qperm + b->B, *sidx = b->qperm + 2 * b->B;
+    hipLaunchKernelGGL(k_iota_best, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->best, skeys, sidx);
+    HKG_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(b->bf_sort_tmp, b->bf_sort_bytes, skeys, b->qperm + 3 * b->B,
+                                                   sidx, b->qperm, (int)nq, 0, 32, st));
+    // the last plan also counts the (query, leaf) pairs it keeps
+    auto plan_tiles = [&](bool count) {
+      launch_fill(st, b->ntiles_g, lists, 0);
+      hipLaunchKernelGGL(k_group_leaf_need, dim3((nleaf + 255) / 256, lists), dim3(256), 0, st, nq, list_qpb, nleaf,
+                         b->lb, b->bound, b->qperm, b->need, count ? b->counters + kCtrLeafPairs : nullptr);
+      hipLaunchKernelGGL(k_group_tiles, dim3((dev.ntiles + 255) / 256, lists), dim3(256), 0, st, dev.ntiles, nleaf,
+                         dev.tile_leaf0, dev.tile_leaf1, b->need, b->tiles, b->ntiles_g);
+      hipLaunchKernelGGL(k_bf16_thr_init, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->bound, thr0);
+    };
+    plan_tiles(!bound_pass);
+    if (p) p->mark(st, HKG_STAGE_COLLECT);
+    if (bound_pass) {
+      // pass 1: histogram of the BF16 upper bounds -> a k-th distance bound
+      // within about one bin of the k-th smallest upper bound
+      launch_fill(st, b->hist, nq * kBf16Bins, 0);
+      launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, kSharedList,
+                         b->bf_cap, b->bf_surv, b->bf_cnt, thr0, b->hist, b->bf_region, b->bf_region_cnt, b->bf_prog);
+      HKG_HIP_TRY(hipMemcpyAsync(b->misc + 4, b->ntiles_g, lists * 4, hipMemcpyDeviceToDevice, st));
+      hipLaunchKernelGGL(k_bf16_tau, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->kq, b->hist, thr0, b->bound);
+      plan_tiles(true);
+      if (p) p->mark(st, HKG_STAGE_BOUND);
     }
-    hipLaunchKernelGGL(k_bf16_tau, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->kq, b->hist, thr0, b->bound);
-    plan_tiles(true);
-    if (pf) pf->mark(st, HKG_STAGE_BOUND);
+    // pass 2: collect the survivors of the certified lower bound
+    launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, kSharedList, b->bf_cap,
+                       b->bf_surv, b->bf_cnt, thr0, nullptr, b->bf_region, b->bf_region_cnt, b->bf_prog);
+    if (p) p->mark(st, HKG_STAGE_REFINE);
+    launch_bf16_refine(st, nq, dev.X, L, b->Q, b->kq, b->kmax, dev.ids, b->cap, b->cand, b->cand_cnt, b->bf_cap,
+                       b->bf_surv, b->bf_cnt, thr0, b->bound, idmap64, b->out_ids, b->out_d2, overflow, b->counters);
+    if (p) p->mark(st, HKG_STAGE_TOPK);
+    return HKG_OK;
+  };
+  if (use_graph && !seed_host)
+    rc = run_captured(*b, 2, nq, bound_pass ? 1u : 0u, st, pf, enqueue);
+  else
+    rc = enqueue(pf);
+  if (rc != HKG_OK) return rc;
+  std::vector<uint32_t> nt_pass1;
+  if (ctr && bound_pass) {
+    nt_pass1.resize(lists);
+    HKG_HIP_TRY(hipMemcpyAsync(nt_pass1.data(), b->misc + 4, lists * 4, hipMemcpyDeviceToHost, st));
   }
-  // pass 2: collect the survivors of the certified lower bound
-  launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, kSharedList, b->bf_cap,
-                     b->bf_surv, b->bf_cnt, thr0, nullptr, b->bf_region, b->bf_region_cnt, b->bf_prog);
-  if (pf) pf->mark(st, HKG_STAGE_REFINE);
-  launch_bf16_refine(st, nq, dev.X, L, b->Q, b->kq, b->kmax, dev.ids, b->cap, b->cand, b->cand_cnt, b->bf_cap,
-                     b->bf_surv, b->bf_cnt, thr0, b->bound, idmap64, b->out_ids, b->out_d2, overflow, b->counters);
-  if (pf) pf->mark(st, HKG_STAGE_TOPK);
   HKG_HIP_TRY(hipGetLastError());
   if (ctr) {
     HKG_HIP_TRY(hipMemcpyAsync(ctr->v, b->counters, sizeof(ctr->v), hipMemcpyDeviceToHost, st));

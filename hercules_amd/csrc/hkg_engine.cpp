@@ -676,22 +676,12 @@ hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, 
   if (call_settings().mode && !scan) {
     rc = enqueue_mode_batch(*b, nq, st, pf, seed_host != nullptr);
     if (rc != HKG_OK) return rc;
-  } else if (use_graph && !scan && !pf) {
-    if (!b->graph_exec || b->graph_stream != st || b->graph_nq != nq || b->graph_fgrid != filter_grid()) {
-      if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
-      b->graph_exec = nullptr;
-      hipGraph_t g;
-      HKG_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      enqueue_index_batch(*b, nq, st, nullptr, false);
-      HKG_HIP_TRY(hipStreamEndCapture(st, &g));
-      hipError_t e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      HKG_HIP_TRY(e);
-      b->graph_stream = st;
-      b->graph_nq = nq;
-      b->graph_fgrid = filter_grid();
-    }
-    HKG_HIP_TRY(hipGraphLaunch(b->graph_exec, st));
+  } else if (use_graph && !scan) {
+    rc = run_captured(*b, 1, nq, 0, st, pf, [&](Prof *gp) {
+      enqueue_index_batch(*b, nq, st, gp, false);
+      return HKG_OK;
+    });
+    if (rc != HKG_OK) return rc;
   } else if (scan) {
     enqueue_scan_batch(*b, nq, st, pf);
   } else {
@@ -706,6 +696,55 @@ hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, 
   }
   *used = b;
   lease.b = nullptr;
+  return HKG_OK;
+}
+
+hkg_status Index::run_captured(QueryBuffers &b, uint32_t kind, uint32_t nq, uint32_t flags, hipStream_t st, Prof *pf,
+                               const std::function<hkg_status(Prof *)> &enqueue)
+{
+  const bool prof = pf != nullptr;
+  const uint32_t fgrid = filter_grid();
+  if (!b.graph_exec || b.graph_stream != st || b.graph_kind != kind || b.graph_nq != nq || b.graph_flags != flags ||
+      b.graph_fgrid != fgrid || b.graph_profiled != prof) {
+    if (b.graph_exec) (void)hipGraphExecDestroy(b.graph_exec);
+    b.graph_exec = nullptr;
+    Prof *gp = nullptr;
+    if (prof) {
+      if (!b.graph_prof) {
+        b.graph_prof.reset(new Prof());
+        if (!b.graph_prof->init()) {
+          b.graph_prof.reset();
+          set_error("stage timing set-up failed (wall clock rate or stamp buffer)");
+          return HKG_EDEVICE;
+        }
+      }
+      gp = b.graph_prof.get();
+      gp->n = 0;
+    }
+    HKG_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const hkg_status rc = enqueue(gp);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc != HKG_OK || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      if (rc != HKG_OK) return rc;
+      HKG_HIP_TRY(e);
+    }
+    e = hipGraphInstantiate(&b.graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HKG_HIP_TRY(e);
+    b.graph_stream = st;
+    b.graph_kind = kind;
+    b.graph_nq = nq;
+    b.graph_flags = flags;
+    b.graph_fgrid = fgrid;
+    b.graph_profiled = prof;
+  }
+  HKG_HIP_TRY(hipGraphLaunch(b.graph_exec, st));
+  if (pf) {
+    pf->n = 0;
+    pf->src = b.graph_prof.get();
+  }
   return HKG_OK;
 }
 
@@ -914,7 +953,7 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
   if (o.profile) {
     pf.reset(new Prof());
     if (!pf->init()) {
-      set_error("hipEventCreate failed");
+      set_error("stage timing set-up failed (wall clock rate or stamp buffer)");
       return HKG_EDEVICE;
     }
   }
@@ -923,10 +962,13 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     QueryBuffers *b = nullptr;
     Counters c;
     hkg_status rc;
-    if (pf) pf->n = 0;
+    if (pf) {
+      pf->n = 0;
+      pf->src = nullptr;
+    }
     if (path == HKG_PATH_BF16)
       rc = run_bf16_batch(dQ + (uint64_t)q0 * dev.L, dk + q0, n, kmax, st, &b, &c, pf.get(), nullptr,
-                          o.bf16_bound_pass == 1);
+                          o.bf16_bound_pass == 1, n == batch);
     else
       rc = run_batch(path, dQ + (uint64_t)q0 * dev.L, dk + q0, n, kmax, path == HKG_PATH_BATCHED && n == batch, st,
                      &b, &c, pf.get());

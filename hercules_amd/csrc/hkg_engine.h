@@ -2,6 +2,7 @@
 #pragma once
 
 #include <atomic>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -25,40 +26,57 @@ struct Counters {
   uint64_t bf16_rows_bound = 0;  // rows the BF16 histogram pass read
 };
 
-// per-stage event timing of one batch (profile mode)
+// per-stage timing of one batch (profile mode): a one-wave kernel stamps the
+// device's constant-rate wall clock into d_ts at every stage boundary, on the
+// batch's stream, so consecutive stamps bracket one stage's kernels.  The
+// stamps are ordinary kernels, so a captured graph records them as well and
+// its replays time their stages without eager launches.
 struct Prof {
-  hipEvent_t ev[12];
-  int stage[12];
+  static constexpr int kMarks = 12;
+  unsigned long long *d_ts = nullptr;  // device: one stamp per mark
+  int stage[kMarks];
   int n = 0;
   bool ok = false;
+  double ticks_per_ms = 0.0;
+  // a captured graph's own Prof, whose stamps its replay writes, read in
+  // place of this one's
+  const Prof *src = nullptr;
   bool init()
   {
-    for (int i = 0; i < 12; ++i)
-      if (hipEventCreate(&ev[i]) != hipSuccess) return false;
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0 ||
+        hipMalloc((void **)&d_ts, kMarks * sizeof(unsigned long long)) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    ticks_per_ms = (double)khz;
     ok = true;
     return true;
   }
   void mark(hipStream_t st, int s)
   {
-    if (n < 12) {
-      (void)hipEventRecord(ev[n], st);
+    if (n < kMarks) {
+      launch_stamp(st, d_ts + n);
       stage[n++] = s;
     }
   }
+  // after the batch's stream has finished
   void accumulate(double *ms, uint64_t *launches) const
   {
-    for (int i = 1; i < n; ++i) {
-      float t = 0.0f;
-      if (hipEventElapsedTime(&t, ev[i - 1], ev[i]) == hipSuccess && stage[i] >= 0) {
-        ms[stage[i]] += t;
-        launches[stage[i]] += 1;
-      }
+    const Prof &p = src ? *src : *this;
+    if (p.n < 2) return;
+    unsigned long long ts[kMarks];
+    if (hipMemcpy(ts, p.d_ts, p.n * sizeof(unsigned long long), hipMemcpyDeviceToHost) != hipSuccess) return;
+    for (int i = 1; i < p.n; ++i) {
+      if (p.stage[i] < 0 || ts[i] < ts[i - 1]) continue;
+      ms[p.stage[i]] += (double)(ts[i] - ts[i - 1]) / p.ticks_per_ms;
+      launches[p.stage[i]] += 1;
     }
   }
   ~Prof()
   {
-    if (ok)
-      for (int i = 0; i < 12; ++i) (void)hipEventDestroy(ev[i]);
+    if (d_ts) (void)hipFree(d_ts);
   }
 };
 
@@ -112,11 +130,16 @@ struct QueryBuffers {
   uint64_t *vtop = nullptr;     // B x kmax: their answers after the approximate phase
   unsigned long long *lab = nullptr;  // B: their iSAX counts
   bool busy = false;            // held by a query call (guarded by Index::mu_bufs)
-  // captured stage sequence of the batched path
+  // captured stage sequence of a batched path (FP32 or BF16), and the shape
+  // it was captured for
   hipGraphExec_t graph_exec = nullptr;
   hipStream_t graph_stream = nullptr;
+  uint32_t graph_kind = 0;      // 1: FP32 batched, 2: BF16
   uint32_t graph_nq = 0;
+  uint32_t graph_flags = 0;     // path-specific (BF16: the bound pass)
   uint32_t graph_fgrid = 0;     // filter + refine grid the graph was captured with
+  bool graph_profiled = false;  // stage events recorded by the graph
+  std::unique_ptr<Prof> graph_prof;
   ~QueryBuffers();
 };
 
@@ -154,7 +177,12 @@ struct Index {
                        const uint32_t *seed_host = nullptr);
   hkg_status run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax, hipStream_t st,
                             QueryBuffers **used, Counters *ctr, Prof *pf, const uint32_t *seed_host,
-                            bool bound_pass);
+                            bool bound_pass, bool use_graph = false);
+  // launch the stage sequence `enqueue` through the buffer set's captured
+  // graph (captured again when kind, nq, flags, grid, stream or profiling
+  // change); with pf, its stage events are the graph's own
+  hkg_status run_captured(QueryBuffers &b, uint32_t kind, uint32_t nq, uint32_t flags, hipStream_t st, Prof *pf,
+                          const std::function<hkg_status(Prof *)> &enqueue);
   uint64_t total_scan_bytes(uint32_t nq, uint32_t batch) const;
   uint64_t total_segs() const;
   // persistent grid of k_filter_refine for the running call

@@ -52,6 +52,8 @@ void launch_reset(hipStream_t st, uint32_t nq, uint32_t *bound, uint32_t *cand_c
 // sees them) and counted in *bad, which the host turns into HKG_EINVAL
 void launch_check_k(hipStream_t st, uint32_t nq, uint32_t kmax, uint32_t *kq, uint32_t *bad);
 void launch_fill(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v);
+// one wave writes wall_clock64() (hipDeviceAttributeWallClockRate ticks) to *t
+void launch_stamp(hipStream_t st, unsigned long long *t);
 void launch_prep(hipStream_t st, const float *Q, uint32_t nq, uint32_t L, float *paa, double *psum,
                  double *qn2);
 void launch_eapca(hipStream_t st, uint32_t nq, uint32_t nleaf, uint32_t L, const uint32_t *seg_off,

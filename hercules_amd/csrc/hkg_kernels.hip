@@ -781,7 +781,15 @@ __global__ void __launch_bounds__(kBlock) k_scan(const float *__restrict__ X, ui
       }
     }
     __syncthreads();
-    for (int j = 0; j < nqg; ++j) topk_merge(tk[j], scan, (int)kq[q0 + j]);
+    for (int j = 0; j < nqg; ++j) {
+      const int k = (int)kq[q0 + j];
+      topk_merge(tk[j], scan, k);
+      // a full list's k-th distinct distance bounds the query's k-th: share
+      // it at once, as the reference's workers share one k-th distance
+      // (parallel_scan_utils.c:381-383), so every block prunes against the
+      // tightest bound found anywhere so far
+      if (threadIdx.x == 0 && tk[j].cnt == k && tk[j].kth_bits < bnd[j]) atomicMin(&bound[q0 + j], tk[j].kth_bits);
+    }
   }
   for (int j = 0; j < nqg; ++j) flush_topk(tk[j], (int)kq[q0 + j], q0 + j, cap, cand, cand_cnt, bound);
 }
@@ -1009,6 +1017,16 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t n, uint32_t v)
 void launch_reset(hipStream_t st, uint32_t nq, uint32_t *bound, uint32_t *cand_cnt, const uint32_t *seed)
 {
   hipLaunchKernelGGL(k_reset_queries, dim3((nq + 255) / 256), dim3(256), 0, st, nq, bound, cand_cnt, seed);
+}
+
+__global__ void k_stamp(unsigned long long *__restrict__ t)
+{
+  if (threadIdx.x == 0) *t = (unsigned long long)wall_clock64();
+}
+
+void launch_stamp(hipStream_t st, unsigned long long *t)
+{
+  hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, st, t);
 }
 
 void launch_check_k(hipStream_t st, uint32_t nq, uint32_t kmax, uint32_t *kq, uint32_t *bad)

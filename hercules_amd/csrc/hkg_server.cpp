@@ -3,9 +3,12 @@
 // Requests (one query each) wait in a FIFO.  max_inflight worker threads,
 // each with its own HIP stream, pinned staging and device buffers, loop:
 // take every pending request that fits (max_batch queries, and the worker's
-// share of the scratch budget at the path's per-query scratch), upload the
-// batch, answer it with Index::knn_device on the worker's stream, copy the
-// results back and complete the requests.  A request that arrives while
+// share of the scratch budget), upload the batch, answer it with
+// Index::knn_device on the worker's stream, copy the results back and
+// complete the requests.  Scratch is charged by k: a batch runs at the
+// smallest power-of-two k class that covers its requests' k, every query of
+// it is charged the path's scratch at that class, and the batch's device
+// buffers are sized for the queries its share admits at that class.  A request that arrives while
 // batches are on the device is admitted by the next worker that frees up,
 // so with two workers one batch is uploaded and started while the other is
 // still running.  Each request is answered exactly as it would be alone:
@@ -49,7 +52,15 @@ struct Server {
   Index *ix = nullptr;
   hkg_server_opts opt{};
   uint64_t batch_scratch = 0;  // scratch share of one batch
-  uint64_t per_query = 0;      // scratch one query needs on the path
+  // k class of a request: the smallest power of two >= k, at most max_k
+  uint32_t k_class(uint32_t k) const
+  {
+    uint32_t c = 1;
+    while (c < k) c <<= 1;
+    return std::min(c, opt.max_k);
+  }
+  // scratch one query needs on the path at a k class
+  uint64_t per_query(uint32_t kc) const { return ix->scratch_per_query((hkg_path)opt.path, kc); }
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::unordered_map<uint64_t, Request> reqs;
@@ -111,8 +122,6 @@ void Server::worker(int wi)
       return;
     }
   }
-  // buffer sets sized for a full batch: batches of any size reuse them
-  call_settings().min_buffer_batch = B;
   hkg_query_opts qo;
   hkg_default_query_opts(&qo);
   qo.path = (hkg_path)opt.path;
@@ -122,24 +131,30 @@ void Server::worker(int wi)
   batch.reserve(B);
   for (;;) {
     batch.clear();
+    uint32_t kc = 1;  // the batch's k class
     {
       std::unique_lock<std::mutex> lock(mu);
       cv_work.wait(lock, [&] { return stopping || !pending.empty(); });
       if (pending.empty()) break;  // stopping and drained
-      // admit what fits: max_batch queries and this batch's scratch share
-      uint64_t used = 0;
-      while (!pending.empty() && batch.size() < B && (batch.empty() || used + per_query <= batch_scratch)) {
-        const uint64_t t = pending.front();
+      // admit what fits: max_batch queries, each charged the scratch of the
+      // batch's k class, within this batch's scratch share
+      while (!pending.empty() && batch.size() < B) {
+        Request &r = reqs[pending.front()];
+        const uint32_t kc2 = std::max(kc, k_class(r.k));
+        if (!batch.empty() && (batch.size() + 1) * per_query(kc2) > batch_scratch) break;
         pending.pop_front();
-        Request &r = reqs[t];
         r.state = 1;
         batch.push_back(&r);
-        used += per_query;
+        kc = kc2;
       }
     }
     // requests on the device are not touched by the client (their state
     // is 1), and unordered_map never moves its elements
     const uint32_t n = (uint32_t)batch.size();
+    // buffer sets sized for what the share admits at this class (batches of
+    // fewer queries reuse them)
+    call_settings().min_buffer_batch =
+        (uint32_t)std::max<uint64_t>(n, std::min<uint64_t>(B, batch_scratch / per_query(kc)));
     for (uint32_t i = 0; i < n; ++i) {
       const Request &r = *batch[i];
       memcpy(s.hQ + (size_t)i * L, r.q.data(), (size_t)L * 4);
@@ -150,10 +165,11 @@ void Server::worker(int wi)
         hipMemcpyAsync(s.dk, s.hk, (size_t)n * 4, hipMemcpyHostToDevice, s.st) != hipSuccess) {
       rc = HKG_EDEVICE;
     }
-    if (rc == HKG_OK) rc = ix->knn_device(s.dQ, n, s.dk, K, &qo, s.dids, s.dd2, s.st, nullptr);
+    // results at the class's stride (kc ranks per query)
+    if (rc == HKG_OK) rc = ix->knn_device(s.dQ, n, s.dk, kc, &qo, s.dids, s.dd2, s.st, nullptr);
     if (rc == HKG_OK &&
-        (hipMemcpyAsync(s.hids, s.dids, (size_t)n * K * 8, hipMemcpyDeviceToHost, s.st) != hipSuccess ||
-         hipMemcpyAsync(s.hd2, s.dd2, (size_t)n * K * 4, hipMemcpyDeviceToHost, s.st) != hipSuccess ||
+        (hipMemcpyAsync(s.hids, s.dids, (size_t)n * kc * 8, hipMemcpyDeviceToHost, s.st) != hipSuccess ||
+         hipMemcpyAsync(s.hd2, s.dd2, (size_t)n * kc * 4, hipMemcpyDeviceToHost, s.st) != hipSuccess ||
          hipStreamSynchronize(s.st) != hipSuccess)) {
       rc = HKG_EDEVICE;
     }
@@ -163,8 +179,8 @@ void Server::worker(int wi)
       Request &r = *batch[i];
       r.rc = rc;
       if (rc == HKG_OK) {
-        r.ids.assign(s.hids + (size_t)i * K, s.hids + (size_t)i * K + r.k);
-        r.d2.assign(s.hd2 + (size_t)i * K, s.hd2 + (size_t)i * K + r.k);
+        r.ids.assign(s.hids + (size_t)i * kc, s.hids + (size_t)i * kc + r.k);
+        r.d2.assign(s.hd2 + (size_t)i * kc, s.hd2 + (size_t)i * kc + r.k);
       }
       r.q.clear();
       r.q.shrink_to_fit();
@@ -215,9 +231,8 @@ hkg_status hkg_server_start(hkg_index *index, const hkg_server_opts *so, hkg_ser
   std::unique_ptr<Server> s(new Server());
   s->ix = reinterpret_cast<hkg::Index *>(index);
   s->opt = o;
-  s->per_query = s->ix->scratch_per_query((hkg_path)o.path, o.max_k);
   s->batch_scratch = o.scratch_bytes / o.max_inflight;
-  if (s->per_query > s->batch_scratch) {
+  if (s->per_query(o.max_k) > s->batch_scratch) {
     hkg::set_error("server: the scratch budget does not hold one query per batch");
     return HKG_EINVAL;
   }

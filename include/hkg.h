@@ -283,7 +283,10 @@ typedef struct hkg_server_opts {
   uint32_t max_batch;        /* queries admitted per device batch (1024)     */
   uint64_t scratch_bytes;    /* device scratch budget of all batches in
                                 flight; a batch admits queries while its
-                                share fits (0 = 16 GiB)                     */
+                                share (scratch_bytes / max_inflight) holds
+                                them, each charged the path's scratch at the
+                                batch's k class (the smallest power of two
+                                covering its requests' k) (0 = 16 GiB)      */
   uint32_t queue_capacity;   /* pending requests before HKG_EBUSY (65536)    */
   uint32_t path;             /* hkg_path of the batches (0 = BATCHED)        */
   uint32_t max_inflight;     /* batches on the device at once (2)            */

@@ -202,6 +202,52 @@ def test_bf16_path_matches_oracle(hkg, oracle, small_data, ref_index, k, bound_p
     assert r.stats["overflow_retries"] == 0
     # the certified bound prunes far more than the iSAX bound
     assert r.stats["series_refined"] <= s.stats["series_refined"] + Q.shape[0] * k
+    # (query, leaf) pairs within the k-th bound after the approximate stage:
+    # at least the best leaf of every query, at most every pair
+    nleaf = ix_leaves(idx)
+    for st in (r.stats, s.stats):
+        assert Q.shape[0] < st["leaves_after_eapca"] <= Q.shape[0] * nleaf
+
+
+def ix_leaves(idx: str) -> int:
+    from hercules_amd import hkg
+    return int(hkg.inspect_dir(idx)["n_leaves"])
+
+
+def test_device_entry_rejects_bad_k_and_path(hkg, small_data, ref_index):
+    """hkg_knn_device checks every per-query k on the device (values
+    outside [1, kmax] are clamped for the kernels and reported as
+    HKG_EINVAL), and an unknown path before any launch."""
+    import ctypes as C
+    X, Q = small_data
+    idx, _ = ref_index
+    _say("bad k / bad path")
+    nq, kmax = 8, 5
+    dQ = hkg.DeviceArray(nq * L256 * 4)
+    dQ.upload(np.ascontiguousarray(Q[:nq]))
+    dids, dd2 = hkg.DeviceArray(nq * kmax * 8), hkg.DeviceArray(nq * kmax * 4)
+    with hkg.HerculesIndex.open(idx) as ix:
+        for path in ("batched", "bf16", "scan", "index"):
+            for bad in (0, kmax + 1):
+                kv = np.full(nq, 3, np.uint32)
+                kv[5] = bad
+                dk = hkg.DeviceArray(kv.nbytes)
+                dk.upload(kv)
+                with pytest.raises(hkg.HkgError) as e:
+                    ix.knn_device(dQ, nq, dk, kmax, dids, dd2, path=path)
+                assert e.value.status == hkg.HKG_EINVAL and "outside [1, kmax" in str(e.value)
+        # good k after the bad calls: the answers are the host entry's
+        kv = np.full(nq, 3, np.uint32)
+        dk = hkg.DeviceArray(kv.nbytes)
+        dk.upload(kv)
+        ix.knn_device(dQ, nq, dk, kmax, dids, dd2, path="batched")
+        r = ix.knn(Q[:nq], kv, path="batched")
+        got = dd2.download(np.empty((nq, kmax), np.float32))
+        assert np.array_equal(got[:, :3].view(np.uint32), r.d2[:, :3].view(np.uint32))
+        qo = hkg.query_opts("batched")
+        qo.path = 9
+        rc = hkg.lib().hkg_knn_device(ix._h, dQ.ptr, nq, dk.ptr, kmax, C.byref(qo), dids.ptr, dd2.ptr, None, None)
+        assert rc == hkg.HKG_EINVAL
 
 
 def test_bf16_embeddings_l96(hkg, oracle, tmp_path):
