@@ -6,7 +6,10 @@ dataset in seconds:
 - 50M x 256 random walks, 10,000 queries with sigma ~ U[0.01, 0.2] and k
   drawn from {1, 10, 100};
 - 100M x 96 embeddings (N(0,1) -> L2 -> z-normalised), 10,000 10-NN
-  queries of the same kind (sigma ~ U[0.01, 0.2]).
+  queries of the same kind (sigma ~ U[0.01, 0.2]);
+- 100M x 256 random walks (the serving workload), 4,096 queries with
+  sigma ~ U[0, 0.2] and k ~ U{1..100}, plus a Poisson burst of them through
+  the serving loop.
 
 The checks are properties that hold at any size:
 - the BF16 path answers every query as the batched path does, bit for bit
@@ -22,7 +25,9 @@ The checks are properties that hold at any size:
 - FP64 distances of the returned pairs stay within FP32 rounding of d2 (the
   error per result), and the bounds prune most series;
 - epsilon answers stay within (1 + epsilon) of the exact ones rank by rank,
-  and no-guarantee answers are never below them.
+  and no-guarantee answers are never below them;
+- served requests (continuous batching, mixed k) get the answers of the
+  same queries asked alone.
 """
 from __future__ import annotations
 
@@ -57,6 +62,10 @@ WORKLOADS = {
     "100Mx96_k10": dict(kind=1, n=100_000_000, L=96, leaf=10000, nq=10000, qseed=6, batched_all=False,
                         sigma=lambda rng, nq: rng.uniform(0.01, 0.2, nq).astype(np.float32),
                         k=lambda rng, nq: np.full(nq, 10, np.uint32)),
+    "100Mx256_kU100": dict(kind=0, n=100_000_000, L=256, leaf=10000, nq=4096, qseed=7, batched_all=True,
+                           serve=512,
+                           sigma=lambda rng, nq: rng.uniform(0.0, 0.2, nq).astype(np.float32),
+                           k=lambda rng, nq: rng.integers(1, 101, nq).astype(np.uint32)),
 }
 
 
@@ -147,3 +156,38 @@ def test_full_size_properties(hkg, oracle, name):
             k = int(k)
             assert (re.d2[i, :k].astype(np.float64) <= 2.0 * ex[i, :k] * (1 + 1e-6)).all(), (name, i)
             assert (rn.d2[i, :k] >= rf.d2[sample[i], :k]).all(), (name, i)
+
+        if w.get("serve"):
+            _serve_burst(hkg, ix, Q, kq, rf, w["serve"], name)
+
+
+def _serve_burst(hkg, ix, Q, kq, rf, nreq: int, name: str) -> None:
+    """Requests arrive as a Poisson stream (about 4 batches' worth in flight)
+    while batches run: each served answer equals the direct batch's and, on
+    a sample, the same query asked alone."""
+    rng = np.random.default_rng(21)
+    rows = rng.integers(0, Q.shape[0], nreq)
+    gaps = rng.exponential(1.0 / 20000.0, nreq)
+    srv = hkg.Server(ix, max_batch=256, path="bf16", max_inflight=2, max_k=int(kq.max()), scratch_bytes=16 << 30)
+    try:
+        tickets = []
+        t = time.perf_counter()
+        for i in range(nreq):
+            while time.perf_counter() - t < gaps[i]:
+                pass
+            t = time.perf_counter()
+            tickets.append(srv.submit(Q[rows[i]], int(kq[rows[i]])))
+        out = [srv.wait(tk, int(kq[rows[i]])) for i, tk in enumerate(tickets)]
+    finally:
+        srv.stop()
+    lat = np.array([o[2] for o in out])
+    _say(f"sizes {name}: {nreq} served, latency p50 {np.percentile(lat, 50) / 1e3:.1f} ms, "
+         f"p99 {np.percentile(lat, 99) / 1e3:.1f} ms")
+    for i, (ids, d2, _) in enumerate(out):
+        q, k = int(rows[i]), int(kq[rows[i]])
+        assert np.array_equal(ids, rf.ids[q, :k]) and np.array_equal(d2.view(np.uint32), rf.d2[q, :k].view(np.uint32)), \
+            (name, "served", i, q)
+    for i in range(0, nreq, nreq // 8):
+        q, k = int(rows[i]), int(kq[rows[i]])
+        r = ix.knn(Q[q][None, :], k, path="bf16")
+        assert np.array_equal(r.ids[0], out[i][0]) and np.array_equal(r.d2[0].view(np.uint32), out[i][1].view(np.uint32))
