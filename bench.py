@@ -5,9 +5,12 @@ Workload (defaults): 100M z-normalised random walks of length 256 (FP32,
 102.4 GB in HBM) generated on the device, an EAPCA index built on the device
 by the bulk loader (leaf 10000), and queries that are noisy copies of
 dataset series with sigma ~ U[0, 0.2] and k ~ U{1..100} (seeded).  A "step"
-is one batch of queries answered exactly through the batched index path
-(EAPCA leaf bounds -> best-leaf scan -> iSAX filter + exact refinement ->
-top-k).  Inputs are resident in HBM when the timed region starts.
+is one batch of queries answered exactly through the BF16 path (EAPCA leaf
+bounds -> best-leaf scan -> BF16 MFMA certified bound -> exact FP32
+refinement -> top-k), replayed as one captured hipGraph.  Inputs are
+resident in HBM when the timed region starts.  Further legs time the PSCAN
+full scan, the plain FP32 paths, the approximate modes, serving under a
+sweep of offered loads, and the reference's own CPU path (cpu_baseline).
 
 Prints one JSON line (rank 0).  `python bench.py --gpus N` with N > 1 exits:
 this build runs on one GPU.
@@ -49,12 +52,22 @@ def parse() -> argparse.Namespace:
     p.add_argument("--sigma-max", type=float, default=0.2)
     p.add_argument("--kmax", type=int, default=100)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-sample", type=int, default=1_000_000, help="rows of the CPU baseline sample")
+    p.add_argument("--cpu-series", type=int, default=10_000_000,
+                   help="CPU baseline: series of the reference-built index the reference's SIMS path answers on")
+    p.add_argument("--cpu-queries", type=int, default=32, help="CPU baseline: queries (k = --cpu-k)")
+    p.add_argument("--cpu-k", type=int, default=50, help="CPU baseline: k (the reference answers one k per run)")
+    p.add_argument("--cpu-pscan-queries", type=int, default=4, help="CPU baseline: queries of the reference PSCAN")
+    p.add_argument("--scan-passes", type=int, default=8, help="scan leg: single-query passes timed (0: skip)")
+    p.add_argument("--fp32-batch", type=int, default=2048, help="FP32 batched leg: queries per step (0: skip)")
+    p.add_argument("--fp32-steps", type=int, default=3, help="FP32 batched leg: timed steps")
+    p.add_argument("--index-queries", type=int, default=64, help="plain per-query FP32 leg: queries (0: skip)")
     p.add_argument("--parity-queries", type=int, default=8)
     p.add_argument("--row-probes", type=int, default=32, help="dataset rows looked up exactly after the build")
     p.add_argument("--bf16-bound-pass", type=int, default=0, choices=[0, 1, 2])
     p.add_argument("--serve-qps", type=float, default=0.0,
-                   help="offered load of the serving leg (0: half the measured throughput)")
+                   help="offered load of the serving leg (0: a sweep at --serve-loads of the measured throughput)")
+    p.add_argument("--serve-loads", default="0.5,0.8,0.95",
+                   help="serving sweep: offered loads as fractions of the measured throughput")
     p.add_argument("--serve-requests", type=int, default=4096, help="requests of the serving leg (0: skip)")
     p.add_argument("--mode-epsilon", type=float, default=1.0,
                    help="epsilon of the approximate-mode leg (0: skip that leg)")
@@ -75,37 +88,111 @@ def parse() -> argparse.Namespace:
     return p.parse_args()
 
 
-def cpu_baseline(args, Q: np.ndarray, kq: np.ndarray) -> dict:
-    """The reference PSCAN (compiled from its sources into oracle/_ref) on a
-    bounded sample: the first cpu_sample rows of the same dataset (generated
-    by the oracle's CPU generator, identical to the device's), a few of the
-    benchmark's queries; queries/s scaled linearly to the full N (a scan's
-    cost is linear in N)."""
+def host_info() -> dict:
+    """The host the CPU baseline ran on: threads, CPU model, RAM."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    ram = None
+    try:
+        for line in open("/proc/meminfo"):
+            if line.startswith("MemTotal"):
+                ram = round(int(line.split()[1]) / 1024 ** 2, 1)
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model, "ram_gib": ram}
+
+
+def cpu_baseline(hkg, args) -> dict:
+    """The reference's own multithreaded SIMS query path
+    (exact_de_knn_search_psq_lf_sims_adaptive, hercules_query_engine.c:706-1083,
+    compiled from its sources into oracle/_ref with the fixes the oracle
+    applies) on every host core, over an index the reference itself builds
+    from the first cpu_series rows of the same dataset (leaf as the GPU
+    run), at k = cpu_k; measured at that N, not scaled.  The reference PSCAN
+    on the same rows is a second figure.  The GPU answers the same queries
+    on the same reference-built index, and the squared distances are
+    compared bit for bit."""
+    import shutil
     sys.path.insert(0, os.path.join(REPO, "tests"))
     try:
         import ref_runner as rr
-        from oracle_bind import Oracle
     except Exception as e:  # pragma: no cover
-        return {"error": f"oracle unavailable: {e}"}
+        return {"error": f"reference runner unavailable: {e}"}
     if not rr.available():
         return {"error": "reference binaries not built (oracle/_ref)"}
-    threads = max(2, min(16, os.cpu_count() or 2))
-    o = Oracle()
-    m = min(args.cpu_sample, args.n)
-    nq = 4
-    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
-        X = o.series(0, 1, 0, m, args.length, threads=threads)
-        rr.write_f32(os.path.join(d, "data.bin"), X)
-        del X
-        rr.write_f32(os.path.join(d, "q.bin"), Q[:nq])
-        k = int(kq[:nq].max())
-        _, wall, _ = rr.pscan(os.path.join(d, "data.bin"), m, os.path.join(d, "q.bin"), nq, args.length, k,
-                              threads=threads - 1)
-    qps_sample = nq / wall
-    return {"value": qps_sample * m / args.n, "unit": "queries/s", "cores": threads, "kind": "reference",
-            "sample": f"reference PSCAN ({threads - 1} scan workers + 1 reader) over the first {m} rows of the "
-                      f"same dataset, {nq} of the benchmark's queries (k={k}), {wall:.2f} s; queries/s scaled "
-                      f"by {m}/{args.n}"}
+    host = host_info()
+    threads = max(2, host["nproc"] or 2)
+    n, L, nq, k = min(args.cpu_series, args.n), args.length, args.cpu_queries, args.cpu_k
+    d = tempfile.mkdtemp(prefix="hkg_cpu_", dir=os.environ.get("TMPDIR", "/tmp"))
+    try:
+        t0 = time.perf_counter()
+        data = os.path.join(d, "data.bin")
+        chunk = 1_000_000
+        dX = hkg.DeviceArray(min(chunk, n) * L * 4)
+        with open(data, "wb") as f:
+            for first in range(0, n, chunk):
+                m = min(chunk, n - first)
+                hkg.gen_series_device(0, 1, first, m, L, dX)
+                f.write(dX.download(np.empty((m, L), np.float32)).tobytes())
+        dX.close()
+        gen_s = time.perf_counter() - t0
+        rq = np.random.default_rng(15)
+        sig = rq.uniform(0.0, args.sigma_max, nq).astype(np.float32)
+        Q, _ = hkg.gen_noisy_queries(0, 1, n, 15, sig, L)
+        qf = os.path.join(d, "q.bin")
+        rr.write_f32(qf, Q)
+        idx = os.path.join(d, "idx")
+        log(f"cpu baseline: {n} x {L} written in {gen_s:.1f} s; reference build on {threads} threads")
+        tb = time.perf_counter()
+        rr.build_index(data, n, L, args.leaf, idx, threads=threads, timeout=1800)
+        build_s = time.perf_counter() - tb
+        with open(os.path.join(idx, "leaves_raw.idx"), "rb") as f:  # page cache warm
+            while f.read(64 << 20):
+                pass
+        log(f"cpu baseline: reference index built in {build_s:.1f} s; {nq} queries, k = {k}")
+        rd2, _, wall, out = rr.query(idx, qf, nq, L, k, threads=threads, sims=True, timeout=1800)
+        # the same run with one query: the CLI's start-up and index load, which
+        # the per-query rate leaves out
+        _, _, wall1, _ = rr.query(idx, qf, 1, L, k, threads=threads, sims=True, timeout=1800)
+        per = max(wall - wall1, 1e-9) / max(nq - 1, 1)
+        # the same queries through the engine on the same index
+        with hkg.HerculesIndex.open(idx) as ix:
+            same = True
+            gpu = {}
+            for path in ("bf16", "batched"):
+                ix.knn(Q[:2], k, path=path)
+                hkg.lib().hkg_device_sync()
+                tg = time.perf_counter()
+                r = ix.knn(Q, k, path=path, batch=nq)
+                gpu[path] = nq / (time.perf_counter() - tg)
+                same &= bool(np.array_equal(r.d2.view(np.uint32), rd2.view(np.uint32)))
+        npq = min(args.cpu_pscan_queries, nq)
+        pscan_qps = None
+        if npq:
+            rr.write_f32(os.path.join(d, "qp.bin"), Q[:npq])
+            _, pwall, _ = rr.pscan(data, n, os.path.join(d, "qp.bin"), npq, L, k, threads=threads, timeout=1800)
+            pscan_qps = npq / pwall
+        return {"value": 1.0 / per, "unit": "queries/s", "cores": threads, "kind": "reference",
+                "sample": f"the reference's SIMS index path (exact_de_knn_search_psq_lf_sims_adaptive, "
+                          f"--num-threads {threads} --sims --eapca-threshold 0.25 --sax-threshold 0.9) on an "
+                          f"index it built from the first {n} rows of the same dataset (leaf {args.leaf}); {nq} "
+                          f"noisy queries of those rows (sigma ~ U[0, {args.sigma_max}], seed 15), k = {k}; "
+                          f"measured at {n} series (not scaled); queries/s = (nq - 1) / (wall(nq) - wall(1))",
+                "n_series": n, "k": k, "queries": nq, "host": host,
+                "reference_wall_s": round(wall, 2), "reference_wall_1_query_s": round(wall1, 2),
+                "reference_build_s": round(build_s, 1),
+                "pscan_queries_per_s": pscan_qps, "pscan_queries": npq,
+                "gpu_same_index_queries_per_s": gpu,
+                "identical_d2_to_reference": same}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
 
 
 def mode_cpu_leg(hkg, args, Q: np.ndarray) -> dict:
@@ -192,7 +279,7 @@ def serving_leg(hkg, ix, args, Qh: np.ndarray, kq: np.ndarray, qps: float) -> di
     rows = rng.integers(0, Qh.shape[0], nreq)
     kmax = int(kq.max())
     srv = hkg.Server(ix, max_batch=args.batch, path=args.path, max_inflight=2, max_k=kmax,
-                     bf16_bound_pass=args.bf16_bound_pass)
+                     bf16_bound_pass=args.bf16_bound_pass, scratch_bytes=32 << 30)
     tickets = [None] * nreq
     ready = threading.Semaphore(0)
 
@@ -230,8 +317,80 @@ def serving_leg(hkg, ix, args, Qh: np.ndarray, kq: np.ndarray, qps: float) -> di
     return {"offered_qps": round(qps, 1), "achieved_qps": round(nreq / wall, 1), "requests": nreq,
             "p50_ms": round(float(np.percentile(lat, 50)) / 1e3, 3),
             "p99_ms": round(float(np.percentile(lat, 99)) / 1e3, 3),
-            "max_batch": args.batch, "inflight": 2, "path": args.path,
+            "max_batch": args.batch, "inflight": 2, "path": args.path, "scratch_gib": 32,
             "answers_identical_to_direct_queries": same, "sampled": int(len(sample))}
+
+
+def scan_leg(hkg, ix, Qh, kq, dQ, dk, kmax: int, dids, dd2, args, N: int, L: int) -> dict:
+    """PSCAN (hkg path scan) over the whole index: passes of one query and of
+    four queries (the workload's queries and k), each timed on its stream by
+    the stage stamps, in the scan grid hkg_autotune picks on 8 other queries
+    of the workload; algorithmic bytes 4 N L per pass."""
+    tune = ix.autotune(Qh[-8:], kq[-8:], path="scan")
+    out = {"tuning": tune}
+    nbytes = 4.0 * N * L
+    for qn, passes in ((1, args.scan_passes), (4, max(2, args.scan_passes // 2))):
+        qv, kv = _views(hkg, dQ, dk, 0, qn, L)
+        ix.knn_device(qv, qn, kv, kmax, dids, dd2, path="scan", batch=qn, profile=True)  # warm
+        ms = []
+        for p in range(passes):
+            qv, kv = _views(hkg, dQ, dk, p, qn, L)
+            st = ix.knn_device(qv, qn, kv, kmax, dids, dd2, path="scan", batch=qn, profile=True)
+            ms.append(st["stage_ms"]["scan"] / max(st["stage_launches"]["scan"], 1))
+        avg = float(np.mean(ms))
+        tb = traffic_bytes("k_scan", N, L, qn)
+        out[f"queries_per_pass_{qn}"] = {
+            "passes": passes, "avg_pass_ms": avg, "min_pass_ms": float(np.min(ms)), "max_pass_ms": float(np.max(ms)),
+            "algorithmic_bytes_per_pass": nbytes, "algorithmic_gbs": nbytes / (avg * 1e6),
+            "algorithmic_frac_of_hbm_peak": nbytes / (avg * 1e6) / HBM_PEAK_GBS,
+            "traffic_bytes_per_pass": tb, "traffic_gbs": tb / (avg * 1e6) if tb else None,
+            "queries_per_s": qn / (avg * 1e-3)}
+    return out
+
+
+def fp32_legs(hkg, ix, dQ, dk, kmax: int, dids, dd2, args, L: int, bf16_qps: float) -> dict:
+    """The plain FP32 GPU paths at the headline workload (the same queries):
+    the batched path (persistent (query, leaf) work queue, iSAX filter +
+    exact refinement, captured graph) and the per-query index path."""
+    out = {}
+    B = args.fp32_batch
+    if B:
+        def step(i):
+            qv, kv = _views(hkg, dQ, dk, i, B, L)
+            return ix.knn_device(qv, B, kv, kmax, dids, dd2, path="batched", batch=B, profile=True)
+        step(0)  # warm: buffer set and graph
+        hkg.lib().hkg_device_sync()
+        t = time.perf_counter()
+        sts = [step(i) for i in range(args.fp32_steps)]
+        wall = time.perf_counter() - t
+        ref_ms = sum(st["stage_ms"]["refine"] for st in sts) / len(sts)
+        ref_b = sum(st["stage_bytes"]["refine"] for st in sts) / len(sts)
+        out["batched"] = {
+            "batch": B, "steps": args.fp32_steps, "queries_per_s": args.fp32_steps * B / wall,
+            "ms_per_step": wall * 1e3 / args.fp32_steps,
+            "stages_ms": {kk: round(sum(st["stage_ms"][kk] for st in sts) / len(sts), 3)
+                          for kk in sts[0]["stage_ms"] if sts[0]["stage_ms"][kk] > 0},
+            "k_filter_refine_ms": ref_ms,
+            # per query: 16 B of iSAX word per checked series + 4 L B per refined series
+            "k_filter_refine_bytes_per_launch": ref_b,
+            "k_filter_refine_gbs": ref_b / (ref_ms * 1e6) if ref_ms else None,
+            "k_filter_refine_frac_of_hbm_peak": ref_b / (ref_ms * 1e6) / HBM_PEAK_GBS if ref_ms else None,
+            "series_lb_checked_per_query": sum(st["series_lb_checked"] for st in sts) / (len(sts) * B),
+            "series_refined_per_query": sum(st["series_refined"] for st in sts) / (len(sts) * B),
+            "leaves_after_eapca_per_query": sum(st["leaves_after_eapca"] for st in sts) / (len(sts) * B)}
+        out["bf16_over_fp32_batched"] = bf16_qps / out["batched"]["queries_per_s"]
+    M = args.index_queries
+    if M:
+        qv, kv = _views(hkg, dQ, dk, 0, M, L)
+        ix.knn_device(qv, 2, kv, kmax, dids, dd2, path="index")  # warm
+        hkg.lib().hkg_device_sync()
+        t = time.perf_counter()
+        st = ix.knn_device(qv, M, kv, kmax, dids, dd2, path="index")
+        wall = time.perf_counter() - t
+        out["index"] = {"queries": M, "queries_per_s": M / wall, "ms_per_query": wall * 1e3 / M,
+                        "series_refined_per_query": st["series_refined"] / M}
+        out["bf16_over_fp32_index"] = bf16_qps / out["index"]["queries_per_s"]
+    return out
 
 
 def loader_leg(hkg, args, L: int, Qh: np.ndarray) -> dict:
@@ -358,6 +517,15 @@ def main() -> int:
     hkg.lib().hkg_device_sync()
     elapsed = time.perf_counter() - t
     qps = steps * B / elapsed
+    # the same steps again without the stage stamps (a cross-check that the
+    # stamp kernels in the graph cost nothing measurable)
+    nchk = min(steps, 5)
+    hkg.lib().hkg_device_sync()
+    t = time.perf_counter()
+    for s in range(warm, warm + nchk):
+        run(s, False)
+    hkg.lib().hkg_device_sync()
+    qps_nostamps = nchk * B / (time.perf_counter() - t)
 
     # dominant kernel and its roofline
     dom = max(stage_ms, key=lambda k: stage_ms[k])
@@ -408,6 +576,15 @@ def main() -> int:
         found = int(np.sum((rr.d2[:, 0] == 0.0) & (rr.ids[:, 0] == rows.astype(np.uint64))))
         par["rows_found_exactly"] = f"{found}/{len(rows)}"
         log(f"dataset rows found at distance 0: {found}/{len(rows)}")
+
+    scan = scan_leg(hkg, ix, Qh, kq, dQ, dk, kmax, dids, dd2, args, N, L) if args.scan_passes else {}
+    if scan:
+        log(f"scan: {scan}")
+    args.fp32_batch = min(args.fp32_batch, B)
+    args.fp32_steps = max(1, min(args.fp32_steps, (steps + warm) - 1))
+    fp32 = fp32_legs(hkg, ix, dQ, dk, kmax, dids, dd2, args, L, qps) if (args.fp32_batch or args.index_queries) else {}
+    if fp32:
+        log(f"fp32 paths: {fp32}")
 
     # the reference's epsilon and no-guarantee modes on one batch of the same
     # queries: throughput, and how their answers compare with the exact ones
@@ -481,11 +658,15 @@ def main() -> int:
 
     serving = None
     if args.serve_requests:
-        sq = args.serve_qps if args.serve_qps > 0 else 0.5 * qps
-        serving = serving_leg(hkg, ix, args, Qh, kq, sq)
-        log(f"serving: {serving}")
+        loads = [args.serve_qps] if args.serve_qps > 0 else [f * qps for f in map(float, args.serve_loads.split(","))]
+        serving = []
+        for sq in loads:
+            leg = serving_leg(hkg, ix, args, Qh, kq, sq)
+            leg["offered_fraction_of_throughput"] = round(sq / qps, 3)
+            serving.append(leg)
+            log(f"serving: {leg}")
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args, Qh, kq)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(hkg, args)
     if cpu:
         log(f"cpu baseline: {cpu}")
     tb = traffic_bytes(kernel_names.get(dom, dom), N, L, B)
@@ -499,6 +680,10 @@ def main() -> int:
         "steps": steps,
         "warmup": warm,
         "ms_per_step": elapsed * 1e3 / steps,
+        "step_ms_p50": float(np.percentile(step_ms, 50)),
+        "step_ms_p99": float(np.percentile(step_ms, 99)),
+        "step_ms_max": float(np.max(step_ms)),
+        "value_without_stage_stamps": qps_nostamps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -523,6 +708,8 @@ def main() -> int:
         "series_refined_per_query": refined / (steps * B),
         "series_lb_checked_per_query": checked / (steps * B),
         "parity": par,
+        "scan": scan,
+        "fp32_paths": fp32,
         "modes": modes,
         "real_queries": real,
         "loader": loader,

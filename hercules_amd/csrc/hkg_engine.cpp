@@ -509,17 +509,6 @@ uint32_t Index::filter_grid() const
   return cu_count() * per;
 }
 
-static uint32_t persistent_grid()
-{
-  static uint32_t g = 0;
-  if (!g) {
-    int dev = 0, cus = 0;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    g = (uint32_t)std::max(cus, 1) * 3;
-  }
-  return g;
-}
 
 void Index::enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf, bool seeded)
 {
@@ -622,14 +611,25 @@ uint64_t Index::total_scan_bytes(uint32_t nq, uint32_t batch) const
 
 uint64_t Index::total_segs() const { return dev.tree.seg_end.size(); }
 
-uint32_t scan_grid_x(uint32_t nq)
+uint32_t scan_grid_x(uint32_t nq, uint32_t blocks_per_cu)
 {
   // a multiple of 8 (the XCD count): block x of every query group runs on
   // the same XCD and reads the same rows
   uint32_t groups = (nq + scan_group_size(nq) - 1) / scan_group_size(nq);
-  uint32_t total = persistent_grid();
+  uint32_t total = cu_count() * std::max<uint32_t>(blocks_per_cu, 1);
   uint32_t gx = total / std::max<uint32_t>(groups, 1);
   return std::max<uint32_t>(gx / 8 * 8, 64);
+}
+
+// persistent grid of the full scan for the running call: blocks per CU from
+// the call, else the tuned shape, else kScanBlocksPerCU (any grid gives the
+// same answers)
+uint32_t Index::scan_grid(uint32_t nq) const
+{
+  uint32_t per = call_settings().sblocks;
+  if (!per) per = tuned_sblocks.load(std::memory_order_relaxed);
+  if (!per) per = kScanBlocksPerCU;
+  return scan_grid_x(nq, per);
 }
 
 void Index::enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf)
@@ -640,7 +640,7 @@ void Index::enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Pro
   launch_check_k(st, nq, b.kmax, b.kq, b.misc + 3);
   launch_fill(st, reinterpret_cast<uint32_t *>(b.counters), 2 * kNumCounters, 0);
   if (pf) pf->mark(st, -1);
-  launch_scan(st, scan_grid_x(nq), dev.X, dev.n, dev.L, b.Q, nq, b.kq, dev.ids, b.cap, b.cand, b.cand_cnt,
+  launch_scan(st, scan_grid(nq), dev.X, dev.n, dev.L, b.Q, nq, b.kq, dev.ids, b.cap, b.cand, b.cand_cnt,
               b.bound);
   if (pf) pf->mark(st, HKG_STAGE_SCAN);
   launch_select(st, nq, b.kmax, b.kq, b.cap, b.cand, b.cand_cnt, idmap64, b.out_ids, b.out_d2, overflow);
@@ -650,7 +650,7 @@ void Index::enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Pro
 // capacity of the per-query candidate buffer for a path
 uint32_t Index::cand_cap(hkg_path path, uint32_t nq, uint32_t kmax) const
 {
-  if (path == HKG_PATH_SCAN || !dev.has_tree) return scan_grid_x(nq) * kmax + 256;
+  if (path == HKG_PATH_SCAN || !dev.has_tree) return scan_grid(nq) * kmax + 256;
   const uint32_t cap_override = call_settings().cap_override;
   if (cap_override) return std::max<uint32_t>(cap_override, kmax);
   return std::max<uint32_t>(8192, 16 * kmax);
@@ -1148,9 +1148,10 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
       grids = {0};
       break;
     default:
-      batches = {16, 64, 256};
-      grids = {0};
+      batches = {1, 4, 16, 64, 256};
+      grids = {3, 6, 8};  // full-scan blocks per CU
       def_batch = 64;
+      def_grid = kScanBlocksPerCU;
       break;
   }
   // batches beyond the sample time the same as the sample-sized one
@@ -1167,9 +1168,13 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
   const size_t m = (size_t)nq * kmax;
   std::vector<uint64_t> ids0(m), ids(m);
   std::vector<float> d20(m), d2(m);
+  const bool scan_path = path == HKG_PATH_SCAN;
   auto run = [&](uint32_t batch, uint32_t grid, uint64_t *I, float *D, double *ms) -> hkg_status {
     qo.batch = batch;
-    call_settings().fblocks = grid;
+    if (scan_path)
+      call_settings().sblocks = grid;
+    else
+      call_settings().fblocks = grid;
     *ms = 1e300;
     for (int rep = 0; rep < 3; ++rep) {
       hkg_query_stats st;
@@ -1215,12 +1220,13 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
     }
   }
   tuned_batch[path].store(best_b, std::memory_order_relaxed);
-  if (best_g) tuned_fblocks.store(best_g, std::memory_order_relaxed);
+  if (best_g) (scan_path ? tuned_sblocks : tuned_fblocks).store(best_g, std::memory_order_relaxed);
   memset(out, 0, sizeof(*out));
   out->path = path;
   out->length = dev.L;
   out->batch = best_b;
-  out->filter_blocks_per_cu = best_g;
+  out->filter_blocks_per_cu = scan_path ? 0 : best_g;
+  out->scan_blocks_per_cu = scan_path ? best_g : 0;
   out->candidates = ncand;
   out->ms_per_query = best / nq;
   out->ms_per_query_default = ms_def / nq;

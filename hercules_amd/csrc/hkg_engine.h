@@ -158,6 +158,7 @@ struct Index {
   // of each path, and k_filter_refine blocks per CU
   std::atomic<uint32_t> tuned_batch[HKG_PATH_BF16 + 1] = {};
   std::atomic<uint32_t> tuned_fblocks{0};
+  std::atomic<uint32_t> tuned_sblocks{0};  // full-scan blocks per CU
 
   ~Index();
   hkg_status init_device(int device);
@@ -187,6 +188,8 @@ struct Index {
   uint64_t total_segs() const;
   // persistent grid of k_filter_refine for the running call
   uint32_t filter_grid() const;
+  // grid.x of the full scan for the running call (per query group)
+  uint32_t scan_grid(uint32_t nq) const;
   hkg_status autotune(const float *Q, uint32_t nq, const uint32_t *k, uint32_t kmax, uint32_t path,
                       hkg_tuning *out);
   hkg_status fix_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax, hipStream_t st, uint32_t *retries);
@@ -205,6 +208,7 @@ struct CallSettings {
   uint32_t min_buffer_batch = 0;  // size buffer sets for at least this many queries (server workers)
   bool mode = false;          // an approximate mode (epsilon > 0 or nprobes > 0) is on
   uint32_t fblocks = 0;       // k_filter_refine blocks per CU (0: the index's tuned or default)
+  uint32_t sblocks = 0;       // full-scan blocks per CU (0: the index's tuned or default)
   ModeParams mp{};            // its parameters (nleaves / nseries filled per index)
 };
 CallSettings &call_settings();
@@ -227,7 +231,8 @@ hkg_status noalloc_setup(int device);
 
 hkg_status open_dir(const char *dir, const hkg_options *opt, Index **out);
 hkg_status open_raw(const float *X, uint64_t n, uint32_t L, const hkg_options *opt, Index **out);
-uint32_t scan_grid_x(uint32_t nq);
+// full-scan grid.x for nq queries at blocks_per_cu resident blocks per CU
+uint32_t scan_grid_x(uint32_t nq, uint32_t blocks_per_cu);
 hkg_status build_device(float *dX, uint64_t n, uint32_t L, const hkg_build_opts &bo, Index **out);
 hkg_status write_dir(const Index &ix, const char *dir);
 

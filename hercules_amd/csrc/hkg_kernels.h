@@ -9,6 +9,7 @@ constexpr int kRefineTile = 2048;        // floats of chunk sums per wave (refin
 constexpr int kRefineTileFilter = 1024;  // the same in k_filter_refine (four blocks per CU)
 constexpr uint32_t kFilterBlocksPerCU = 4;  // persistent grid of k_filter_refine
 constexpr int kScanQ = 4;                // queries per block in the full scan
+constexpr uint32_t kScanBlocksPerCU = 3;  // default persistent grid of the full scan
 // queries per pass of the full scan for a batch of nq
 inline uint32_t scan_group_size(uint32_t nq) { return nq <= 2 ? (nq ? nq : 1) : (uint32_t)kScanQ; }
 constexpr size_t kEapcaLdsBytes = 48 * 1024;

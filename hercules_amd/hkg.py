@@ -82,7 +82,7 @@ class QueryStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("path", C.c_uint32), ("length", C.c_uint32), ("batch", C.c_uint32),
-                ("filter_blocks_per_cu", C.c_uint32), ("candidates", C.c_uint32), ("reserved", C.c_uint32),
+                ("filter_blocks_per_cu", C.c_uint32), ("candidates", C.c_uint32), ("scan_blocks_per_cu", C.c_uint32),
                 ("ms_per_query", C.c_double), ("ms_per_query_default", C.c_double)]
 
 
@@ -351,7 +351,8 @@ class HerculesIndex:
                                   C.byref(t)))
         names = {v: n for n, v in PATHS.items()}
         return {"path": names[t.path], "length": t.length, "batch": t.batch,
-                "filter_blocks_per_cu": t.filter_blocks_per_cu, "candidates": t.candidates,
+                "filter_blocks_per_cu": t.filter_blocks_per_cu, "scan_blocks_per_cu": t.scan_blocks_per_cu,
+                "candidates": t.candidates,
                 "ms_per_query": t.ms_per_query, "ms_per_query_default": t.ms_per_query_default}
 
     def knn_device(self, dQ: DeviceArray, nq: int, dk: DeviceArray, kmax: int, dids: DeviceArray,

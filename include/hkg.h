@@ -223,8 +223,9 @@ hkg_status hkg_scan_knn_device_noalloc(hkg_index *ix, const float *dQ, uint32_t 
    by hand (--num-threads / --num-read-threads, hercules.c:104-105,227;
    PSCAN's --num-query-threads / --dbsize, parallel_scan.c:64-65).  Here the
    shapes are timed on the index itself, for its series length: the queries
-   per device batch of a path and, on the FP32 index paths, the persistent
-   grid of the iSAX filter + refine kernel (blocks per CU).  Every candidate
+   per device batch of a path and the persistent grid of its main kernel
+   (blocks per CU): the iSAX filter + refine kernel on the FP32 index paths,
+   the scan kernel on the full scan.  Every candidate
    must return the same answers, bit for bit (HKG_EINTERNAL otherwise). */
 typedef struct hkg_tuning {
   uint32_t path;                 /* the path tuned (hkg_path; AUTO tunes BATCHED) */
@@ -233,7 +234,8 @@ typedef struct hkg_tuning {
   uint32_t filter_blocks_per_cu; /* chosen grid of the FP32 filter + refine
                                     (0: the path does not launch it)              */
   uint32_t candidates;           /* shapes timed                                  */
-  uint32_t reserved;
+  uint32_t scan_blocks_per_cu;   /* chosen grid of the full scan (scan path; 0
+                                    otherwise)                                    */
   double ms_per_query;           /* device time per query, the chosen shape       */
   double ms_per_query_default;   /* device time per query, the default shape      */
 } hkg_tuning;

@@ -11,7 +11,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-CANDIDATE_BATCHES = {"batched": {256, 512, 1024, 2048}, "bf16": {512, 1024, 2048, 4096}, "scan": {16, 64, 256},
+CANDIDATE_BATCHES = {"batched": {256, 512, 1024, 2048}, "bf16": {512, 1024, 2048, 4096}, "scan": {1, 4, 16, 64, 256},
                      "index": {1}}
 
 
@@ -41,9 +41,11 @@ def test_autotune_paths(hkg, oracle, kind, L):
             assert t["path"] == path and t["length"] == L
             assert t["batch"] in CANDIDATE_BATCHES[path]
             if path in ("batched", "index"):
-                assert t["filter_blocks_per_cu"] in (2, 4, 8)
+                assert t["filter_blocks_per_cu"] in (2, 4, 8) and t["scan_blocks_per_cu"] == 0
+            elif path == "scan":
+                assert t["scan_blocks_per_cu"] in (3, 6, 8) and t["filter_blocks_per_cu"] == 0
             else:
-                assert t["filter_blocks_per_cu"] == 0
+                assert t["filter_blocks_per_cu"] == 0 and t["scan_blocks_per_cu"] == 0
             assert t["candidates"] >= 1 and t["ms_per_query"] > 0 and t["ms_per_query_default"] > 0
             # the largest batch within 1% of the fastest shape is kept
             assert t["ms_per_query"] <= t["ms_per_query_default"] * 1.01
