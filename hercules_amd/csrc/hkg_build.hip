@@ -837,7 +837,7 @@ hkg_status hkg_build_device(float *dX, uint64_t n, uint32_t L, const hkg_build_o
   hkg_build_opts o;
   hkg_default_build_opts(&o);
   if (bo) o = *bo;
-  if (n == 0 || n >= (1ull << 31) || L == 0 || L % 16 != 0 || L > hkg::kMaxLength || o.paa_segments != 16) {
+  if (n == 0 || n >= (1ull << 31) || L == 0 || L % 16 != 0 || L > hkg::kMaxBuildLength || o.paa_segments != 16) {
     hkg::set_error("build: need 0 < n < 2^31, L a multiple of 16 up to 2048, 16 iSAX segments");
     return HKG_EINVAL;
   }

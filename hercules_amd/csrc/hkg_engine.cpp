@@ -154,6 +154,8 @@ hkg_status Index::init_device(int device)
   HKG_HIP_TRY(hipSetDevice(device));
   HKG_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   HKG_HIP_TRY(upload_breakpoints_dev());
+  set_lds_limits();
+  set_mode_lds_limits();
   return noalloc_setup(device);
 }
 
@@ -604,18 +606,18 @@ uint64_t Index::total_scan_bytes(uint32_t nq, uint32_t batch) const
   uint64_t passes = 0;
   for (uint32_t q0 = 0; q0 < nq; q0 += batch) {
     uint32_t n = std::min(batch, nq - q0);
-    passes += (n + scan_group_size(n) - 1) / scan_group_size(n);
+    passes += (n + scan_group_size(n, dev.L) - 1) / scan_group_size(n, dev.L);
   }
   return passes * dev.n * dev.L * 4;
 }
 
 uint64_t Index::total_segs() const { return dev.tree.seg_end.size(); }
 
-uint32_t scan_grid_x(uint32_t nq, uint32_t blocks_per_cu)
+uint32_t scan_grid_x(uint32_t nq, uint32_t L, uint32_t blocks_per_cu)
 {
   // a multiple of 8 (the XCD count): block x of every query group runs on
   // the same XCD and reads the same rows
-  uint32_t groups = (nq + scan_group_size(nq) - 1) / scan_group_size(nq);
+  uint32_t groups = (nq + scan_group_size(nq, L) - 1) / scan_group_size(nq, L);
   uint32_t total = cu_count() * std::max<uint32_t>(blocks_per_cu, 1);
   uint32_t gx = total / std::max<uint32_t>(groups, 1);
   return std::max<uint32_t>(gx / 8 * 8, 64);
@@ -629,7 +631,7 @@ uint32_t Index::scan_grid(uint32_t nq) const
   uint32_t per = call_settings().sblocks;
   if (!per) per = tuned_sblocks.load(std::memory_order_relaxed);
   if (!per) per = kScanBlocksPerCU;
-  return scan_grid_x(nq, per);
+  return scan_grid_x(nq, dev.L, per);
 }
 
 void Index::enqueue_scan_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf)

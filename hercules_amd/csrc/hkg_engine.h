@@ -231,8 +231,9 @@ hkg_status noalloc_setup(int device);
 
 hkg_status open_dir(const char *dir, const hkg_options *opt, Index **out);
 hkg_status open_raw(const float *X, uint64_t n, uint32_t L, const hkg_options *opt, Index **out);
-// full-scan grid.x for nq queries at blocks_per_cu resident blocks per CU
-uint32_t scan_grid_x(uint32_t nq, uint32_t blocks_per_cu);
+// full-scan grid.x for nq queries of length L at blocks_per_cu resident
+// blocks per CU
+uint32_t scan_grid_x(uint32_t nq, uint32_t L, uint32_t blocks_per_cu);
 hkg_status build_device(float *dX, uint64_t n, uint32_t L, const hkg_build_opts &bo, Index **out);
 hkg_status write_dir(const Index &ix, const char *dir);
 

@@ -17,7 +17,8 @@ constexpr int kPaaSegments = 16;    // iSAX words are 16 symbols (sax.c:320-344)
 constexpr int kMaxK = 256;          // per-query k
 constexpr int kTopKCap = 512;       // LDS merge buffer (list + staged)
 constexpr int kBlock = 256;         // threads per block of the query kernels
-constexpr uint32_t kMaxLength = 2048;  // longest series the chunk tiles hold
+constexpr uint32_t kMaxLength = 16384;  // longest series the query paths take (64 KB of LDS)
+constexpr uint32_t kMaxBuildLength = 2048;  // longest series the device builder takes
 constexpr uint32_t kNoId32 = 0xFFFFFFFFu;
 constexpr uint32_t kInfBits = 0x7F7FFFFFu;  // FLT_MAX bits: "no bound yet"
 

@@ -10,8 +10,14 @@ constexpr int kRefineTileFilter = 1024;  // the same in k_filter_refine (four bl
 constexpr uint32_t kFilterBlocksPerCU = 4;  // persistent grid of k_filter_refine
 constexpr int kScanQ = 4;                // queries per block in the full scan
 constexpr uint32_t kScanBlocksPerCU = 3;  // default persistent grid of the full scan
-// queries per pass of the full scan for a batch of nq
-inline uint32_t scan_group_size(uint32_t nq) { return nq <= 2 ? (nq ? nq : 1) : (uint32_t)kScanQ; }
+// queries per pass of the full scan for a batch of nq series of length L:
+// up to kScanQ, fewer where their rows would not fit 64 KB of LDS
+inline uint32_t scan_group_size(uint32_t nq, uint32_t L)
+{
+  const uint32_t fit = L <= 4096 ? 4u : (L <= 8192 ? 2u : 1u);
+  const uint32_t g = nq <= 2 ? (nq ? nq : 1) : (uint32_t)kScanQ;
+  return g < fit ? g : fit;
+}
 constexpr size_t kEapcaLdsBytes = 48 * 1024;
 constexpr int kWorkChunk = 8;            // work items taken per atomic
 constexpr int kFilterRows = 1024;        // rows per iSAX filter step of k_filter_refine
@@ -47,6 +53,11 @@ constexpr size_t kModeLdsMax = 144 * 1024;
 bool mode_heap_in_lds(uint32_t L, uint32_t nnodes);
 
 hipError_t upload_breakpoints_dev();
+// the query kernels that hold a query (or a group of them) in LDS may use
+// up to kQueryLdsMax bytes of dynamic LDS (series up to kMaxLength)
+constexpr size_t kQueryLdsMax = 96 * 1024;
+void set_lds_limits();
+void set_mode_lds_limits();
 
 void launch_reset(hipStream_t st, uint32_t nq, uint32_t *bound, uint32_t *cand_cnt, const uint32_t *seed);
 // per-query k in place: values outside [1, kmax] are clamped (so no kernel

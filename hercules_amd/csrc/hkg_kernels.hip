@@ -199,6 +199,7 @@ struct RefineSharedT {
 };
 using RefineShared = RefineSharedT<kRefineTile>;
 
+// nrefined: lane 0 of every wave adds the rows its wave refined.
 template <int TILE>
 __device__ void refine_rows(const float *__restrict__ X, uint32_t L, const float *qs,
                             const uint64_t *rows, int nrows, const uint32_t *__restrict__ ids,
@@ -207,11 +208,46 @@ __device__ void refine_rows(const float *__restrict__ X, uint32_t L, const float
   const int cpr = (int)(L >> 3);
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
+  float *cs = rs.csum[wave];
+  if (cpr + 1 > TILE) {
+    // rows longer than the tile: one row per wave at a time, its chunk sums
+    // TILE at a time, folded in order into a running sum by lane 0 (every
+    // wave runs the same number of rounds, so the block barriers match)
+    for (int base = 0; base < nrows; base += kBlock / 64) {
+      const int r = base + wave;
+      const bool have = r < nrows;
+      const uint64_t row = have ? rows[r] : 0;
+      float acc = 0.0f;
+      for (int c0 = 0; c0 < cpr; c0 += TILE) {
+        const int nc = cpr - c0 < TILE ? cpr - c0 : TILE;
+        if (have) {
+          for (int c = lane; c < nc; c += 64) {
+            const float *xp = X + row * L + (uint64_t)(c0 + c) * 8;
+            const float4 xa = ld_stream(xp);
+            const float4 xb = ld_stream(xp + 4);
+            const float4 *qp = reinterpret_cast<const float4 *>(qs + (c0 + c) * 8);
+            cs[c] = ed_chunk(qp[0], qp[1], xa, xb);
+          }
+        }
+        __syncthreads();
+        if (have && lane == 0)
+          for (int c = 0; c < nc; ++c) acc = acc + cs[c];
+        __syncthreads();
+      }
+      if (have && lane == 0) {
+        if (__float_as_uint(acc) <= bound_bits) {
+          const uint32_t id = ids ? ids[row] : (uint32_t)row;
+          topk_stage(tk, make_key(acc, id));
+        }
+        ++nrefined;
+      }
+    }
+    return;
+  }
   // rows per wave group: the padded chunk tile must fit TILE floats
   int g = TILE / (cpr + 1);
   if (g > 64) g = 64;
   const int per_block = g * (kBlock / 64);
-  float *cs = rs.csum[wave];
   for (int base = 0; base < nrows; base += per_block) {
     int wbeg = base + wave * g;
     int cnt = nrows - wbeg;
@@ -239,7 +275,7 @@ __device__ void refine_rows(const float *__restrict__ X, uint32_t L, const float
         topk_stage(tk, make_key(acc, id));
       }
     }
-    nrefined += (lane < cnt) ? 1u : 0u;
+    nrefined += lane == 0 ? (uint32_t)cnt : 0u;
     __syncthreads();
   }
 }
@@ -1120,7 +1156,7 @@ void launch_scan(hipStream_t st, uint32_t grid_x, const float *X, uint64_t n, ui
                  uint32_t *cand_cnt, uint32_t *bound)
 {
   // queries per pass: up to kScanQ (a lone query scans alone)
-  const uint32_t qn = scan_group_size(nq);
+  const uint32_t qn = scan_group_size(nq, L);
   const uint32_t groups = (nq + qn - 1) / qn;
   const size_t lds = (size_t)qn * L * sizeof(float);
   switch (qn) {
@@ -1128,6 +1164,18 @@ void launch_scan(hipStream_t st, uint32_t grid_x, const float *X, uint64_t n, ui
   case 2: hipLaunchKernelGGL(k_scan<2>, dim3(grid_x, groups), dim3(kBlock), lds, st, X, n, L, Q, nq, kq, ids, cap, cand, cand_cnt, bound); break;
   default: hipLaunchKernelGGL(k_scan<kScanQ>, dim3(grid_x, groups), dim3(kBlock), lds, st, X, n, L, Q, nq, kq, ids, cap, cand, cand_cnt, bound); break;
   }
+}
+
+// for the current device: the query kernels may take kQueryLdsMax bytes of
+// dynamic LDS (a series of 16384 floats is 64 KB)
+void set_lds_limits()
+{
+  const void *fns[] = {reinterpret_cast<const void *>(k_scan_best), reinterpret_cast<const void *>(k_filter_refine),
+                       reinterpret_cast<const void *>(k_scan<1>), reinterpret_cast<const void *>(k_scan<2>),
+                       reinterpret_cast<const void *>(k_scan<kScanQ>)};
+  for (const void *f : fns)
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kQueryLdsMax);
+  (void)hipGetLastError();
 }
 
 void launch_select(hipStream_t st, uint32_t nq, uint32_t kmax, const uint32_t *kq, uint32_t cap,
@@ -1307,8 +1355,11 @@ __device__ void scan_rows(const float *__restrict__ X, uint32_t L, const float *
         alive = __float_as_uint(acc) <= bnd;
       }
       pass = alive;
-      ++nref;
     }
+    // rows examined: lane 0 of every wave counts its wave's (its nref is
+    // the one reported)
+    const uint64_t live = __ballot(r < end);
+    if ((threadIdx.x & 63) == 0) nref += (uint32_t)__popcll(live);
     const uint64_t bal = __ballot(pass);
     if ((threadIdx.x & 63) == 0 && bal) atomicAdd(&s_pass, (uint32_t)__popcll(bal));
     __syncthreads();
@@ -1732,6 +1783,17 @@ bool mode_heap_in_lds(uint32_t L, uint32_t nnodes)
   return mode_qs_bytes(L) + mode_heap_bytes(nnodes) <= kModeLdsMax;
 }
 
+// the mode kernels that hold the query in LDS (as set_lds_limits)
+void set_mode_lds_limits()
+{
+  const void *fns[] = {reinterpret_cast<const void *>(k_node_lb), reinterpret_cast<const void *>(k_mode_finish)};
+  for (const void *f : fns)
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kQueryLdsMax);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_mode_search), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kModeLdsMax);
+  (void)hipGetLastError();
+}
+
 void launch_mode_search(hipStream_t st, uint32_t nq, const float *X, const uint8_t *sax, uint32_t L, const float *Q,
                         const float *paa, const uint32_t *kq, uint32_t nnodes, const int4 *nd_link,
                         const int4 *nd_split, const float *nlb, uint32_t nleaf, const uint64_t *leaf_pos,
@@ -1742,12 +1804,6 @@ void launch_mode_search(hipStream_t st, uint32_t nq, const float *X, const uint8
 {
   const bool in_lds = mode_heap_in_lds(L, nnodes);
   const size_t lds = mode_qs_bytes(L) + (in_lds ? mode_heap_bytes(nnodes) : 0);
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_mode_search),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kModeLdsMax);
-    attr = true;
-  }
   hipLaunchKernelGGL(k_mode_search, dim3(nq), dim3(kBlock), lds, st, X, sax, L, Q,
                      paa, kq, nnodes, nd_link, nd_split, nlb, nleaf, leaf_pos, leaf_size, leaf_node, ids, lb, heap_g,
                      in_lds ? 1 : 0, bfs, mp, cap, cand, cand_cnt, bound, keys, nwork, info, counters, mstate, vtop,

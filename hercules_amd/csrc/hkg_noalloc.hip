@@ -75,7 +75,7 @@ hkg_status scan_knn_noalloc(Index &ix, const float *dQ, uint32_t nq, const uint3
   // the candidate buffer holds every block's flushed list: it cannot
   // overflow (one list of at most k entries per block of the scan grid).
   // The grid is the default one, never a tuned one, so the limit holds.
-  const uint32_t gx = scan_grid_x(nq, kScanBlocksPerCU);
+  const uint32_t gx = scan_grid_x(nq, dv.L, kScanBlocksPerCU);
   const uint32_t cap = gx * kmax + 256;
   const size_t need = noalloc_need(nq, cap);
   if (need > kNoAllocBytes) {

@@ -173,7 +173,11 @@ void hkg_default_query_opts(hkg_query_opts *q);
 hkg_status hkg_open_dir(const char *index_dir, const hkg_options *opt, hkg_index **out);
 
 /* A flat dataset for the full scan only (no tree); X is host memory,
-   row-major n x L float32, L % 8 == 0. */
+   row-major n x L float32, L % 8 == 0.
+   Series lengths: the query paths take L up to 16384 (the paper's longest);
+   the BF16 path takes L = 96, 128 and 256; the device builder (hkg_build*)
+   takes L up to 2048 (longer indexes come from the reference's builder and
+   hkg_open_dir). */
 hkg_status hkg_open_raw(const float *X, uint64_t n, uint32_t L, const hkg_options *opt,
                         hkg_index **out);
 
