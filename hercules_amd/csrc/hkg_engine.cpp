@@ -116,6 +116,7 @@ Index::~Index()
   bufs.clear();
   dfree(dev.X);
   dfree(dev.sax);
+  dfree(dev.lohi);
   dfree(dev.ids);
   dfree(dev.leaf_pos);
   dfree(dev.leaf_size);
@@ -154,6 +155,8 @@ hkg_status Index::init_device(int device)
   HKG_HIP_TRY(hipSetDevice(device));
   HKG_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   HKG_HIP_TRY(upload_breakpoints_dev());
+  hkg_status rc = upload_isax_tables(dev);
+  if (rc != HKG_OK) return rc;
   set_lds_limits();
   set_mode_lds_limits();
   return noalloc_setup(device);
@@ -377,6 +380,7 @@ QueryBuffers::~QueryBuffers()
   if (sort_tmp) (void)hipFree(sort_tmp);
   dfree(seed);
   free_bf16();
+  free_isax();
   dfree(misc);
   dfree(counters);
   dfree(out_ids);
@@ -655,6 +659,9 @@ uint32_t Index::cand_cap(hkg_path path, uint32_t nq, uint32_t kmax) const
   if (path == HKG_PATH_SCAN || !dev.has_tree) return scan_grid(nq) * kmax + 256;
   const uint32_t cap_override = call_settings().cap_override;
   if (cap_override) return std::max<uint32_t>(cap_override, kmax);
+  // the batched iSAX filter keeps every series within the bound of the
+  // approximate stage, without a per-query top-k
+  if (path == HKG_PATH_BATCHED && dev.has_tree && isax_batched(nq)) return std::max<uint32_t>(32768, 16 * kmax);
   return std::max<uint32_t>(8192, 16 * kmax);
 }
 
@@ -968,9 +975,12 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
       pf->n = 0;
       pf->src = nullptr;
     }
+    const bool isax = path == HKG_PATH_BATCHED && !mode && dev.has_tree && isax_batched(n);
     if (path == HKG_PATH_BF16)
       rc = run_bf16_batch(dQ + (uint64_t)q0 * dev.L, dk + q0, n, kmax, st, &b, &c, pf.get(), nullptr,
                           o.bf16_bound_pass == 1, n == batch);
+    else if (isax)
+      rc = run_isax_batch(dQ + (uint64_t)q0 * dev.L, dk + q0, n, kmax, st, &b, &c, pf.get(), nullptr, n == batch);
     else
       rc = run_batch(path, dQ + (uint64_t)q0 * dev.L, dk + q0, n, kmax, path == HKG_PATH_BATCHED && n == batch, st,
                      &b, &c, pf.get());
@@ -990,7 +1000,10 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
       HKG_HIP_TRY(hipMemcpy(dbg_cnt.data(), b->bf_cnt, n * 4, hipMemcpyDeviceToHost));
       HKG_HIP_TRY(hipMemcpy(dbg_thr.data(), b->thr, n * 4, hipMemcpyDeviceToHost));
     }
+    c.isax = isax;
     if (c.overflow) {
+      // the batched iSAX path's candidate buffers overflow like the
+      // per-item filter's, and re-run through it (fix_overflow)
       rc = path == HKG_PATH_BF16 ? fix_bf16_overflow(*b, n, kmax, st, &retries)
                                  : fix_overflow(*b, n, kmax, st, &retries);
       if (rc != HKG_OK) return rc;
@@ -1006,6 +1019,7 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     HKG_HIP_TRY(hipStreamSynchronize(st));  // before the set is released
     for (int i = 0; i < kNumCounters; ++i) total.v[i] += c.v[i];
     total.nwork += c.nwork;
+    total.isax |= c.isax;
     total.bf16_rows += c.bf16_rows;
     total.bf16_rows_bound += c.bf16_rows_bound;
   }
@@ -1038,9 +1052,14 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     }
     const uint64_t L4 = (uint64_t)dev.L * 4;
     stats->stage_bytes[HKG_STAGE_APPROX] = total.v[kCtrApprox] * L4;
+    // the leaf-ordered iSAX filter reads each candidate leaf's words once per
+    // wave chunk (up to 64 queries), and refines its survivors (at most 4 L
+    // bytes each: they are abandoned early)
     stats->stage_bytes[HKG_STAGE_REFINE] =
         path == HKG_PATH_BF16 ? total.bf16_rows * ((uint64_t)dev.L * 2 + 16)
+        : total.isax          ? total.v[kCtrSaxWords] * 16
                               : total.v[kCtrSaxChecked] * 16 + total.v[kCtrRefined] * L4;
+    if (total.isax) stats->stage_bytes[HKG_STAGE_REFINE] += total.v[kCtrRefined] * L4;
     if (path == HKG_PATH_BF16) {
       stats->stage_bytes[HKG_STAGE_TOPK] = total.v[kCtrRefined] * L4;
       stats->stage_bytes[HKG_STAGE_BOUND] = total.bf16_rows_bound * ((uint64_t)dev.L * 2 + 16);

@@ -24,6 +24,7 @@ struct Counters {
   uint32_t bad_k = 0;        // per-query k outside [1, kmax] (clamped on the device)
   uint64_t bf16_rows = 0;    // rows the BF16 collecting pass read (summed over block groups)
   uint64_t bf16_rows_bound = 0;  // rows the BF16 histogram pass read
+  bool isax = false;         // the batch ran the leaf-ordered iSAX filter
 };
 
 // per-stage timing of one batch (profile mode): a one-wave kernel stamps the
@@ -121,6 +122,9 @@ struct QueryBuffers {
   uint32_t *bf_region_cnt = nullptr;  // per filter wave: pre-test passes in its region slice
   uint32_t *bf_prog = nullptr;  // per filter block: tiles done (sibling blocks keep pace)
   void free_bf16();
+  // batched iSAX path scratch
+  float *is_thr = nullptr;      // B: filter threshold
+  void free_isax();
   // approximate modes
   float *nlb = nullptr;         // B x nnodes exact node bounds
   uint64_t *heap = nullptr;     // B x (nnodes + 1) traversal heaps (when too big for LDS)
@@ -179,6 +183,11 @@ struct Index {
   hkg_status run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax, hipStream_t st,
                             QueryBuffers **used, Counters *ctr, Prof *pf, const uint32_t *seed_host,
                             bool bound_pass, bool use_graph = false);
+  // the batched FP32 path: leaf-ordered iSAX filter + refinement in bound
+  // order (hkg_isax.hip)
+  hkg_status run_isax_batch(const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax, hipStream_t st,
+                            QueryBuffers **used, Counters *ctr, Prof *pf, const uint32_t *seed_host,
+                            bool use_graph = false);
   // launch the stage sequence `enqueue` through the buffer set's captured
   // graph (captured again when kind, nq, flags, grid, stream or profiling
   // change); with pf, its stage events are the graph's own
@@ -214,6 +223,10 @@ struct CallSettings {
 CallSettings &call_settings();
 // BF16 path scratch of one query beyond the FP32 buffer set
 uint64_t bf16_scratch_per_query(uint32_t L, uint64_t nleaf, uint64_t ntiles);
+// batched iSAX path (hkg_isax.hip): whether a batch of nq takes it, and its
+// breakpoint table (once per device)
+bool isax_batched(uint32_t nq);
+hkg_status upload_isax_tables(DevIndex &dv);
 
 // releases a held buffer set when it goes out of scope
 struct BufLease {

@@ -78,6 +78,7 @@ struct DevIndex {
   // SoA in HBM
   float *X = nullptr;          // n x L, leaf order
   uint8_t *sax = nullptr;      // n x 16
+  float2 *lohi = nullptr;      // 256 {lower, upper} breakpoints per iSAX symbol
   uint32_t *ids = nullptr;     // n, public id per row (nullable: row = id)
   uint64_t *leaf_pos = nullptr;   // nleaf
   uint32_t *leaf_size = nullptr;  // nleaf

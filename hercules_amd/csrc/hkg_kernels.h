@@ -37,6 +37,7 @@ enum Counter : int {
   kCtrModeSeq = 8,     // approximate modes: queries whose candidates went sequential
   kCtrApproxLeaves = 9,  // (query, leaf) pairs scanned in the approximate stage
   kCtrLeafPairs = 10,  // BF16 path: (query, leaf) pairs within the k-th bound after the approximate stage
+  kCtrSaxWords = 11,   // batched iSAX path: iSAX words streamed (once per leaf per wave chunk)
   kNumCounters = 12
 };
 

@@ -13,6 +13,8 @@
 
 #include <float.h>
 
+#include "hkg_ed.h"
+
 #include <hipcub/hipcub.hpp>
 
 namespace hkg {
@@ -36,25 +38,6 @@ hipError_t upload_breakpoints_dev()
 // ---------------------------------------------------------------------------
 // device helpers
 // ---------------------------------------------------------------------------
-typedef float nf4 __attribute__((ext_vector_type(4)));
-
-// streaming 16-byte load (the rows are touched once per query)
-__device__ __forceinline__ float4 ld_stream(const float *p)
-{
-  nf4 v = __builtin_nontemporal_load(reinterpret_cast<const nf4 *>(p));
-  return make_float4(v.x, v.y, v.z, v.w);
-}
-
-// one 8-float chunk of ts_euclidean_distance_SIMD: separate sub and mul
-// roundings, then the two-hadd tree ((d0+d1)+(d2+d3)) + ((d4+d5)+(d6+d7))
-__device__ __forceinline__ float ed_chunk(float4 qa, float4 qb, float4 xa, float4 xb)
-{
-  float d0 = qa.x - xa.x, d1 = qa.y - xa.y, d2 = qa.z - xa.z, d3 = qa.w - xa.w;
-  float d4 = qb.x - xb.x, d5 = qb.y - xb.y, d6 = qb.z - xb.z, d7 = qb.w - xb.w;
-  d0 = d0 * d0; d1 = d1 * d1; d2 = d2 * d2; d3 = d3 * d3;
-  d4 = d4 * d4; d5 = d5 * d5; d6 = d6 * d6; d7 = d7 * d7;
-  return ((d0 + d1) + (d2 + d3)) + ((d4 + d5) + (d6 + d7));
-}
 
 // minidist_paa_to_isax_raw_SIMD for full-cardinality 16-symbol words
 __device__ __forceinline__ float lb_sax_word(const float *paa, uint4 w, const float *bp, float ratio)
@@ -92,10 +75,6 @@ __device__ __forceinline__ bool lb_keep(float lb, uint32_t bound_bits)
   return lb <= b * (1.0f + kLbRel) + kLbAbs;
 }
 
-__device__ __forceinline__ uint64_t make_key(float d2, uint32_t id)
-{
-  return ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)id;
-}
 
 // ---------------------------------------------------------------------------
 // block top-k of distinct distances (256 threads)
@@ -278,6 +257,46 @@ __device__ void refine_rows(const float *__restrict__ X, uint32_t L, const float
     nrefined += lane == 0 ? (uint32_t)cnt : 0u;
     __syncthreads();
   }
+}
+
+// Exact distances of up to kBlock rows held in LDS, one row per thread,
+// streamed in 128-byte steps and folded in the reference's order
+// (ts.c:83-109), abandoned once the partial sum exceeds bound_bits (partial
+// sums only grow, so no row within the bound is lost); rows within the bound
+// are staged into tk.  For candidates most of which lie beyond the bound.
+__device__ void refine_rows_abandon(const float *__restrict__ X, uint32_t L, const float *qs, const uint64_t *rows,
+                                    int nrows, const uint32_t *__restrict__ ids, uint32_t bound_bits, TopK &tk,
+                                    uint32_t &nrefined)
+{
+  const int nch = (int)(L >> 3);
+  const float4 *q4 = reinterpret_cast<const float4 *>(qs);
+  const int t = threadIdx.x;
+  if (t < nrows) {
+    const uint64_t row = rows[t];
+    const float4 *xp = reinterpret_cast<const float4 *>(X + row * L);
+    float acc = 0.0f;
+    bool alive = true;
+    int c = 0;
+    for (; c + 4 <= nch && alive; c += 4) {
+      float4 xv[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) xv[i] = ld_stream(reinterpret_cast<const float *>(xp + 2 * c + i));
+#pragma unroll
+      for (int sub = 0; sub < 4; ++sub)
+        acc = acc + ed_chunk(q4[2 * (c + sub)], q4[2 * (c + sub) + 1], xv[2 * sub], xv[2 * sub + 1]);
+      alive = __float_as_uint(acc) <= bound_bits;
+    }
+    for (; c < nch && alive; ++c) {
+      const float4 xa = ld_stream(reinterpret_cast<const float *>(xp + 2 * c));
+      const float4 xb = ld_stream(reinterpret_cast<const float *>(xp + 2 * c + 1));
+      acc = acc + ed_chunk(q4[2 * c], q4[2 * c + 1], xa, xb);
+      alive = __float_as_uint(acc) <= bound_bits;
+    }
+    if (alive) topk_stage(tk, make_key(acc, ids ? ids[row] : (uint32_t)row));
+  }
+  // lane 0 of every wave counts its wave's rows (its count is the one reported)
+  const uint64_t live = __ballot(t < nrows);
+  if ((threadIdx.x & 63) == 0) nrefined += (uint32_t)__popcll(live);
 }
 
 // ---------------------------------------------------------------------------
@@ -873,13 +892,14 @@ __global__ void __launch_bounds__(kBlock) k_select(uint32_t kmax, const uint32_t
   }
 }
 
-// BF16 path, last stage: one block per query merges the approximate stage's
-// candidates and the exact distances of the BF16 survivors into the final
-// top-k, refining survivors in two passes guided by a histogram of their
-// bounds: first the ~kRefineFirst smallest bounds (which fixes a k-th
-// distance close to the final one), then only the survivors whose bound is
-// within the current k-th distance.  Every survivor whose bound is within the
-// final k-th distance is refined.  The survivors are every series whose
+// Last stage of the BF16 path and of the batched iSAX path: one block per
+// query merges the approximate stage's candidates and the exact distances of
+// the survivors of the lower bound into the final top-k, refining survivors
+// in passes of increasing bound, guided by a histogram of their bounds:
+// first the ~kRefineFirst smallest bounds (which fixes a k-th distance close
+// to the final one), then about 4x as many, and so on, each pass taking only
+// survivors whose bound is within the current k-th distance.  Every survivor
+// whose bound is within the final k-th distance is refined.  The survivors are every series whose
 // bound is within `bound` (an upper bound of the k-th distance counted over
 // series), so the answer is exact when no survivor was dropped for lack of
 // room and the final k-th distinct distance is within `bound`; otherwise the
@@ -887,6 +907,7 @@ __global__ void __launch_bounds__(kBlock) k_select(uint32_t kmax, const uint32_t
 // this k-th distance.
 constexpr int kSurvBins = 256;
 constexpr uint32_t kRefineFirst = 2048;
+constexpr int kRefinePasses = 4;
 
 __global__ void __launch_bounds__(kBlock) k_bf16_refine(
     const float *__restrict__ X, uint32_t L, const float *__restrict__ Q, const uint32_t *__restrict__ kq,
@@ -898,11 +919,11 @@ __global__ void __launch_bounds__(kBlock) k_bf16_refine(
 {
   extern __shared__ float qs[];
   __shared__ TopK tk;
-  __shared__ RefineShared rs;
   __shared__ uint64_t rows[kBlock];
   __shared__ uint32_t scan[kBlock];
   __shared__ uint32_t hist[kSurvBins];
-  __shared__ uint32_t s_nsel, s_bin;
+  __shared__ uint32_t s_nsel;
+  __shared__ int s_pass_bin[kRefinePasses];
   const uint32_t q = blockIdx.x;
   const int k = (int)kq[q];
   load_query(Q, q, L, qs);
@@ -910,6 +931,8 @@ __global__ void __launch_bounds__(kBlock) k_bf16_refine(
   for (int i = threadIdx.x; i < kSurvBins; i += kBlock) hist[i] = 0;
   __syncthreads();
   uint32_t nc = cand_cnt[q];
+  // candidates of the approximate stage that found no room: answered again
+  const bool cand_dropped = nc > cap;
   if (nc > cap) nc = cap;
   for (uint32_t t0 = 0; t0 < nc; t0 += kBlock) {
     uint32_t i = t0 + threadIdx.x;
@@ -921,8 +944,8 @@ __global__ void __launch_bounds__(kBlock) k_bf16_refine(
     topk_merge(tk, scan, k);
   }
   uint32_t ns = scnt[q];
-  const bool dropped = ns > scap;
-  if (dropped) ns = scap;
+  const bool dropped = ns > scap || cand_dropped;
+  if (ns > scap) ns = scap;
   const uint64_t *sv = surv + (uint64_t)q * scap;
   const float t0v = thr0[q];
   const float scale = t0v < FLT_MAX && t0v > 0.0f ? (float)kSurvBins / t0v : 0.0f;
@@ -934,18 +957,24 @@ __global__ void __launch_bounds__(kBlock) k_bf16_refine(
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t cum = 0;
-    int b = 0;
-    for (; b < kSurvBins - 1; ++b) {
+    // last bin of every pass: the pass reaches kRefineFirst * 4^pass
+    // survivors counted from the smallest bound (the last pass: the rest)
+    uint32_t cum = 0, want = kRefineFirst;
+    int b = 0, p = 0;
+    for (; b < kSurvBins - 1 && p < kRefinePasses - 1; ++b) {
       cum += hist[b];
-      if (cum >= kRefineFirst) break;
+      while (p < kRefinePasses - 1 && cum >= want) {
+        s_pass_bin[p++] = b;
+        want *= 4;
+      }
     }
-    s_bin = (uint32_t)b;
+    for (; p < kRefinePasses; ++p) s_pass_bin[p] = kSurvBins - 1;
   }
   __syncthreads();
-  const int first_bin = (int)s_bin;
   uint32_t nref = 0;
-  for (int pass = 0; pass < 2; ++pass) {
+  for (int pass = 0; pass < kRefinePasses; ++pass) {
+    const int bin_lo = pass == 0 ? -1 : s_pass_bin[pass - 1], bin_hi = s_pass_bin[pass];
+    if (bin_hi <= bin_lo) continue;  // uniform: an empty pass
     // the list is read four rounds at a time (four loads in flight per
     // thread): a round's work is short, so one load per round would leave
     // a long list waiting on memory latency
@@ -969,8 +998,7 @@ __global__ void __launch_bounds__(kBlock) k_bf16_refine(
         e = u == 0 ? ev[0] : u == 1 ? ev[1] : u == 2 ? ev[2] : ev[3];
         const float lbv = __uint_as_float((uint32_t)(e >> 32));
         const int bin = min(kSurvBins - 1, (int)(lbv * scale));
-        const bool in_first = bin <= first_bin;
-        take = (pass == 0 ? in_first : !in_first) && lb_keep(lbv, bnd);
+        take = bin > bin_lo && bin <= bin_hi && lb_keep(lbv, bnd);
       }
       const uint64_t m = __ballot(take);
       const int lane = threadIdx.x & 63;
@@ -982,7 +1010,7 @@ __global__ void __launch_bounds__(kBlock) k_bf16_refine(
       }
       if (take) rows[wb + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)e;
       __syncthreads();
-      refine_rows(X, L, qs, rows, (int)s_nsel, ids, bnd, tk, rs, nref);
+      refine_rows_abandon(X, L, qs, rows, (int)s_nsel, ids, bnd, tk, nref);
       __syncthreads();
       topk_merge(tk, scan, k);
     }
@@ -1171,6 +1199,7 @@ void launch_scan(hipStream_t st, uint32_t grid_x, const float *X, uint64_t n, ui
 void set_lds_limits()
 {
   const void *fns[] = {reinterpret_cast<const void *>(k_scan_best), reinterpret_cast<const void *>(k_filter_refine),
+                       reinterpret_cast<const void *>(k_bf16_refine),
                        reinterpret_cast<const void *>(k_scan<1>), reinterpret_cast<const void *>(k_scan<2>),
                        reinterpret_cast<const void *>(k_scan<kScanQ>)};
   for (const void *f : fns)

@@ -158,7 +158,9 @@ def test_mixed_k_and_determinism(hkg, oracle, small_data, ref_index):
 
 def test_overflow_rerun_stays_exact(hkg, oracle, small_data, ref_index):
     """A tiny candidate buffer forces overflows; the seeded re-runs (and the
-    full-scan last resort) must still give the exact answers."""
+    full-scan last resort) must still give the exact answers, on the
+    leaf-ordered filter of a whole batch and on the per-item filter of small
+    batches."""
     X, Q = small_data
     idx, _ = ref_index
     _say("overflow re-run")
@@ -166,8 +168,10 @@ def test_overflow_rerun_stays_exact(hkg, oracle, small_data, ref_index):
     with hkg.HerculesIndex.open(idx) as ix:
         ix.set_ids(row_map(idx, X))
         r = ix.knn(Q, 50, path="batched", cand_cap=64)
-    assert r.stats["overflow_retries"] > 0
-    assert_same(r.ids, r.d2, ids, d2, "overflow")
+        assert r.stats["overflow_retries"] > 0
+        assert_same(r.ids, r.d2, ids, d2, "overflow")
+        r = ix.knn(Q, 50, path="batched", batch=8, cand_cap=64)
+        assert_same(r.ids, r.d2, ids, d2, "overflow, batches of 8")
 
 
 def test_profile_stages(hkg, small_data, ref_index):
