@@ -68,6 +68,7 @@ def parse() -> argparse.Namespace:
                    help="offered load of the serving leg (0: a sweep at --serve-loads of the measured throughput)")
     p.add_argument("--serve-loads", default="0.5,0.8,0.95",
                    help="serving sweep: offered loads as fractions of the measured throughput")
+    p.add_argument("--serve-inflight", type=int, default=2, help="serving: batches on the device at once")
     p.add_argument("--serve-requests", type=int, default=4096, help="requests of the serving leg (0: skip)")
     p.add_argument("--mode-epsilon", type=float, default=1.0,
                    help="epsilon of the approximate-mode leg (0: skip that leg)")
@@ -278,7 +279,7 @@ def serving_leg(hkg, ix, args, Qh: np.ndarray, kq: np.ndarray, qps: float) -> di
     arrivals = np.cumsum(rng.exponential(1.0 / qps, nreq))
     rows = rng.integers(0, Qh.shape[0], nreq)
     kmax = int(kq.max())
-    srv = hkg.Server(ix, max_batch=args.batch, path=args.path, max_inflight=2, max_k=kmax,
+    srv = hkg.Server(ix, max_batch=args.batch, path=args.path, max_inflight=args.serve_inflight, max_k=kmax,
                      bf16_bound_pass=args.bf16_bound_pass, scratch_bytes=32 << 30)
     tickets = [None] * nreq
     ready = threading.Semaphore(0)
@@ -317,7 +318,7 @@ def serving_leg(hkg, ix, args, Qh: np.ndarray, kq: np.ndarray, qps: float) -> di
     return {"offered_qps": round(qps, 1), "achieved_qps": round(nreq / wall, 1), "requests": nreq,
             "p50_ms": round(float(np.percentile(lat, 50)) / 1e3, 3),
             "p99_ms": round(float(np.percentile(lat, 99)) / 1e3, 3),
-            "max_batch": args.batch, "inflight": 2, "path": args.path, "scratch_gib": 32,
+            "max_batch": args.batch, "inflight": args.serve_inflight, "path": args.path, "scratch_gib": 32,
             "answers_identical_to_direct_queries": same, "sampled": int(len(sample))}
 
 
@@ -365,16 +366,22 @@ def fp32_legs(hkg, ix, dQ, dk, kmax: int, dids, dd2, args, L: int, bf16_qps: flo
         wall = time.perf_counter() - t
         ref_ms = sum(st["stage_ms"]["refine"] for st in sts) / len(sts)
         ref_b = sum(st["stage_bytes"]["refine"] for st in sts) / len(sts)
+        tb = traffic_bytes("k_isax_filter", args.n, L, B)
         out["batched"] = {
             "batch": B, "steps": args.fp32_steps, "queries_per_s": args.fp32_steps * B / wall,
             "ms_per_step": wall * 1e3 / args.fp32_steps,
             "stages_ms": {kk: round(sum(st["stage_ms"][kk] for st in sts) / len(sts), 3)
                           for kk in sts[0]["stage_ms"] if sts[0]["stage_ms"][kk] > 0},
-            "k_filter_refine_ms": ref_ms,
-            # per query: 16 B of iSAX word per checked series + 4 L B per refined series
-            "k_filter_refine_bytes_per_launch": ref_b,
-            "k_filter_refine_gbs": ref_b / (ref_ms * 1e6) if ref_ms else None,
-            "k_filter_refine_frac_of_hbm_peak": ref_b / (ref_ms * 1e6) / HBM_PEAK_GBS if ref_ms else None,
+            "kernel": "k_isax_filter", "kernel_ms": ref_ms,
+            # 16 B per iSAX word streamed (once per leaf per 64 queries) + at
+            # most 4 L B per refined (query, series) pair (an upper bound: the
+            # refinement abandons early)
+            "kernel_algorithmic_bytes_upper_bound": ref_b,
+            "kernel_algorithmic_gbs_upper_bound": ref_b / (ref_ms * 1e6) if ref_ms else None,
+            # HBM bytes the counters measured per launch (profiles/traffic.txt)
+            "kernel_traffic_bytes": tb,
+            "kernel_traffic_gbs": tb / (ref_ms * 1e6) if tb and ref_ms else None,
+            "kernel_traffic_frac_of_hbm_peak": tb / (ref_ms * 1e6) / HBM_PEAK_GBS if tb and ref_ms else None,
             "series_lb_checked_per_query": sum(st["series_lb_checked"] for st in sts) / (len(sts) * B),
             "series_refined_per_query": sum(st["series_refined"] for st in sts) / (len(sts) * B),
             "leaves_after_eapca_per_query": sum(st["leaves_after_eapca"] for st in sts) / (len(sts) * B)}
@@ -660,11 +667,13 @@ def main() -> int:
     if args.serve_requests:
         loads = [args.serve_qps] if args.serve_qps > 0 else [f * qps for f in map(float, args.serve_loads.split(","))]
         serving = []
-        for sq in loads:
-            leg = serving_leg(hkg, ix, args, Qh, kq, sq)
-            leg["offered_fraction_of_throughput"] = round(sq / qps, 3)
-            serving.append(leg)
-            log(f"serving: {leg}")
+        for inflight in sorted({1, args.serve_inflight}):
+            args_i = argparse.Namespace(**{**vars(args), "serve_inflight": inflight})
+            for sq in loads:
+                leg = serving_leg(hkg, ix, args_i, Qh, kq, sq)
+                leg["offered_fraction_of_throughput"] = round(sq / qps, 3)
+                serving.append(leg)
+                log(f"serving: {leg}")
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(hkg, args)
     if cpu:

@@ -699,6 +699,7 @@ __global__ void k_bf16_tau(uint32_t nq, const uint32_t *__restrict__ kq, const u
 
 // queries per filter block (block group) for series of length L
 static uint32_t bf16_qpb(uint32_t L) { return (uint32_t)filter_waves((int)L / 16) * kQPerWave; }
+uint32_t bf16_group_queries(uint32_t L) { return bf16_qpb(L); }
 
 uint64_t bf16_scratch_per_query(uint32_t L, uint64_t nleaf, uint64_t ntiles)
 {

@@ -223,6 +223,8 @@ struct CallSettings {
 CallSettings &call_settings();
 // BF16 path scratch of one query beyond the FP32 buffer set
 uint64_t bf16_scratch_per_query(uint32_t L, uint64_t nleaf, uint64_t ntiles);
+// queries of one BF16 filter block group (a batch's unit of work)
+uint32_t bf16_group_queries(uint32_t L);
 // batched iSAX path (hkg_isax.hip): whether a batch of nq takes it, and its
 // breakpoint table (once per device)
 bool isax_batched(uint32_t nq);

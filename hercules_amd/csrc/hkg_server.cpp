@@ -137,8 +137,15 @@ void Server::worker(int wi)
       cv_work.wait(lock, [&] { return stopping || !pending.empty(); });
       if (pending.empty()) break;  // stopping and drained
       // admit what fits: max_batch queries, each charged the scratch of the
-      // batch's k class, within this batch's scratch share
-      while (!pending.empty() && batch.size() < B) {
+      // batch's k class, within this batch's scratch share; on the BF16
+      // path in whole query groups when more than one group waits (a
+      // partial group leaves filter blocks idle)
+      uint32_t bmax = B;
+      if (opt.path == HKG_PATH_BF16 && pending.size() < B) {
+        const uint32_t gq = bf16_group_queries(ix->dev.L);
+        if (pending.size() > gq) bmax = (uint32_t)(pending.size() / gq * gq);
+      }
+      while (!pending.empty() && batch.size() < bmax) {
         Request &r = reqs[pending.front()];
         const uint32_t kc2 = std::max(kc, k_class(r.k));
         if (!batch.empty() && (batch.size() + 1) * per_query(kc2) > batch_scratch) break;
