@@ -515,6 +515,17 @@ uint32_t Index::filter_grid() const
   return cu_count() * per;
 }
 
+bool Index::leaf_ordered(uint32_t nq) const
+{
+  if (!dev.has_tree) return false;
+  const uint32_t f = call_settings().batched_filter;
+  if (f == 1) return false;
+  if (f == 2) return true;
+  const uint32_t tf = tuned_filter.load(std::memory_order_relaxed);
+  if (tf && nq == tuned_batch[HKG_PATH_BATCHED].load(std::memory_order_relaxed)) return tf == 2;
+  return isax_batched(nq, dev.nleaf);
+}
+
 
 void Index::enqueue_index_batch(QueryBuffers &b, uint32_t nq, hipStream_t st, Prof *pf, bool seeded)
 {
@@ -661,7 +672,7 @@ uint32_t Index::cand_cap(hkg_path path, uint32_t nq, uint32_t kmax) const
   if (cap_override) return std::max<uint32_t>(cap_override, kmax);
   // the batched iSAX filter keeps every series within the bound of the
   // approximate stage, without a per-query top-k
-  if (path == HKG_PATH_BATCHED && dev.has_tree && isax_batched(nq)) return std::max<uint32_t>(32768, 16 * kmax);
+  if (path == HKG_PATH_BATCHED && leaf_ordered(nq)) return std::max<uint32_t>(32768, 16 * kmax);
   return std::max<uint32_t>(8192, 16 * kmax);
 }
 
@@ -915,6 +926,10 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     set_error("unknown query path " + std::to_string((uint32_t)o.path));
     return HKG_EINVAL;
   }
+  if (o.batched_filter > 2) {
+    set_error("batched_filter must be 0 (by size), 1 (per item) or 2 (leaf-ordered)");
+    return HKG_EINVAL;
+  }
   const bool mode = o.nprobes > 0 || o.epsilon != 0.0f;
   if (!(o.epsilon >= 0.0f) || std::isinf(o.epsilon) || std::isnan(o.eapca_threshold) ||
       std::isnan(o.sax_threshold)) {
@@ -946,6 +961,7 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
   } restore{call_settings()};
   call_settings().cap_override = o.cand_cap;
   call_settings().bound_pass = o.bf16_bound_pass;
+  call_settings().batched_filter = o.batched_filter;
   call_settings().mode = mode;
   call_settings().mp = ModeParams{o.epsilon, o.eapca_threshold, o.sax_threshold, o.approx_leaves, o.nprobes, 0, 0};
   HKG_HIP_TRY(hipSetDevice(dev.device));
@@ -975,7 +991,7 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
       pf->n = 0;
       pf->src = nullptr;
     }
-    const bool isax = path == HKG_PATH_BATCHED && !mode && dev.has_tree && isax_batched(n);
+    const bool isax = path == HKG_PATH_BATCHED && !mode && leaf_ordered(n);
     if (path == HKG_PATH_BF16)
       rc = run_bf16_batch(dQ + (uint64_t)q0 * dev.L, dk + q0, n, kmax, st, &b, &c, pf.get(), nullptr,
                           o.bf16_bound_pass == 1, n == batch);
@@ -1150,27 +1166,34 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
   }
   hkg_path path = path_in == HKG_PATH_AUTO ? HKG_PATH_BATCHED : (hkg_path)path_in;
   if (!dev.has_tree) path = HKG_PATH_SCAN;
-  std::vector<uint32_t> batches, grids;
+  // a shape: queries per batch, the main kernel's grid (blocks per CU) and,
+  // on the batched path, its filter (1 per item, 2 leaf-ordered; the grid
+  // only shapes the per-item filter)
+  struct Shape {
+    uint32_t g, f;
+  };
+  std::vector<uint32_t> batches;
+  std::vector<Shape> shapes;
   uint32_t def_batch = 1024, def_grid = 0;
   switch (path) {
     case HKG_PATH_INDEX:
       batches = {1};
-      grids = {2, 4, 8};
+      shapes = {{2, 0}, {4, 0}, {8, 0}};
       def_batch = 1;
       def_grid = kFilterBlocksPerCU;
       break;
     case HKG_PATH_BATCHED:
       batches = {256, 512, 1024, 2048};
-      grids = {2, 4, 8};
+      shapes = {{2, 1}, {4, 1}, {8, 1}, {0, 2}};
       def_grid = kFilterBlocksPerCU;
       break;
     case HKG_PATH_BF16:
       batches = {512, 1024, 2048, 4096};
-      grids = {0};
+      shapes = {{0, 0}};
       break;
     default:
       batches = {1, 4, 16, 64, 256};
-      grids = {3, 6, 8};  // full-scan blocks per CU
+      shapes = {{3, 0}, {6, 0}, {8, 0}};  // full-scan blocks per CU
       def_batch = 64;
       def_grid = kScanBlocksPerCU;
       break;
@@ -1183,15 +1206,20 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
     CallSettings saved;
     ~Restore() { call_settings() = saved; }
   } restore{call_settings()};
+  const bool scan_path = path == HKG_PATH_SCAN;
+  const bool batched = path == HKG_PATH_BATCHED;
+  // the default shape is the untuned one
+  tuned_batch[path].store(0, std::memory_order_relaxed);
+  if (batched) tuned_filter.store(0, std::memory_order_relaxed);
   hkg_query_opts qo;
   hkg_default_query_opts(&qo);
   qo.path = path;
   const size_t m = (size_t)nq * kmax;
   std::vector<uint64_t> ids0(m), ids(m);
   std::vector<float> d20(m), d2(m);
-  const bool scan_path = path == HKG_PATH_SCAN;
-  auto run = [&](uint32_t batch, uint32_t grid, uint64_t *I, float *D, double *ms) -> hkg_status {
+  auto run = [&](uint32_t batch, uint32_t grid, uint32_t filt, uint64_t *I, float *D, double *ms) -> hkg_status {
     qo.batch = batch;
+    qo.batched_filter = filt;
     if (scan_path)
       call_settings().sblocks = grid;
     else
@@ -1205,51 +1233,51 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
     }
     return HKG_OK;
   };
+  // the default shape: default batch and grid, the filter its size picks
+  const uint32_t def_filt = batched ? (isax_batched(def_batch, dev.nleaf) ? 2u : 1u) : 0u;
   double ms_def = 0.0, ms;
-  hkg_status rc = run(def_batch, def_grid, ids0.data(), d20.data(), &ms);  // warm-up
+  hkg_status rc = run(def_batch, def_grid, def_filt, ids0.data(), d20.data(), &ms);  // warm-up
   if (rc != HKG_OK) return rc;
-  if ((rc = run(def_batch, def_grid, ids0.data(), d20.data(), &ms_def)) != HKG_OK) return rc;
+  if ((rc = run(def_batch, def_grid, def_filt, ids0.data(), d20.data(), &ms_def)) != HKG_OK) return rc;
   struct Timed {
-    uint32_t b, g;
+    uint32_t b, g, f;
     double ms;
   };
-  std::vector<Timed> timed{{def_batch, def_grid, ms_def}};
+  std::vector<Timed> timed{{def_batch, def_filt == 2 ? 0u : def_grid, def_filt, ms_def}};
   uint32_t ncand = 1;
   for (uint32_t b : bs) {
-    for (uint32_t g : grids) {
-      if (b == def_batch && g == def_grid) continue;
-      if ((rc = run(b, g, ids.data(), d2.data(), &ms)) != HKG_OK) return rc;
+    for (const Shape &sh : shapes) {
+      if (b == def_batch && sh.f == def_filt && (sh.f == 2 || sh.g == def_grid)) continue;
+      if ((rc = run(b, sh.g, sh.f, ids.data(), d2.data(), &ms)) != HKG_OK) return rc;
       ++ncand;
       if (memcmp(ids.data(), ids0.data(), m * 8) != 0 || memcmp(d2.data(), d20.data(), m * 4) != 0) {
         set_error("autotune: a launch shape changed the answers (batch " + std::to_string(b) + ", grid " +
-                  std::to_string(g) + " blocks per CU)");
+                  std::to_string(sh.g) + " blocks per CU, filter " + std::to_string(sh.f) + ")");
         return HKG_EINTERNAL;
       }
-      timed.push_back({b, g, ms});
+      timed.push_back({b, sh.g, sh.f, ms});
     }
   }
   double fastest = ms_def;
   for (const Timed &t : timed) fastest = std::min(fastest, t.ms);
-  uint32_t best_b = 0, best_g = 0;
-  double best = 0.0;
+  // the largest batch within the margin of the fastest, then its fastest shape
+  const Timed *best = nullptr;
   for (const Timed &t : timed) {
     if (t.ms > fastest * (1.0 + kTuneMargin)) continue;
-    if (best_b == 0 || t.b > best_b || (t.b == best_b && t.ms < best)) {
-      best_b = t.b;
-      best_g = t.g;
-      best = t.ms;
-    }
+    if (!best || t.b > best->b || (t.b == best->b && t.ms < best->ms)) best = &t;
   }
-  tuned_batch[path].store(best_b, std::memory_order_relaxed);
-  if (best_g) (scan_path ? tuned_sblocks : tuned_fblocks).store(best_g, std::memory_order_relaxed);
+  tuned_batch[path].store(best->b, std::memory_order_relaxed);
+  if (best->g) (scan_path ? tuned_sblocks : tuned_fblocks).store(best->g, std::memory_order_relaxed);
+  if (batched) tuned_filter.store(best->f, std::memory_order_relaxed);
   memset(out, 0, sizeof(*out));
   out->path = path;
   out->length = dev.L;
-  out->batch = best_b;
-  out->filter_blocks_per_cu = scan_path ? 0 : best_g;
-  out->scan_blocks_per_cu = scan_path ? best_g : 0;
+  out->batch = best->b;
+  out->filter_blocks_per_cu = scan_path ? 0 : best->g;
+  out->scan_blocks_per_cu = scan_path ? best->g : 0;
+  out->batched_filter = best->f;
   out->candidates = ncand;
-  out->ms_per_query = best / nq;
+  out->ms_per_query = best->ms / nq;
   out->ms_per_query_default = ms_def / nq;
   return HKG_OK;
 }

@@ -163,6 +163,7 @@ struct Index {
   std::atomic<uint32_t> tuned_batch[HKG_PATH_BF16 + 1] = {};
   std::atomic<uint32_t> tuned_fblocks{0};
   std::atomic<uint32_t> tuned_sblocks{0};  // full-scan blocks per CU
+  std::atomic<uint32_t> tuned_filter{0};   // batched path's filter at its tuned batch (1 items, 2 leaf; 0 none)
 
   ~Index();
   hkg_status init_device(int device);
@@ -199,6 +200,9 @@ struct Index {
   uint32_t filter_grid() const;
   // grid.x of the full scan for the running call (per query group)
   uint32_t scan_grid(uint32_t nq) const;
+  // whether an exact batch of nq on the batched path takes the leaf-ordered
+  // iSAX filter (the call's batched_filter, else by batch and index size)
+  bool leaf_ordered(uint32_t nq) const;
   hkg_status autotune(const float *Q, uint32_t nq, const uint32_t *k, uint32_t kmax, uint32_t path,
                       hkg_tuning *out);
   hkg_status fix_overflow(QueryBuffers &b, uint32_t nq, uint32_t kmax, hipStream_t st, uint32_t *retries);
@@ -218,6 +222,7 @@ struct CallSettings {
   bool mode = false;          // an approximate mode (epsilon > 0 or nprobes > 0) is on
   uint32_t fblocks = 0;       // k_filter_refine blocks per CU (0: the index's tuned or default)
   uint32_t sblocks = 0;       // full-scan blocks per CU (0: the index's tuned or default)
+  uint32_t batched_filter = 0;  // hkg_query_opts.batched_filter
   ModeParams mp{};            // its parameters (nleaves / nseries filled per index)
 };
 CallSettings &call_settings();
@@ -225,9 +230,10 @@ CallSettings &call_settings();
 uint64_t bf16_scratch_per_query(uint32_t L, uint64_t nleaf, uint64_t ntiles);
 // queries of one BF16 filter block group (a batch's unit of work)
 uint32_t bf16_group_queries(uint32_t L);
-// batched iSAX path (hkg_isax.hip): whether a batch of nq takes it, and its
-// breakpoint table (once per device)
-bool isax_batched(uint32_t nq);
+// batched iSAX path (hkg_isax.hip): whether a batch of nq queries over nleaf
+// leaves takes the leaf-ordered filter by default, and its breakpoint table
+// (once per device)
+bool isax_batched(uint32_t nq, uint32_t nleaf);
 hkg_status upload_isax_tables(DevIndex &dv);
 
 // releases a held buffer set when it goes out of scope

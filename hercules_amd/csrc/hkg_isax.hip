@@ -28,7 +28,15 @@ namespace {
 
 constexpr int kStage = 16;               // survivors staged per lane before a flush
 constexpr int kIsaxWaves = 4;            // waves per filter block
-constexpr uint32_t kIsaxMinBatch = 16;   // smaller batches take the per-item filter
+// The leaf-ordered filter streams each leaf's iSAX words once for all the
+// queries that need the leaf, one per lane; with few queries per leaf its
+// waves run mostly empty and the per-item filter is faster.  Measured on
+// MI355X (series of 256, leaves of 10000, k ~ U{1..100}): it wins from 1024
+// queries at 1811 leaves and from 512 at 17861, and loses up to 2048 queries
+// at 188 leaves; hence a batch of at least kIsaxMinBatch queries and
+// kIsaxMinPairs (query, leaf) pairs takes it.
+constexpr uint32_t kIsaxMinBatch = 512;
+constexpr uint64_t kIsaxMinPairs = 1536ull * 1024;
 
 
 // dense list of the candidate pairs keyed by leaf: key = leaf when the pair
@@ -252,7 +260,10 @@ hkg_status upload_isax_tables(DevIndex &dv)
   return HKG_OK;
 }
 
-bool isax_batched(uint32_t nq) { return nq >= kIsaxMinBatch; }
+bool isax_batched(uint32_t nq, uint32_t nleaf)
+{
+  return nq >= kIsaxMinBatch && (uint64_t)nleaf * nq >= kIsaxMinPairs;
+}
 
 void QueryBuffers::free_isax()
 {

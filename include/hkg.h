@@ -107,7 +107,15 @@ typedef struct hkg_query_opts {
                               as 1, as in the reference)                     */
   float eapca_threshold;   /* epsilon search: candidate-branch thresholds    */
   float sax_threshold;     /* (--eapca-threshold 0.25, --sax-threshold 0.9)  */
-  uint32_t reserved[2];
+  uint32_t batched_filter; /* HKG_PATH_BATCHED, exact search: the filter of a
+                              batch.  0 = the autotuner's choice at its tuned
+                              batch, else by batch and index size (the
+                              leaf-ordered filter from 512 queries and
+                              1.5 M (query, leaf) pairs per batch),
+                              1 = per (query, leaf) work items,
+                              2 = leaf-ordered: the queries of each leaf
+                              together, one query per lane.  Same answers.  */
+  uint32_t reserved;
 } hkg_query_opts;
 
 /* stages reported in hkg_query_stats.stage_ms / stage_bytes */
@@ -229,8 +237,11 @@ hkg_status hkg_scan_knn_device_noalloc(hkg_index *ix, const float *dQ, uint32_t 
    shapes are timed on the index itself, for its series length: the queries
    per device batch of a path and the persistent grid of its main kernel
    (blocks per CU): the iSAX filter + refine kernel on the FP32 index paths,
-   the scan kernel on the full scan.  Every candidate
-   must return the same answers, bit for bit (HKG_EINTERNAL otherwise). */
+   the scan kernel on the full scan; on the batched path also its filter
+   (per item or leaf-ordered, hkg_query_opts.batched_filter), kept for calls
+   that leave batched_filter at 0 and whose batch is the tuned one.  Every
+   candidate must return the same answers, bit for bit (HKG_EINTERNAL
+   otherwise). */
 typedef struct hkg_tuning {
   uint32_t path;                 /* the path tuned (hkg_path; AUTO tunes BATCHED) */
   uint32_t length;               /* series length of the index                    */
@@ -240,6 +251,9 @@ typedef struct hkg_tuning {
   uint32_t candidates;           /* shapes timed                                  */
   uint32_t scan_blocks_per_cu;   /* chosen grid of the full scan (scan path; 0
                                     otherwise)                                    */
+  uint32_t batched_filter;       /* batched path: the filter chosen (1 per item,
+                                    2 leaf-ordered; 0 on the other paths)         */
+  uint32_t reserved;
   double ms_per_query;           /* device time per query, the chosen shape       */
   double ms_per_query_default;   /* device time per query, the default shape      */
 } hkg_tuning;

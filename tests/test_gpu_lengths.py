@@ -46,14 +46,16 @@ def test_long_series(hkg, oracle, tmp_path, L, n, leaf):
                                extra=["--epsilon", 1.0, "--approx-stop-condition", 80])
     with hkg.HerculesIndex.open(idx) as ix:
         ix.set_ids(row_map(idx, X))
-        for path, batch in (("index", 0), ("batched", 0), ("scan", 1), ("scan", 0)):
-            print(f"lengths: L={L} path={path} batch={batch}", flush=True)
-            r = ix.knn(Q, k, path=path, batch=batch)
-            assert_same(r.ids, r.d2, ids, d2, f"L={L} {path} batch {batch}")
+        for path, batch, filt in (("index", 0, "auto"), ("batched", 0, "items"), ("batched", 0, "leaf"),
+                                  ("scan", 1, "auto"), ("scan", 0, "auto")):
+            print(f"lengths: L={L} path={path} batch={batch} filter={filt}", flush=True)
+            r = ix.knn(Q, k, path=path, batch=batch, batched_filter=filt)
+            assert_same(r.ids, r.d2, ids, d2, f"L={L} {path} batch {batch} filter {filt}")
         kv = np.array([1, 3, 10, 7] * (Q.shape[0] // 4), np.uint32)
-        r = ix.knn(Q, kv, path="batched")
-        for i, kq in enumerate(kv):
-            assert np.array_equal(r.d2[i, :kq].view(np.uint32), d2[i, :kq].view(np.uint32)), (L, i)
+        for filt in ("items", "leaf"):
+            r = ix.knn(Q, kv, path="batched", batched_filter=filt)
+            for i, kq in enumerate(kv):
+                assert np.array_equal(r.d2[i, :kq].view(np.uint32), d2[i, :kq].view(np.uint32)), (L, filt, i)
         print(f"lengths: L={L} epsilon 1", flush=True)
         e = ix.knn(Q, k, epsilon=1.0, approx_leaves=80)
         assert np.array_equal(e.d2.view(np.uint32), eps_d2.view(np.uint32))

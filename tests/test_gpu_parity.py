@@ -114,7 +114,13 @@ def ref_index(tmp_path_factory, small_data):
     return str(d / "idx"), str(d / "q.bin")
 
 
-@pytest.mark.parametrize("path", ["index", "batched", "scan"])
+# the batched path's two filters: per (query, leaf) item and leaf-ordered
+PATHS = {"index": {"path": "index"}, "batched": {"path": "batched"},
+         "batched-items": {"path": "batched", "batched_filter": "items"},
+         "batched-leaf": {"path": "batched", "batched_filter": "leaf"}, "scan": {"path": "scan"}}
+
+
+@pytest.mark.parametrize("path", list(PATHS))
 @pytest.mark.parametrize("k", [1, 10])
 def test_index_paths_match_oracle(hkg, oracle, small_data, ref_index, path, k):
     X, Q = small_data
@@ -123,7 +129,7 @@ def test_index_paths_match_oracle(hkg, oracle, small_data, ref_index, path, k):
     ids, d2 = oracle.knn(X, Q, k)
     with hkg.HerculesIndex.open(idx) as ix:
         ix.set_ids(row_map(idx, X))
-        r = ix.knn(Q, k, path=path)
+        r = ix.knn(Q, k, **PATHS[path])
     assert_same(r.ids, r.d2, ids, d2, f"{path} k={k}")
 
 
@@ -147,13 +153,14 @@ def test_mixed_k_and_determinism(hkg, oracle, small_data, ref_index):
     ids, d2 = oracle.knn(X, Q, 100)
     with hkg.HerculesIndex.open(idx) as ix:
         ix.set_ids(row_map(idx, X))
-        r1 = ix.knn(Q, kv, path="batched")
-        r2 = ix.knn(Q, kv, path="batched")
-    assert np.array_equal(r1.ids, r2.ids) and np.array_equal(r1.d2, r2.d2)
-    for i, k in enumerate(kv):
-        assert np.array_equal(r1.d2[i, :k].view(np.uint32), d2[i, :k].view(np.uint32)), i
-        assert np.array_equal(r1.ids[i, :k], ids[i, :k]), i
-        assert (r1.d2[i, k:] == np.finfo(np.float32).max).all()
+        for filt in ("items", "leaf"):
+            r1 = ix.knn(Q, kv, path="batched", batched_filter=filt)
+            r2 = ix.knn(Q, kv, path="batched", batched_filter=filt)
+            assert np.array_equal(r1.ids, r2.ids) and np.array_equal(r1.d2, r2.d2)
+            for i, k in enumerate(kv):
+                assert np.array_equal(r1.d2[i, :k].view(np.uint32), d2[i, :k].view(np.uint32)), (filt, i)
+                assert np.array_equal(r1.ids[i, :k], ids[i, :k]), (filt, i)
+                assert (r1.d2[i, k:] == np.finfo(np.float32).max).all()
 
 
 def test_overflow_rerun_stays_exact(hkg, oracle, small_data, ref_index):
@@ -167,10 +174,11 @@ def test_overflow_rerun_stays_exact(hkg, oracle, small_data, ref_index):
     ids, d2 = oracle.knn(X, Q, 50)
     with hkg.HerculesIndex.open(idx) as ix:
         ix.set_ids(row_map(idx, X))
-        r = ix.knn(Q, 50, path="batched", cand_cap=64)
+        r = ix.knn(Q, 50, path="batched", cand_cap=64, batched_filter="leaf")
         assert r.stats["overflow_retries"] > 0
         assert_same(r.ids, r.d2, ids, d2, "overflow")
-        r = ix.knn(Q, 50, path="batched", batch=8, cand_cap=64)
+        r = ix.knn(Q, 50, path="batched", batch=8, cand_cap=64, batched_filter="items")
+        assert r.stats["overflow_retries"] > 0
         assert_same(r.ids, r.d2, ids, d2, "overflow, batches of 8")
 
 
@@ -252,6 +260,10 @@ def test_device_entry_rejects_bad_k_and_path(hkg, small_data, ref_index):
         qo.path = 9
         rc = hkg.lib().hkg_knn_device(ix._h, dQ.ptr, nq, dk.ptr, kmax, C.byref(qo), dids.ptr, dd2.ptr, None, None)
         assert rc == hkg.HKG_EINVAL
+        qo = hkg.query_opts("batched")
+        qo.batched_filter = 3
+        rc = hkg.lib().hkg_knn_device(ix._h, dQ.ptr, nq, dk.ptr, kmax, C.byref(qo), dids.ptr, dd2.ptr, None, None)
+        assert rc == hkg.HKG_EINVAL and b"batched_filter" in hkg.lib().hkg_last_error()
 
 
 def test_bf16_embeddings_l96(hkg, oracle, tmp_path):

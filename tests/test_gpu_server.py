@@ -105,21 +105,21 @@ def test_server_rejects_bad_requests(hkg, setup):
         srv.stop()
 
 
-@pytest.mark.parametrize("path", ["batched", "bf16"])
-def test_graph_replays_equal_eager_runs(hkg, setup, path):
-    """batch == number of queries: the batched paths (FP32 and BF16) capture
-    their stage sequence once and replay it, with the stage events recorded
-    by the graph when profiling; every replay, with eager work between,
-    equals the eager result (a batch larger than the queries runs eagerly)
-    and the oracle."""
+@pytest.mark.parametrize("path,filt", [("batched", "items"), ("batched", "leaf"), ("bf16", "auto")])
+def test_graph_replays_equal_eager_runs(hkg, setup, path, filt):
+    """batch == number of queries: the batched paths (FP32 with either
+    filter, and BF16) capture their stage sequence once and replay it, with
+    the stage stamps recorded by the graph when profiling; every replay, with
+    eager work between, equals the eager result (a batch larger than the
+    queries runs eagerly) and the oracle."""
     ix, Q, kv, ids, d2 = setup
     n = 64
     Qg, kg = np.ascontiguousarray(Q[:n]), np.ascontiguousarray(kv[:n])
-    eager = ix.knn(Qg, kg, path=path, batch=n + 1, profile=True)
+    eager = ix.knn(Qg, kg, path=path, batch=n + 1, profile=True, batched_filter=filt)
     for rep in range(4):
         prof = rep % 2 == 1
-        _say(f"{path}: graph replay {rep} (profile {prof})")
-        r = ix.knn(Qg, kg, path=path, batch=n, profile=prof)
+        _say(f"{path} ({filt}): graph replay {rep} (profile {prof})")
+        r = ix.knn(Qg, kg, path=path, batch=n, profile=prof, batched_filter=filt)
         assert np.array_equal(r.ids, eager.ids) and np.array_equal(r.d2.view(np.uint32), eager.d2.view(np.uint32))
         for i in range(n):
             _check(i, int(kg[i]), r.ids[i, :kg[i]], r.d2[i, :kg[i]], ids, d2, f"{path} replay {rep}")
@@ -129,4 +129,4 @@ def test_graph_replays_equal_eager_runs(hkg, setup, path):
             assert r.stats["stage_ms"]["topk"] > 0
         # eager work of other shapes between replays
         ix.knn(Q[n:n + 17], 5, path="scan")
-        ix.knn(Q[n:n + 9], 7, path=path, profile=True)
+        ix.knn(Q[n:n + 9], 7, path=path, profile=True, batched_filter=filt)

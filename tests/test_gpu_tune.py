@@ -40,7 +40,11 @@ def test_autotune_paths(hkg, oracle, kind, L):
             print(f"  -> {t}", flush=True)
             assert t["path"] == path and t["length"] == L
             assert t["batch"] in CANDIDATE_BATCHES[path]
-            if path in ("batched", "index"):
+            if path == "batched":
+                # the per-item filter with its grid, or the leaf-ordered one (no grid)
+                assert t["batched_filter"] in ("items", "leaf") and t["scan_blocks_per_cu"] == 0
+                assert t["filter_blocks_per_cu"] in ((2, 4, 8) if t["batched_filter"] == "items" else (0,))
+            elif path == "index":
                 assert t["filter_blocks_per_cu"] in (2, 4, 8) and t["scan_blocks_per_cu"] == 0
             elif path == "scan":
                 assert t["scan_blocks_per_cu"] in (3, 6, 8) and t["filter_blocks_per_cu"] == 0
