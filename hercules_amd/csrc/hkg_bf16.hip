@@ -377,7 +377,12 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   // a0/a1.  The series operands are read in batches of G k-steps, each
   // batch issued right after the first MFMA of the batch before: with the
   // tile copies in flight the compiler can only wait for all LDS reads at
-  // once, so every read gets most of a batch of MFMAs to land.  Between the
+  // once, so every read gets most of a batch of MFMAs to land.  The first
+  // batch of the next sub-tile of the same tile is read the same way during
+  // the last batch (NEXT), so that it is in opc when that sub-tile starts
+  // (PRE); only the first sub-tile of a tile waits for its first reads.
+  // The sub-tile's own |x|^2 (for its pre-test, which runs during the next
+  // sub-tile) is read in its last batch too, into x4o.  Between the
   // MFMAs this wave issues its copies of the tile at rows r0n.. into buffer
   // BNc, a third per sub-tile, at batch starts: all waves issuing a whole
   // tile at once would queue on the CU's copy path and stall, while spread
@@ -390,23 +395,28 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   constexpr int NB = KS / G;
   static_assert(KS % G == 0, "operand batches");
   auto mfma_sub = [&](auto Bc, auto SUBc, f32x16 &a0, f32x16 &a1, auto PTc, const f32x16 &p0, const f32x16 &p1,
-                      const float4 (&x4)[4], float &m0, float &m1, uint64_t r0n,
-                      auto BNc) __attribute__((always_inline)) {
+                      const float4 (&x4)[4], float &m0, float &m1, uint64_t r0n, auto BNc, bf16x8 (&opc)[G],
+                      auto PREc, auto NEXTc, float4 (&x4o)[4]) __attribute__((always_inline)) {
     constexpr bool PT = decltype(PTc)::value;
     constexpr int SUB = decltype(SUBc)::value;
+    constexpr bool PRE = decltype(PREc)::value, NEXT = decltype(NEXTc)::value;
     constexpr int c_lo = SUB * NCI / 3, c_n = (SUB + 1) * NCI / 3 - c_lo;  // this sub-tile's copy chunks
     const int crow = 32 * SUB;
     const bf16x8 *xv = reinterpret_cast<const bf16x8 *>(&buf(Bc).x[0]);
-    const int row = crow + l32;
+    const int row = crow + l32, nrow = row + 32;
     auto op = [&](int k) { return xv[row * VPR + ((2 * k + half) ^ (row & SW))]; };
+    auto op_next = [&](int k) { return xv[nrow * VPR + ((2 * k + half) ^ (nrow & SW))]; };
     // pre-test elements done after k-step k: 0 for k < 2, then spread evenly
     auto done = [](int k) { return k < 2 ? 0 : (k - 2) * 16 / (KS - 2); };
     __builtin_amdgcn_sched_barrier(0);
     a0 = f32x16{};
     a1 = f32x16{};
-    bf16x8 opc[G], opn[G];
+    bf16x8 opn[G];
+    if constexpr (!PRE) {
 #pragma unroll
-    for (int g = 0; g < G; ++g) opc[g] = op(g);
+      for (int g = 0; g < G; ++g) opc[g] = op(g);
+    }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
 #pragma unroll
@@ -424,6 +434,12 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
           if (b + 1 < NB) {
 #pragma unroll
             for (int h = 0; h < G; ++h) opn[h] = op((b + 1) * G + h);
+          } else {
+            if constexpr (NEXT) {
+#pragma unroll
+              for (int h = 0; h < G; ++h) opn[h] = op_next(h);
+            }
+            n2_of(Bc, crow, x4o);
           }
         }
         if constexpr (PT) {
@@ -432,7 +448,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
         }
         __builtin_amdgcn_sched_barrier(0);
       }
-      if (b + 1 < NB) {
+      if (b + 1 < NB || NEXT) {
 #pragma unroll
         for (int g = 0; g < G; ++g) opc[g] = opn[g];
       }
@@ -567,16 +583,19 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     const uint64_t r0n = (uint64_t)tile_at(j + 2, j) * kRowTile;
     const uint64_t r0 = (uint64_t)tile_at(j, j) * kRowTile;
     using PT = std::true_type;
+    using T_ = std::true_type;
+    using F_ = std::false_type;
     using S0 = std::integral_constant<int, 0>;
     using S1 = std::integral_constant<int, 1>;
     using S2 = std::integral_constant<int, 2>;
     f32x16 A0, A1, X0, X1;
-    float4 xa[4], xb[4];
+    float4 xa[4], xb[4];  // |x|^2 of sub-tiles 0 and 1 (read by mfma_sub)
     float m0, m1;
     bool h0, h1;
     // sub-tiles 0..2, the MFMAs of one beside the pre-test of the one
     // before (for sub-tile 0: the previous step's sub-tile 2)
-    mfma_sub(Bc, S0{}, A0, A1, PT{}, Y0, Y1, xc, m0, m1, r0n, BN{});
+    bf16x8 opc[G];  // first operand batch of the next sub-tile
+    mfma_sub(Bc, S0{}, A0, A1, PT{}, Y0, Y1, xc, m0, m1, r0n, BN{}, opc, F_{}, T_{}, xa);
     h0 = __ballot(m0 <= Cp0) != 0;
     h1 = __ballot(m1 <= Cp1) != 0;
     if (__builtin_expect(h0 || h1, 0)) slow(64, r0p, Y0, Y1, Cp0, Cp1, xc, h0, h1);
@@ -584,19 +603,16 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     const float2 tsl = slack_at(j);
     const float C0 = (thr[0] - qn2[0]) + (qe[0] * tsl.x + qbn[0] * tsl.y + kEvalSlack);
     const float C1 = (thr[1] - qn2[1]) + (qe[1] * tsl.x + qbn[1] * tsl.y + kEvalSlack);
-    n2_of(Bc, 0, xa);
-    mfma_sub(Bc, S1{}, X0, X1, PT{}, A0, A1, xa, m0, m1, r0n, BN{});
+    mfma_sub(Bc, S1{}, X0, X1, PT{}, A0, A1, xa, m0, m1, r0n, BN{}, opc, T_{}, T_{}, xb);
     h0 = __ballot(m0 <= C0) != 0;
     h1 = __ballot(m1 <= C1) != 0;
     if (__builtin_expect(h0 || h1, 0)) slow(0, r0, A0, A1, C0, C1, xa, h0, h1);
-    n2_of(Bc, 32, xb);
-    mfma_sub(Bc, S2{}, Y0, Y1, PT{}, X0, X1, xb, m0, m1, r0n, BN{});
+    mfma_sub(Bc, S2{}, Y0, Y1, PT{}, X0, X1, xb, m0, m1, r0n, BN{}, opc, T_{}, F_{}, xc);
     h0 = __ballot(m0 <= C0) != 0;
     h1 = __ballot(m1 <= C1) != 0;
     if (__builtin_expect(h0 || h1, 0)) slow(32, r0, X0, X1, C0, C1, xb, h0, h1);
-    // sub-tile 2 is tested during the next step; its |x|^2 is read now, as
-    // the next barrier frees this buffer for new copies
-    n2_of(Bc, 64, xc);
+    // sub-tile 2 is tested during the next step; its |x|^2 (xc) was read
+    // in its last batch, before the next barrier frees this buffer
     Cp0 = C0;
     Cp1 = C1;
     r0p = r0;
