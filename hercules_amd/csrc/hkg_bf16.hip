@@ -2,22 +2,24 @@
 // prunes (query, series) pairs; every survivor is refined exactly in FP32.
 //
 // For a query q and a series x (FP32 values), with q~, x~ their BF16
-// roundings and s the FP32-accumulated MFMA dot product of q~ and x~:
+// roundings, the MFMAs accumulate, in FP32 from the initial value
+// xn2 = |x|^2 (rounded down), the L products x~_i (-2 q~_i) (exact in FP32:
+// -2 q~ is a BF16 value and a product of two BF16 values fits FP32), to v:
 //   d2(q, x) = |q|^2 + |x|^2 - 2 q.x
 //   |q.x - q~.x~| <= |q - q~| |x| + |q~| |x - x~|          (Cauchy-Schwarz)
-//   |q~.x~ - s|   <= g_L |q~| |x~|,  g_L = 2 L 2^-24        (products of two
-//       BF16 values are exact in FP32; any summation order of L terms)
-// so  LB = |q|^2 + |x|^2 - 2 s - (|q - q~| A_x + |q~| B_x)
-// with A_x = 2|x| and B_x = 2 (|x - x~| + g_L |x~|) is a lower bound of d2
-// (and the same expression with + is an upper bound).  The norms are
-// computed in FP64 and rounded in the safe direction; the FP32 evaluation
-// is covered by a further absolute slack, and the comparison with the k-th
-// distance uses the same certified margin as the other bounds.  LB is used
-// only to prune.
+//   |v - (xn2 - 2 q~.x~)| <= g (xn2 + 2 |q~| |x~|),  g = 2 (L + 1) 2^-24
+//       (any summation order of the L + 1 terms; sum |x~_i q~_i| <= |q~||x~|)
+// so  LB = |q|^2 + v - (|q - q~| A_x + |q~| B_x + D_x)
+// with A_x = 2|x|, B_x = 2 (|x - x~| + g |x~|) and D_x = g xn2 is a lower
+// bound of d2 (and the same expression with + is an upper bound).  The norms
+// are computed in FP64 and rounded in the safe direction; the FP32
+// evaluation is covered by a further absolute slack, and the comparison
+// with the k-th distance uses the same certified margin as the other
+// bounds.  LB is used only to prune.
 //
 // The filter tests every pair first against the bound with the largest
-// A_x, B_x of its 128-row tile (two instructions per pair); only pairs that
-// pass are re-tested with their own A_x, B_x.
+// slack terms of its 96-row tile (a minimum per pair: the accumulators
+// already hold v); only pairs that pass are re-tested with their own.
 #include <float.h>
 #include <math.h>
 #include <string.h>
@@ -47,7 +49,8 @@ constexpr int kQPerWave = 64;              // two 32-query MFMA tiles
 __host__ __device__ constexpr int filter_waves(int) { return 4; }
 constexpr int kRowTile = 96;               // series per LDS tile
 constexpr int kBuf = 3;                    // LDS tile buffers (tiles loaded two ahead)
-constexpr uint32_t kRegionCap = 65536;     // survivors per filter block before k_bf16_scatter
+constexpr uint32_t kRegionCap = 262144;    // 16-byte units per filter block for survivor records (4 MB)
+constexpr uint32_t kRecU4 = 5;             // 16-byte units per survivor record
 constexpr float kEvalSlack = 1.0e-3f;      // FP32 evaluation of the bound
 constexpr uint32_t kBf16SurvivorCap = 524288;  // survivors kept per query (8 B each)
 constexpr float kHistLo = 0.6f;            // histogram range: [0.6 bound, threshold]
@@ -61,6 +64,10 @@ __device__ __forceinline__ uint16_t to_bf16_bits(float f)
   return (uint16_t)(r >> 16);
 }
 __device__ __forceinline__ float from_bf16_bits(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// rounding bound of the filter's accumulation (L products and |x|^2)
+__host__ __device__ constexpr double acc_gamma_d(uint32_t L) { return 2.0 * (double)(L + 1) * 5.9604644775390625e-08; }
+__host__ __device__ constexpr float acc_gamma(uint32_t L) { return (float)acc_gamma_d(L); }
 
 __device__ __forceinline__ float rd(double v)
 {
@@ -89,7 +96,7 @@ __device__ __forceinline__ HistRange hist_range(uint32_t bound_bits, float hi)
   return r;
 }
 
-// rows -> BF16 copy, {|x|^2 (down), A_x (up), B_x (up), 0} and |x|^2 alone
+// rows -> BF16 copy, {|x|^2 (down), A_x (up), B_x (up), D_x (up)} and |x|^2 alone
 __global__ void k_make_bf16(uint64_t n, uint32_t L, const float *__restrict__ X, uint16_t *__restrict__ Xb,
                             float4 *__restrict__ st, float *__restrict__ xn2)
 {
@@ -107,10 +114,11 @@ __global__ void k_make_bf16(uint64_t n, uint32_t L, const float *__restrict__ X,
     e2 += ((double)a - da) * ((double)a - da) + ((double)c - dc) * ((double)c - dc);
     b2 += da * da + dc * dc;
   }
-  const double gL = 2.0 * (double)L * 5.9604644775390625e-08;  // 2 L 2^-24
+  const double gL = acc_gamma_d(L);
   const double up = 1.0 + 1e-15;
   const float n2d = rd(n2 * (1.0 - 1e-15));
-  st[r] = make_float4(n2d, ru(2.0 * sqrt(n2) * up), ru(2.0 * (sqrt(e2) * up + gL * sqrt(b2) * up) * up), 0.0f);
+  st[r] = make_float4(n2d, ru(2.0 * sqrt(n2) * up), ru(2.0 * (sqrt(e2) * up + gL * sqrt(b2) * up) * up),
+                      ru(gL * (double)n2d * up));
   xn2[r] = n2d;
 }
 
@@ -129,7 +137,8 @@ __global__ void k_tile_slack(uint64_t n, const float4 *__restrict__ st, float2 *
   ts[t] = make_float2(a, b);
 }
 
-// queries -> BF16 + {|q|^2 (down), |q - q~| (up), |q~| (up), 0}
+// queries -> -2 q~ in BF16 (exact: q~ = bf16(q) scaled by a power of two)
+// + {|q|^2 (down), |q - q~| (up), |q~| (up), 0}
 __global__ void k_query_bf16(uint32_t nq, uint32_t L, const float *__restrict__ Q, uint16_t *__restrict__ Qb,
                              float4 *__restrict__ st)
 {
@@ -140,7 +149,7 @@ __global__ void k_query_bf16(uint32_t nq, uint32_t L, const float *__restrict__ 
   for (uint32_t t = 0; t < L; ++t) {
     float a = x[t];
     uint16_t b = to_bf16_bits(a);
-    Qb[(uint64_t)q * L + t] = b;
+    Qb[(uint64_t)q * L + t] = to_bf16_bits(-2.0f * from_bf16_bits(b));
     double d = (double)from_bf16_bits(b);
     n2 += (double)a * a;
     e2 += ((double)a - d) * ((double)a - d);
@@ -203,6 +212,9 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
 {
   constexpr int L = KS * 16;
   constexpr int VPR = L / 8;                       // 16-byte vectors per row
+  // rounding of an MFMA chain of L products and the initial |x|^2 (any
+  // order): at most kAccGamma (|x|^2 + 2 |q~| |x~|)
+  constexpr float kAccGamma = acc_gamma(L);
   // swizzle mask (VPR is a multiple of SW + 1): the 16 lanes that one LDS
   // cycle serves for a 16-byte read are 16 consecutive rows at the same
   // vector position, so 16 distinct positions keep them conflict-free
@@ -210,17 +222,16 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   constexpr int NCH = kRowTile * VPR / 64;         // 1 KB chunks per tile
   constexpr int kWaves = W;
   // vector memory ops every wave issues per tile, at least: its share of
-  // the series chunks, and with four waves one |x|^2 or slack-term load each
-  constexpr int kWaitOps = NCH / W + (W == 4 ? 1 : 0);
-  static_assert(VPR % (SW + 1) == 0 && (kRowTile * VPR) % 64 == 0 && kRowTile == 96 && W >= 4 && kWaitOps <= 15,
+  // the series chunks and one |x|^2 load (kRowTile / W rows)
+  constexpr int kWaitOps = NCH / W + 1;
+  static_assert(VPR % (SW + 1) == 0 && (kRowTile * VPR) % 64 == 0 && kRowTile == 96 && W == 4 && kWaitOps <= 15,
                 "tile shape");
   // three distinct buffers (not one array): the compiler then sees that the
   // copies in flight into one buffer do not alias reads of another and
   // does not wait for them
   struct TileBuf {
     uint16_t x[kRowTile * L];  // series, BF16, swizzled rows
-    float n2[kRowTile];        // |x|^2
-    float4 st[kRowTile];       // {|x|^2, A_x, B_x, 0}
+    float n2[kRowTile];        // |x|^2 (rounded down), the accumulators' initial values
   };
   __shared__ __attribute__((aligned(16))) TileBuf tb0, tb1, tb2;
   auto buf = [&](auto Bc) -> TileBuf & {
@@ -279,9 +290,11 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
       asm volatile("" : "+a"(qop[t][k]));
     }
   }
-  // survivors of this wave: its slice of the block's region, counted in a
+  // survivor records of this wave: its slice of the block's region
+  // (region_cap / W 16-byte units, kRecU4 per record), counted in a
   // wave-uniform register (positions from ballots, no atomics)
   const uint32_t wcap = region_cap / kWaves;
+  const uint32_t wrec = wcap / kRecU4;
   const uint64_t wbase = ((uint64_t)blk * kWaves + wave) * wcap;
   uint32_t wcnt = 0;
 
@@ -324,9 +337,10 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cts.y), j & 63)));
   };
   // Copies of the tile at rows r0.. into buffer Bc: 1 KB series chunks c =
-  // wave + i W (this wave's i-th), and one load of |x|^2 (rows 0..63: wave
-  // 0, 64..95: wave 1) or of the slack terms (waves 2, 3)
+  // wave + i W (this wave's i-th), and one load of the |x|^2 of kRowTile / W
+  // rows per wave
   constexpr int NCI = (NCH + kWaves - 1) / kWaves;  // series chunks per wave, at most
+  constexpr int kN2Rows = kRowTile / kWaves;       // |x|^2 rows per wave
   auto issue_chunk = [&](uint64_t r0, auto Bc, int i) __attribute__((always_inline)) {
     TileBuf &tbuf = buf(Bc);
     const int c = wave + i * kWaves;
@@ -341,13 +355,9 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   };
   auto issue_stats = [&](uint64_t r0, auto Bc) __attribute__((always_inline)) {
     TileBuf &tbuf = buf(Bc);
-    const int part = wave & 1, rbase = part * 64;
-    if (wave < 4 && (part == 0 || lane < kRowTile - 64)) {
-      if (wave < 2)
-        load_lds4(xn2g + r0 + rbase + lane, (lds_void *)(reinterpret_cast<char *>(&tbuf.n2[0]) + rbase * 4));
-      else
-        load_lds16(xst + r0 + rbase + lane, (lds_void *)(reinterpret_cast<char *>(&tbuf.st[0]) + rbase * 16));
-    }
+    const int rbase = wave * kN2Rows;
+    if (lane < kN2Rows)
+      load_lds4(xn2g + r0 + rbase + lane, (lds_void *)(reinterpret_cast<char *>(&tbuf.n2[0]) + rbase * 4));
   };
   auto issue = [&](uint32_t tile, auto Bc) __attribute__((always_inline)) {
     const uint64_t r0 = (uint64_t)tile * kRowTile;
@@ -356,47 +366,50 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     issue_stats(r0, Bc);
   };
 
-  // Pre-test of one sub-tile's pairs (series of register r is crow +
-  // 8(r/4) + 4 half + r%4 of the tile): m_t = min over the lane's 16 series
-  // of xn2 - 2 s for query tile t; some pair passes (xn2 - 2 s <= C_t) iff
-  // m_t <= C_t, as fminf skips NaN the way <= rejects it.
-  auto n2_of = [&](auto Bc, int crow, float4 (&x4)[4]) __attribute__((always_inline)) {
+  // The accumulators of a sub-tile start at the |x|^2 of their series
+  // (register r: series crow + 8(r/4) + 4 half + r%4 of the tile) and the
+  // query operands hold -2 q~, so an MFMA chain ends at v = xn2 - 2 s: the
+  // pre-test needs no arithmetic but a minimum.  m_t = min over the lane's 16
+  // series of v for query tile t; some pair passes (v <= C_t) iff m_t <= C_t,
+  // as fminf skips NaN the way <= rejects it.
+  auto n2_of = [&](auto Bc, int crow, f32x16 &c) __attribute__((always_inline)) {
     const TileBuf &tbuf = buf(Bc);
 #pragma unroll
-    for (int gi = 0; gi < 4; ++gi)
-      x4[gi] = *reinterpret_cast<const float4 *>(&tbuf.n2[crow + 8 * gi + 4 * half]);
+    for (int gi = 0; gi < 4; ++gi) {
+      const float4 x4 = *reinterpret_cast<const float4 *>(&tbuf.n2[crow + 8 * gi + 4 * half]);
+      c[4 * gi] = x4.x;
+      c[4 * gi + 1] = x4.y;
+      c[4 * gi + 2] = x4.z;
+      c[4 * gi + 3] = x4.w;
+    }
   };
-  auto pt_elem = [&](int r, const f32x16 &p0, const f32x16 &p1, const float4 (&x4)[4], float &m0,
-                     float &m1) __attribute__((always_inline)) {
-    const float xr = x4[r >> 2][r & 3];
-    const float v0 = fmaf(p0[r], -2.0f, xr), v1 = fmaf(p1[r], -2.0f, xr);
-    m0 = r == 0 ? v0 : fminf(m0, v0);
-    m1 = r == 0 ? v1 : fminf(m1, v1);
+  auto pt_elem = [&](int r, const f32x16 &p0, const f32x16 &p1, float &m0, float &m1) __attribute__((always_inline)) {
+    m0 = r == 0 ? p0[r] : fminf(m0, p0[r]);
+    m1 = r == 0 ? p1[r] : fminf(m1, p1[r]);
   };
   // MFMAs of 32-row sub-tile SUB (rows crow = 32 SUB.. of buffer Bc) into
-  // a0/a1.  The series operands are read in batches of G k-steps, each
-  // batch issued right after the first MFMA of the batch before: with the
-  // tile copies in flight the compiler can only wait for all LDS reads at
-  // once, so every read gets most of a batch of MFMAs to land.  The first
-  // batch of the next sub-tile of the same tile is read the same way during
-  // the last batch (NEXT), so that it is in opc when that sub-tile starts
+  // a0/a1, both starting from cin (the sub-tile's |x|^2).  The series
+  // operands are read in batches of G k-steps, each batch issued right after
+  // the first MFMA of the batch before: with the tile copies in flight the
+  // compiler can only wait for all LDS reads at once, so every read gets most
+  // of a batch of MFMAs to land.  The first batch of the next sub-tile of the
+  // same tile and its |x|^2 are read the same way during the last batch
+  // (NEXT), into opc and cin, so that they are there when that sub-tile starts
   // (PRE); only the first sub-tile of a tile waits for its first reads.
-  // The sub-tile's own |x|^2 (for its pre-test, which runs during the next
-  // sub-tile) is read in its last batch too, into x4o.  Between the
-  // MFMAs this wave issues its copies of the tile at rows r0n.. into buffer
-  // BNc, a third per sub-tile, at batch starts: all waves issuing a whole
-  // tile at once would queue on the CU's copy path and stall, while spread
-  // over the MFMAs the copies proceed in the background.  With PT,
-  // the pre-test of the previous sub-tile (accumulators p0/p1, its |x|^2 in
-  // x4, read by the caller) runs between the MFMAs, from the third k-step
-  // on (its accumulators are complete by then): the matrix cores stay busy
-  // while the wave, alone on its SIMD, does the vector work.
+  // Between the MFMAs this wave issues its copies of the tile at rows r0n..
+  // into buffer BNc, a third per sub-tile, at batch starts: all waves issuing
+  // a whole tile at once would queue on the CU's copy path and stall, while
+  // spread over the MFMAs the copies proceed in the background.  With PT,
+  // the pre-test of the previous sub-tile (accumulators p0/p1) runs between
+  // the MFMAs, from the third k-step on (its accumulators are complete by
+  // then): the matrix cores stay busy while the wave, alone on its SIMD, does
+  // the vector work.
   constexpr int G = KS % 4 == 0 ? 4 : (KS % 3 == 0 ? 3 : 2);  // k-steps per operand batch
   constexpr int NB = KS / G;
   static_assert(KS % G == 0, "operand batches");
   auto mfma_sub = [&](auto Bc, auto SUBc, f32x16 &a0, f32x16 &a1, auto PTc, const f32x16 &p0, const f32x16 &p1,
-                      const float4 (&x4)[4], float &m0, float &m1, uint64_t r0n, auto BNc, bf16x8 (&opc)[G],
-                      auto PREc, auto NEXTc, float4 (&x4o)[4]) __attribute__((always_inline)) {
+                      float &m0, float &m1, uint64_t r0n, auto BNc, bf16x8 (&opc)[G], f32x16 &cin, auto PREc,
+                      auto NEXTc) __attribute__((always_inline)) {
     constexpr bool PT = decltype(PTc)::value;
     constexpr int SUB = decltype(SUBc)::value;
     constexpr bool PRE = decltype(PREc)::value, NEXT = decltype(NEXTc)::value;
@@ -409,12 +422,11 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     // pre-test elements done after k-step k: 0 for k < 2, then spread evenly
     auto done = [](int k) { return k < 2 ? 0 : (k - 2) * 16 / (KS - 2); };
     __builtin_amdgcn_sched_barrier(0);
-    a0 = f32x16{};
-    a1 = f32x16{};
     bf16x8 opn[G];
     if constexpr (!PRE) {
 #pragma unroll
       for (int g = 0; g < G; ++g) opc[g] = op(g);
+      n2_of(Bc, crow, cin);
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -422,11 +434,11 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         const int k = b * G + g;
-        a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(opc[g], qop[0][k], a0, 0, 0, 0);
-        a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(opc[g], qop[1][k], a1, 0, 0, 0);
+        a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(opc[g], qop[0][k], k == 0 ? cin : a0, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(opc[g], qop[1][k], k == 0 ? cin : a1, 0, 0, 0);
         if (g == 0) {
-          // copies: chunk c_lo + q at batch q NB / c_n; |x|^2 or slack
-          // terms at batch 0 of sub-tile 0
+          // copies: chunk c_lo + q at batch q NB / c_n; |x|^2 at batch 0 of
+          // sub-tile 0
 #pragma unroll
           for (int q = 0; q < c_n; ++q)
             if (b == q * NB / c_n) issue_chunk(r0n, BNc, c_lo + q);
@@ -434,17 +446,15 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
           if (b + 1 < NB) {
 #pragma unroll
             for (int h = 0; h < G; ++h) opn[h] = op((b + 1) * G + h);
-          } else {
-            if constexpr (NEXT) {
+          } else if constexpr (NEXT) {
 #pragma unroll
-              for (int h = 0; h < G; ++h) opn[h] = op_next(h);
-            }
-            n2_of(Bc, crow, x4o);
+            for (int h = 0; h < G; ++h) opn[h] = op_next(h);
+            n2_of(Bc, crow + 32, cin);
           }
         }
         if constexpr (PT) {
 #pragma unroll
-          for (int r = done(k); r < done(k + 1); ++r) pt_elem(r, p0, p1, x4, m0, m1);
+          for (int r = done(k); r < done(k + 1); ++r) pt_elem(r, p0, p1, m0, m1);
         }
         __builtin_amdgcn_sched_barrier(0);
       }
@@ -454,59 +464,78 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
       }
     }
   };
-  auto pretest = [&](const f32x16 &a0, const f32x16 &a1, const float4 (&x4)[4], float &m0,
-                     float &m1) __attribute__((always_inline)) {
+  auto pretest = [&](const f32x16 &a0, const f32x16 &a1, float &m0, float &m1) __attribute__((always_inline)) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) pt_elem(r, a0, a1, x4, m0, m1);
+    for (int r = 0; r < 16; ++r) pt_elem(r, a0, a1, m0, m1);
   };
   // Pairs of a sub-tile whose pre-test passed (rare; reads no LDS, so it
-  // may run after the tile's buffer is reused).  HIST: the pairs
-  // whose upper bound with their own slack is within the threshold go to
-  // the query's histogram.  Otherwise every pair that passes the pre-test
-  // goes, as {row, xn2 - 2 s, query, query slot}, to the wave's slice of the region:
-  // k_bf16_scatter tests it with its own slack.  Per pair: one FMA and a
-  // ballot, and a uniform skip when no lane passes (the passing case is laid
-  // out of line, so the checks stay contiguous in the instruction cache);
-  // only the query tiles t with a pass (h0, h1) are looked at.
-  auto slow = [&](int crow, uint64_t r0, const f32x16 &a0, const f32x16 &a1, float C0, float C1,
-                  const float4 (&x4)[4], bool h0, bool h1) __attribute__((always_inline)) {
+  // may run after the tile's buffer is reused); only the query tiles t with
+  // a pass (h0, h1) are looked at.  HIST: the pairs whose upper bound with
+  // their own slack is within the threshold go to the query's histogram,
+  // one ballot per pair.  Otherwise each lane with a passing pair writes one
+  // record to the wave's slice of the region: {series of its register 0,
+  // query, slot | pass bits << 16, 0} and its 16 values xn2 - 2 s, five
+  // 16-byte stores at a position from one ballot; k_bf16_scatter tests each
+  // passing pair with its own slack.  The pass bits cost three VALU
+  // instructions per pair and no scalar round trip, so a wave that finds a
+  // survivor holds up its block's barrier for a few hundred cycles, not for
+  // a ballot and branch per pair.
+  auto slow = [&](int crow, uint64_t r0, const f32x16 &a0, const f32x16 &a1, float C0, float C1, bool h0,
+                  bool h1) __attribute__((always_inline)) {
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       if (!(t == 0 ? h0 : h1)) continue;
+      const float Ct = t == 0 ? C0 : C1;
+      if constexpr (HIST) {
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const float sv = t == 0 ? a0[r] : a1[r];
-        const float v = fmaf(sv, -2.0f, x4[r >> 2][r & 3]);  // xn2 - 2 s
-        const bool pass = v <= (t == 0 ? C0 : C1);
-        const uint64_t m = __ballot(pass);
-        if (__builtin_expect(m == 0, 1)) continue;
-        const int tr = crow + 8 * (r >> 2) + 4 * half + (r & 3);
-        const uint64_t grow = r0 + tr;
-        if constexpr (HIST) {
+        for (int r = 0; r < 16; ++r) {
+          const float v = t == 0 ? a0[r] : a1[r];  // xn2 - 2 s
+          const bool pass = v <= Ct;
+          if (__builtin_expect(__ballot(pass) == 0, 1)) continue;
+          const uint64_t grow = r0 + crow + 8 * (r >> 2) + 4 * half + (r & 3);
           if (pass && grow < n) {
             const float4 xsv = xst[grow];
-            const float ub = (v + qn2[t]) + (qe[t] * xsv.y + qbn[t] * xsv.z + kEvalSlack);
+            const float ub = (v + qn2[t]) + (qe[t] * xsv.y + qbn[t] * xsv.z + xsv.w + kEvalSlack);
             if (ub <= thr[t]) {
               const float lo = hlo[t];
               const int bin = ub < lo ? 0 : min(kBf16Bins - 1, 1 + (int)fminf((ub - lo) * hinv[t], (float)kBf16Bins));
               atomicAdd(&hist[(uint64_t)qid[t] * kBf16Bins + bin], 1u);
             }
           }
-        } else {
-          const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-          const uint32_t pos = wcnt + below;
-          wcnt += (uint32_t)__popcll(m);
-          if (pass) {
-            if (pos < wcap) {
-              region[wbase + pos] = make_uint4((uint32_t)grow, __float_as_uint(v), qid[t], (uint32_t)(t * 32 + l32));
-            } else if (grow < n) {
-              // the wave's slice is full: tested here, straight to the query's list
-              const float4 xsv = xst[grow];
-              const float lb = (v + qn2[t]) - (qe[t] * xsv.y + qbn[t] * xsv.z + kEvalSlack);
-              if (lb <= thr[t]) {
-                const uint32_t qp = atomicAdd(&scnt[qid[t]], 1u);
-                if (qp < scap)
-                  surv[(uint64_t)qid[t] * scap + qp] = ((uint64_t)__float_as_uint(fmaxf(lb, 0.0f)) << 32) | (uint32_t)grow;
+        }
+      } else {
+        float v[16];
+        uint32_t bits = 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          v[r] = t == 0 ? a0[r] : a1[r];  // xn2 - 2 s
+          bits |= v[r] <= Ct ? 1u << r : 0u;
+        }
+        const uint64_t m = __ballot(bits != 0);
+        const uint32_t pos = wcnt + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        wcnt += (uint32_t)__popcll(m);
+        const uint64_t rowb = r0 + crow + 4 * half;  // series of register 0; register r: + 8 (r / 4) + r % 4
+        if (bits != 0) {
+          if (pos < wrec) {
+            uint4 *rec = region + wbase + (uint64_t)pos * kRecU4;
+            rec[0] = make_uint4((uint32_t)rowb, qid[t], (uint32_t)(t * 32 + l32) | (bits << 16), 0u);
+#pragma unroll
+            for (int gi = 0; gi < 4; ++gi)
+              rec[1 + gi] = make_uint4(__float_as_uint(v[4 * gi]), __float_as_uint(v[4 * gi + 1]),
+                                       __float_as_uint(v[4 * gi + 2]), __float_as_uint(v[4 * gi + 3]));
+          } else {
+            // the wave's slice is full: tested here, straight to the query's list
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+              const uint64_t grow = rowb + 8 * (r >> 2) + (r & 3);
+              if (((bits >> r) & 1u) && grow < n) {
+                const float4 xsv = xst[grow];
+                const float lb = (v[r] + qn2[t]) - (qe[t] * xsv.y + qbn[t] * xsv.z + xsv.w + kEvalSlack);
+                if (lb <= thr[t]) {
+                  const uint32_t qp = atomicAdd(&scnt[qid[t]], 1u);
+                  if (qp < scap)
+                    surv[(uint64_t)qid[t] * scap + qp] = ((uint64_t)__float_as_uint(fmaxf(lb, 0.0f)) << 32) | (uint32_t)grow;
+                }
               }
             }
           }
@@ -548,11 +577,10 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     }
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
   };
-  // sub-tile 2 of the previous step (accumulators, |x|^2, pre-test
-  // constants, first row), tested during the next step's first MFMAs;
-  // nothing passes before the first tile
+  // sub-tile 2 of the previous step (accumulators, pre-test constants,
+  // first row), tested during the next step's first MFMAs; nothing passes
+  // before the first tile
   f32x16 Y0 = {}, Y1 = {};
-  float4 xc[4] = {};
   float Cp0 = -FLT_MAX, Cp1 = -FLT_MAX;
   uint64_t r0p = 0;
   auto step = [&](auto Bc, uint32_t j) __attribute__((always_inline)) {
@@ -589,30 +617,33 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     using S1 = std::integral_constant<int, 1>;
     using S2 = std::integral_constant<int, 2>;
     f32x16 A0, A1, X0, X1;
-    float4 xa[4], xb[4];  // |x|^2 of sub-tiles 0 and 1 (read by mfma_sub)
     float m0, m1;
     bool h0, h1;
     // sub-tiles 0..2, the MFMAs of one beside the pre-test of the one
     // before (for sub-tile 0: the previous step's sub-tile 2)
     bf16x8 opc[G];  // first operand batch of the next sub-tile
-    mfma_sub(Bc, S0{}, A0, A1, PT{}, Y0, Y1, xc, m0, m1, r0n, BN{}, opc, F_{}, T_{}, xa);
+    f32x16 cin;     // |x|^2 of the next sub-tile
+    mfma_sub(Bc, S0{}, A0, A1, PT{}, Y0, Y1, m0, m1, r0n, BN{}, opc, cin, F_{}, T_{});
     h0 = __ballot(m0 <= Cp0) != 0;
     h1 = __ballot(m1 <= Cp1) != 0;
-    if (__builtin_expect(h0 || h1, 0)) slow(64, r0p, Y0, Y1, Cp0, Cp1, xc, h0, h1);
-    // pre-test constant: xn2 - 2 s <= C  <=>  bound with the tile's largest slack <= threshold
+    if (__builtin_expect(h0 || h1, 0)) slow(64, r0p, Y0, Y1, Cp0, Cp1, h0, h1);
+    // pre-test constant: xn2 - 2 s <= C  <=>  bound with the tile's largest
+    // slack terms <= threshold (the accumulation term g |x|^2 at most
+    // g (A_x / 2)^2)
     const float2 tsl = slack_at(j);
-    const float C0 = (thr[0] - qn2[0]) + (qe[0] * tsl.x + qbn[0] * tsl.y + kEvalSlack);
-    const float C1 = (thr[1] - qn2[1]) + (qe[1] * tsl.x + qbn[1] * tsl.y + kEvalSlack);
-    mfma_sub(Bc, S1{}, X0, X1, PT{}, A0, A1, xa, m0, m1, r0n, BN{}, opc, T_{}, T_{}, xb);
+    const float dmax = kAccGamma * (0.25f * tsl.x * tsl.x);
+    const float C0 = (thr[0] - qn2[0]) + (qe[0] * tsl.x + qbn[0] * tsl.y + dmax + kEvalSlack);
+    const float C1 = (thr[1] - qn2[1]) + (qe[1] * tsl.x + qbn[1] * tsl.y + dmax + kEvalSlack);
+    mfma_sub(Bc, S1{}, X0, X1, PT{}, A0, A1, m0, m1, r0n, BN{}, opc, cin, T_{}, T_{});
     h0 = __ballot(m0 <= C0) != 0;
     h1 = __ballot(m1 <= C1) != 0;
-    if (__builtin_expect(h0 || h1, 0)) slow(0, r0, A0, A1, C0, C1, xa, h0, h1);
-    mfma_sub(Bc, S2{}, Y0, Y1, PT{}, X0, X1, xb, m0, m1, r0n, BN{}, opc, T_{}, F_{}, xc);
+    if (__builtin_expect(h0 || h1, 0)) slow(0, r0, A0, A1, C0, C1, h0, h1);
+    mfma_sub(Bc, S2{}, Y0, Y1, PT{}, X0, X1, m0, m1, r0n, BN{}, opc, cin, T_{}, F_{});
     h0 = __ballot(m0 <= C0) != 0;
     h1 = __ballot(m1 <= C1) != 0;
-    if (__builtin_expect(h0 || h1, 0)) slow(32, r0, X0, X1, C0, C1, xb, h0, h1);
-    // sub-tile 2 is tested during the next step; its |x|^2 (xc) was read
-    // in its last batch, before the next barrier frees this buffer
+    if (__builtin_expect(h0 || h1, 0)) slow(32, r0, X0, X1, C0, C1, h0, h1);
+    // sub-tile 2 is tested during the next step (from its accumulators, so
+    // after the next barrier frees this buffer)
     Cp0 = C0;
     Cp1 = C1;
     r0p = r0;
@@ -626,62 +657,51 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   // the last tile's sub-tile 2
   {
     float m0, m1;
-    pretest(Y0, Y1, xc, m0, m1);
+    pretest(Y0, Y1, m0, m1);
     const bool h0 = __ballot(m0 <= Cp0) != 0, h1 = __ballot(m1 <= Cp1) != 0;
-    if (h0 || h1) slow(64, r0p, Y0, Y1, Cp0, Cp1, xc, h0, h1);
+    if (h0 || h1) slow(64, r0p, Y0, Y1, Cp0, Cp1, h0, h1);
   }
   // the last steps' copies (re-reads into free buffers) land before the
   // block's LDS is released; siblings no longer wait for this block
   __builtin_amdgcn_s_waitcnt(0);
   if (pace_grid && wave == 0 && lane == 0)
     __hip_atomic_store(&prog[blk], 0xFFFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (!HIST && lane == 0) region_cnt[(uint64_t)blk * kWaves + wave] = wcnt < wcap ? wcnt : wcap;
+  if (!HIST && lane == 0) region_cnt[(uint64_t)blk * kWaves + wave] = wcnt < wrec ? wcnt : wrec;
 }
 
-// The pairs every filter wave found (one region slice each, {row, xn2 -
-// 2 s, query, slot of the query among the wave's 64}): the lower bound with
-// the row's own slack, and the pairs within the query's threshold into its
-// list, keyed by that bound.  One block per slice: positions are counted
-// per query slot in LDS, then one global atomic per query reserves them.
-constexpr uint32_t kScatterMax = 16384;  // entries per slice (kRegionCap / filter waves)
-static_assert(kRegionCap / filter_waves(16) <= kScatterMax, "region slice larger than the scatter's LDS");
+// The survivor records every filter wave wrote (one region slice each, one
+// record per lane and query tile with a pair that passed the pre-test): each
+// passing pair's lower bound with the row's own slack, and the pairs within
+// the query's threshold into its list, keyed by that bound.  One thread per
+// record (usually one or two passing pairs); a global atomic per kept pair.
 __global__ void __launch_bounds__(256) k_bf16_scatter(const uint4 *__restrict__ region, uint32_t slice_cap,
                                                       const uint32_t *__restrict__ region_cnt, uint64_t n,
                                                       const float4 *__restrict__ xst, const float4 *__restrict__ qst,
                                                       const float *__restrict__ thr0, uint32_t scap,
                                                       uint64_t *__restrict__ surv, uint32_t *__restrict__ scnt)
 {
-  __shared__ uint32_t s_cnt[kQPerWave], s_q[kQPerWave], s_base[kQPerWave];
-  __shared__ uint16_t s_pos[kScatterMax];
-  __shared__ float s_lb[kScatterMax];
   const uint32_t sl = blockIdx.x;
-  const uint32_t cnt = min(region_cnt[sl], min(slice_cap, kScatterMax));
+  const uint32_t cnt = min(region_cnt[sl], slice_cap / kRecU4);
   const uint4 *reg = region + (uint64_t)sl * slice_cap;
-  if (threadIdx.x < kQPerWave) s_cnt[threadIdx.x] = 0;
-  __syncthreads();
   for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
-    const uint4 e = reg[i];
-    uint16_t pos = 0xFFFFu;
-    if (e.x < n) {  // not a padding row of the last tile
-      const float4 xsv = xst[e.x], qs = qst[e.z];
-      const float lb = (__uint_as_float(e.y) + qs.x) - (qs.y * xsv.y + qs.z * xsv.z + kEvalSlack);
-      if (lb <= thr0[e.z]) {
-        pos = (uint16_t)atomicAdd(&s_cnt[e.w], 1u);
-        s_q[e.w] = e.z;
-        s_lb[i] = fmaxf(lb, 0.0f);
+    const uint4 *rec = reg + (uint64_t)i * kRecU4;
+    const uint4 h = rec[0];
+    const uint32_t q = h.y;
+    const float4 qs = qst[q];
+    const float th = thr0[q];
+    for (uint32_t bits = h.z >> 16; bits != 0; bits &= bits - 1) {
+      const int r = __ffs(bits) - 1;
+      const uint64_t row = (uint64_t)h.x + 8 * (r >> 2) + (r & 3);
+      if (row >= n) continue;  // a padding row of the last tile
+      const uint4 vv = rec[1 + (r >> 2)];
+      const uint32_t vb = (r & 3) == 0 ? vv.x : (r & 3) == 1 ? vv.y : (r & 3) == 2 ? vv.z : vv.w;
+      const float4 xsv = xst[row];
+      const float lb = (__uint_as_float(vb) + qs.x) - (qs.y * xsv.y + qs.z * xsv.z + xsv.w + kEvalSlack);
+      if (lb <= th) {
+        const uint32_t p = atomicAdd(&scnt[q], 1u);
+        if (p < scap) surv[(uint64_t)q * scap + p] = ((uint64_t)__float_as_uint(fmaxf(lb, 0.0f)) << 32) | (uint32_t)row;
       }
     }
-    s_pos[i] = pos;
-  }
-  __syncthreads();
-  if (threadIdx.x < kQPerWave && s_cnt[threadIdx.x] > 0)
-    s_base[threadIdx.x] = atomicAdd(&scnt[s_q[threadIdx.x]], s_cnt[threadIdx.x]);
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
-    if (s_pos[i] == 0xFFFFu) continue;
-    const uint4 e = reg[i];
-    const uint32_t pos = s_base[e.w] + s_pos[i];
-    if (pos < scap) surv[(uint64_t)e.z * scap + pos] = ((uint64_t)__float_as_uint(s_lb[i]) << 32) | e.x;
   }
 }
 

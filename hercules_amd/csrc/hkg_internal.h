@@ -95,10 +95,11 @@ struct DevIndex {
   int32_t *leaf_node = nullptr;  // nleaf: node of each leaf
   // BF16 filter data (optional)
   uint16_t *Xb = nullptr;      // n x L bf16 bits (round to nearest even)
-  float4 *xst = nullptr;       // n x {|x|^2 down, 2|x| up, 2(|x - x~| + g_L |x~|) up, 0}, x~ = bf16(x)
+  float4 *xst = nullptr;       // n x {|x|^2 down, 2|x| up, 2(|x - x~| + g |x~|) up, g |x|^2 up}, x~ = bf16(x),
+                               // g = 2 (L + 1) 2^-24
   float *xn2 = nullptr;        // n x |x|^2 (down), contiguous for the filter's tile loads
-  float2 *tile_slack = nullptr;  // per 128-row filter tile: max of xst.y and of xst.z over its rows
-  uint32_t ntiles = 0;         // 128-row tiles of the BF16 filter
+  float2 *tile_slack = nullptr;  // per 96-row filter tile: max of xst.y and of xst.z over its rows
+  uint32_t ntiles = 0;         // 96-row tiles of the BF16 filter
   uint32_t *tile_leaf0 = nullptr, *tile_leaf1 = nullptr;  // leaves overlapping each tile
   uint64_t device_bytes = 0;
   // host copies kept for writing / info
