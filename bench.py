@@ -64,6 +64,8 @@ def parse() -> argparse.Namespace:
     p.add_argument("--parity-queries", type=int, default=8)
     p.add_argument("--row-probes", type=int, default=32, help="dataset rows looked up exactly after the build")
     p.add_argument("--bf16-bound-pass", type=int, default=0, choices=[0, 1, 2])
+    p.add_argument("--bf16-query-tiles", type=int, default=0, choices=[0, 2, 3, 4],
+                   help="BF16 filter: 32-query MFMA tiles per wave (0: the autotuner's choice, else 2)")
     p.add_argument("--serve-qps", type=float, default=0.0,
                    help="offered load of the serving leg (0: a sweep at --serve-loads of the measured throughput)")
     p.add_argument("--serve-loads", default="0.5,0.8,0.95",
@@ -493,11 +495,12 @@ def main() -> int:
     kmax = int(kq.max())
     dids = hkg.DeviceArray(B * kmax * 8)
     dd2 = hkg.DeviceArray(B * kmax * 4)
+    qt = args.bf16_query_tiles if args.path == "bf16" else 0
     # per-step views into the resident query array
     def run(step: int, profile: bool) -> dict:
         qv, kv = _views(hkg, dQ, dk, step, B, L)
         return ix.knn_device(qv, B, kv, kmax, dids, dd2, path=args.path, batch=B, profile=profile,
-                             bf16_bound_pass=args.bf16_bound_pass)
+                             bf16_bound_pass=args.bf16_bound_pass, bf16_query_tiles=qt)
 
     for s in range(warm):
         st = run(s, True)
@@ -702,6 +705,7 @@ def main() -> int:
         "config": {"workload": f"{N // 1_000_000}M series x len {L}, leaf {args.leaf}, {B} queries per step, "
                                f"k ~ U{{1..{args.kmax}}}, path {args.path}",
                    "n_series": N, "length": L, "batch": B, "leaf_size": args.leaf, "n_leaves": info["n_leaves"],
+                   "bf16_query_tiles": (qt or tuning.get("bf16_query_tiles") or 2) if args.path == "bf16" else None,
                    "build_s": round(build_s, 2)},
         "roofline": ({"bound": "mfma", "kernel": kernel_names.get(dom, dom), "achieved": achieved_tf,
                       "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved_tf / MFMA_BF16_PEAK_TFLOPS,

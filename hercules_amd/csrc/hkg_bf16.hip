@@ -42,7 +42,6 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 namespace {
 
-constexpr int kQPerWave = 64;              // two 32-query MFMA tiles
 // Waves per filter block: one per SIMD, so a wave can hold its query
 // operands (8 KS registers) with up to 512 registers including the
 // accumulation registers.
@@ -200,7 +199,7 @@ __device__ __forceinline__ void load_lds4(const void *g, lds_void *lds)
 // (global atomics; only pairs passing the lower-bound pre-test get there,
 // which undercounts nothing that matters: an undercount only loosens the
 // bound).
-template <int KS, bool HIST, int W = filter_waves(KS)>
+template <int KS, bool HIST, int QT, int W = filter_waves(KS)>
 __global__ void __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const float *__restrict__ xn2g,
               const float4 *__restrict__ xst, const float2 *__restrict__ tslack, const uint16_t *__restrict__ Qb,
@@ -221,6 +220,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   constexpr int SW = (VPR % 16 == 0) ? 15 : (VPR % 8 == 0) ? 7 : 3;
   constexpr int NCH = kRowTile * VPR / 64;         // 1 KB chunks per tile
   constexpr int kWaves = W;
+  constexpr int QPW = 32 * QT;                     // queries per wave (QT 32-query MFMA tiles)
   // vector memory ops every wave issues per tile, at least: its share of
   // the series chunks and one |x|^2 load (kRowTile / W rows)
   constexpr int kWaitOps = NCH / W + 1;
@@ -249,16 +249,16 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   const uint32_t blk = blockIdx.y * gridDim.x + blockIdx.x;
 
   // this lane's query of each query tile, and its constants
-  uint32_t qid[2];
-  float qn2[2], qe[2], qbn[2], thr[2], hlo[2], hinv[2];
-  bf16x8 qop[2][KS];
+  uint32_t qid[QT];
+  float qn2[QT], qe[QT], qbn[QT], thr[QT], hlo[QT], hinv[QT];
+  bf16x8 qop[QT][KS];
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
+  for (int t = 0; t < QT; ++t) {
     // the batch is ordered by best leaf, so a wave holds neighbouring
     // queries (their survivors fall in the same tiles); waves are dealt to
     // the block groups round-robin, so each group gets a share of every
     // region and the groups' survivor loads even out
-    const uint32_t slot = (wave * gridDim.y + g) * kQPerWave + t * 32 + l32;
+    const uint32_t slot = (wave * gridDim.y + g) * QPW + t * 32 + l32;
     const uint32_t q = slot < nq ? qperm[slot] : 0xFFFFFFFFu;
     qid[t] = q;
     const uint32_t qs = q < nq ? q : 0;
@@ -383,12 +383,13 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
       c[4 * gi + 3] = x4.w;
     }
   };
-  auto pt_elem = [&](int r, const f32x16 &p0, const f32x16 &p1, float &m0, float &m1) __attribute__((always_inline)) {
-    m0 = r == 0 ? p0[r] : fminf(m0, p0[r]);
-    m1 = r == 0 ? p1[r] : fminf(m1, p1[r]);
+  using Acc = f32x16[QT];
+  auto pt_elem = [&](int r, const Acc &p, float (&m)[QT]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int t = 0; t < QT; ++t) m[t] = r == 0 ? p[t][r] : fminf(m[t], p[t][r]);
   };
   // MFMAs of 32-row sub-tile SUB (rows crow = 32 SUB.. of buffer Bc) into
-  // a0/a1, both starting from cin (the sub-tile's |x|^2).  The series
+  // a[0..QT), all starting from cin (the sub-tile's |x|^2).  The series
   // operands are read in batches of G k-steps, each batch issued right after
   // the first MFMA of the batch before: with the tile copies in flight the
   // compiler can only wait for all LDS reads at once, so every read gets most
@@ -400,16 +401,15 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   // into buffer BNc, a third per sub-tile, at batch starts: all waves issuing
   // a whole tile at once would queue on the CU's copy path and stall, while
   // spread over the MFMAs the copies proceed in the background.  With PT,
-  // the pre-test of the previous sub-tile (accumulators p0/p1) runs between
+  // the pre-test of the previous sub-tile (accumulators p) runs between
   // the MFMAs, from the third k-step on (its accumulators are complete by
   // then): the matrix cores stay busy while the wave, alone on its SIMD, does
   // the vector work.
   constexpr int G = KS % 4 == 0 ? 4 : (KS % 3 == 0 ? 3 : 2);  // k-steps per operand batch
   constexpr int NB = KS / G;
   static_assert(KS % G == 0, "operand batches");
-  auto mfma_sub = [&](auto Bc, auto SUBc, f32x16 &a0, f32x16 &a1, auto PTc, const f32x16 &p0, const f32x16 &p1,
-                      float &m0, float &m1, uint64_t r0n, auto BNc, bf16x8 (&opc)[G], f32x16 &cin, auto PREc,
-                      auto NEXTc) __attribute__((always_inline)) {
+  auto mfma_sub = [&](auto Bc, auto SUBc, Acc &a, auto PTc, const Acc &p, float (&m)[QT], uint64_t r0n, auto BNc,
+                      bf16x8 (&opc)[G], f32x16 &cin, auto PREc, auto NEXTc) __attribute__((always_inline)) {
     constexpr bool PT = decltype(PTc)::value;
     constexpr int SUB = decltype(SUBc)::value;
     constexpr bool PRE = decltype(PREc)::value, NEXT = decltype(NEXTc)::value;
@@ -434,8 +434,9 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         const int k = b * G + g;
-        a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(opc[g], qop[0][k], k == 0 ? cin : a0, 0, 0, 0);
-        a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(opc[g], qop[1][k], k == 0 ? cin : a1, 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < QT; ++t)
+          a[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(opc[g], qop[t][k], k == 0 ? cin : a[t], 0, 0, 0);
         if (g == 0) {
           // copies: chunk c_lo + q at batch q NB / c_n; |x|^2 at batch 0 of
           // sub-tile 0
@@ -454,7 +455,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
         }
         if constexpr (PT) {
 #pragma unroll
-          for (int r = done(k); r < done(k + 1); ++r) pt_elem(r, p0, p1, m0, m1);
+          for (int r = done(k); r < done(k + 1); ++r) pt_elem(r, p, m);
         }
         __builtin_amdgcn_sched_barrier(0);
       }
@@ -464,13 +465,13 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
       }
     }
   };
-  auto pretest = [&](const f32x16 &a0, const f32x16 &a1, float &m0, float &m1) __attribute__((always_inline)) {
+  auto pretest = [&](const Acc &a, float (&m)[QT]) __attribute__((always_inline)) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) pt_elem(r, a0, a1, m0, m1);
+    for (int r = 0; r < 16; ++r) pt_elem(r, a, m);
   };
   // Pairs of a sub-tile whose pre-test passed (rare; reads no LDS, so it
   // may run after the tile's buffer is reused); only the query tiles t with
-  // a pass (h0, h1) are looked at.  HIST: the pairs whose upper bound with
+  // a pass (h) are looked at.  HIST: the pairs whose upper bound with
   // their own slack is within the threshold go to the query's histogram,
   // one ballot per pair.  Otherwise each lane with a passing pair writes one
   // record to the wave's slice of the region: {series of its register 0,
@@ -480,16 +481,15 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   // instructions per pair and no scalar round trip, so a wave that finds a
   // survivor holds up its block's barrier for a few hundred cycles, not for
   // a ballot and branch per pair.
-  auto slow = [&](int crow, uint64_t r0, const f32x16 &a0, const f32x16 &a1, float C0, float C1, bool h0,
-                  bool h1) __attribute__((always_inline)) {
+  auto slow = [&](int crow, uint64_t r0, const Acc &a, const float (&C)[QT], const bool (&h)[QT]) __attribute__((always_inline)) {
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      if (!(t == 0 ? h0 : h1)) continue;
-      const float Ct = t == 0 ? C0 : C1;
+    for (int t = 0; t < QT; ++t) {
+      if (!h[t]) continue;
+      const float Ct = C[t];
       if constexpr (HIST) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          const float v = t == 0 ? a0[r] : a1[r];  // xn2 - 2 s
+          const float v = a[t][r];  // xn2 - 2 s
           const bool pass = v <= Ct;
           if (__builtin_expect(__ballot(pass) == 0, 1)) continue;
           const uint64_t grow = r0 + crow + 8 * (r >> 2) + 4 * half + (r & 3);
@@ -508,7 +508,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
         uint32_t bits = 0;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          v[r] = t == 0 ? a0[r] : a1[r];  // xn2 - 2 s
+          v[r] = a[t][r];  // xn2 - 2 s
           bits |= v[r] <= Ct ? 1u << r : 0u;
         }
         const uint64_t m = __ballot(bits != 0);
@@ -580,9 +580,24 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   // sub-tile 2 of the previous step (accumulators, pre-test constants,
   // first row), tested during the next step's first MFMAs; nothing passes
   // before the first tile
-  f32x16 Y0 = {}, Y1 = {};
-  float Cp0 = -FLT_MAX, Cp1 = -FLT_MAX;
+  Acc Y;
+  float Cp[QT];
+#pragma unroll
+  for (int t = 0; t < QT; ++t) {
+    Y[t] = f32x16{};
+    Cp[t] = -FLT_MAX;
+  }
   uint64_t r0p = 0;
+  // which query tiles have a pair within their pre-test constant
+  auto ballots = [&](const float (&m)[QT], const float (&C)[QT], bool (&h)[QT]) __attribute__((always_inline)) {
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < QT; ++t) {
+      h[t] = __ballot(m[t] <= C[t]) != 0;
+      any |= h[t];
+    }
+    return any;
+  };
   auto step = [&](auto Bc, uint32_t j) __attribute__((always_inline)) {
     constexpr int B = decltype(Bc)::value;
     using BN = std::integral_constant<int, (B + 2) % kBuf>;
@@ -616,36 +631,30 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     using S0 = std::integral_constant<int, 0>;
     using S1 = std::integral_constant<int, 1>;
     using S2 = std::integral_constant<int, 2>;
-    f32x16 A0, A1, X0, X1;
-    float m0, m1;
-    bool h0, h1;
+    Acc A, X;
+    float m[QT], C[QT];
+    bool h[QT];
     // sub-tiles 0..2, the MFMAs of one beside the pre-test of the one
     // before (for sub-tile 0: the previous step's sub-tile 2)
     bf16x8 opc[G];  // first operand batch of the next sub-tile
     f32x16 cin;     // |x|^2 of the next sub-tile
-    mfma_sub(Bc, S0{}, A0, A1, PT{}, Y0, Y1, m0, m1, r0n, BN{}, opc, cin, F_{}, T_{});
-    h0 = __ballot(m0 <= Cp0) != 0;
-    h1 = __ballot(m1 <= Cp1) != 0;
-    if (__builtin_expect(h0 || h1, 0)) slow(64, r0p, Y0, Y1, Cp0, Cp1, h0, h1);
+    mfma_sub(Bc, S0{}, A, PT{}, Y, m, r0n, BN{}, opc, cin, F_{}, T_{});
+    if (__builtin_expect(ballots(m, Cp, h), 0)) slow(64, r0p, Y, Cp, h);
     // pre-test constant: xn2 - 2 s <= C  <=>  bound with the tile's largest
     // slack terms <= threshold (the accumulation term g |x|^2 at most
     // g (A_x / 2)^2)
     const float2 tsl = slack_at(j);
     const float dmax = kAccGamma * (0.25f * tsl.x * tsl.x);
-    const float C0 = (thr[0] - qn2[0]) + (qe[0] * tsl.x + qbn[0] * tsl.y + dmax + kEvalSlack);
-    const float C1 = (thr[1] - qn2[1]) + (qe[1] * tsl.x + qbn[1] * tsl.y + dmax + kEvalSlack);
-    mfma_sub(Bc, S1{}, X0, X1, PT{}, A0, A1, m0, m1, r0n, BN{}, opc, cin, T_{}, T_{});
-    h0 = __ballot(m0 <= C0) != 0;
-    h1 = __ballot(m1 <= C1) != 0;
-    if (__builtin_expect(h0 || h1, 0)) slow(0, r0, A0, A1, C0, C1, h0, h1);
-    mfma_sub(Bc, S2{}, Y0, Y1, PT{}, X0, X1, m0, m1, r0n, BN{}, opc, cin, T_{}, F_{});
-    h0 = __ballot(m0 <= C0) != 0;
-    h1 = __ballot(m1 <= C1) != 0;
-    if (__builtin_expect(h0 || h1, 0)) slow(32, r0, X0, X1, C0, C1, h0, h1);
+#pragma unroll
+    for (int t = 0; t < QT; ++t) C[t] = (thr[t] - qn2[t]) + (qe[t] * tsl.x + qbn[t] * tsl.y + dmax + kEvalSlack);
+    mfma_sub(Bc, S1{}, X, PT{}, A, m, r0n, BN{}, opc, cin, T_{}, T_{});
+    if (__builtin_expect(ballots(m, C, h), 0)) slow(0, r0, A, C, h);
+    mfma_sub(Bc, S2{}, Y, PT{}, X, m, r0n, BN{}, opc, cin, T_{}, F_{});
+    if (__builtin_expect(ballots(m, C, h), 0)) slow(32, r0, X, C, h);
     // sub-tile 2 is tested during the next step (from its accumulators, so
     // after the next barrier frees this buffer)
-    Cp0 = C0;
-    Cp1 = C1;
+#pragma unroll
+    for (int t = 0; t < QT; ++t) Cp[t] = C[t];
     r0p = r0;
   };
   static_assert(kBuf == 3, "the loop below cycles three buffers");
@@ -656,10 +665,10 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   }
   // the last tile's sub-tile 2
   {
-    float m0, m1;
-    pretest(Y0, Y1, m0, m1);
-    const bool h0 = __ballot(m0 <= Cp0) != 0, h1 = __ballot(m1 <= Cp1) != 0;
-    if (h0 || h1) slow(64, r0p, Y0, Y1, Cp0, Cp1, h0, h1);
+    float m[QT];
+    bool h[QT];
+    pretest(Y, m);
+    if (ballots(m, Cp, h)) slow(64, r0p, Y, Cp, h);
   }
   // the last steps' copies (re-reads into free buffers) land before the
   // block's LDS is released; siblings no longer wait for this block
@@ -733,14 +742,20 @@ __global__ void k_bf16_tau(uint32_t nq, const uint32_t *__restrict__ kq, const u
 
 }  // namespace
 
+// 32-query MFMA tiles per filter wave: the launch shapes compiled for each
+// length (the query operands, KS registers per tile, share the wave's 256
+// accumulation registers with nothing else; at L = 256 the accumulators of
+// four tiles would not fit beside them)
+bool bf16_query_tiles_ok(uint32_t L, uint32_t qt) { return qt == 2 || qt == 3 || (qt == 4 && L <= 128); }
+
 // queries per filter block (block group) for series of length L
-static uint32_t bf16_qpb(uint32_t L) { return (uint32_t)filter_waves((int)L / 16) * kQPerWave; }
-uint32_t bf16_group_queries(uint32_t L) { return bf16_qpb(L); }
+static uint32_t bf16_qpb(uint32_t L, uint32_t qt) { return (uint32_t)filter_waves((int)L / 16) * 32 * qt; }
+uint32_t bf16_group_queries(uint32_t L, uint32_t qt) { return bf16_qpb(L, qt); }
 
 uint64_t bf16_scratch_per_query(uint32_t L, uint64_t nleaf, uint64_t ntiles)
 {
   return (uint64_t)kBf16SurvivorCap * 8 + 4 + (uint64_t)L * 2 + 16 + 16 + 4 + 4ull * kBf16Bins +
-         (nleaf + 4 * ntiles) / bf16_qpb(L) + 16;
+         (nleaf + 4 * ntiles) / bf16_qpb(L, kBf16QueryTiles) + 16;
 }
 
 // one resident block per CU, split over the block groups; a multiple of 8
@@ -760,27 +775,34 @@ void launch_bf16_filter(hipStream_t st, const DevIndex &dv, uint32_t nq, const u
                         const uint32_t *bound, const uint32_t *qperm, const uint32_t *tiles,
                         const uint32_t *ntiles_g, bool shared_list, uint32_t scap, uint64_t *surv, uint32_t *scnt,
                         const float *thr0, uint32_t *hist, uint4 *region, uint32_t *region_cnt,
-                        uint32_t *prog)
+                        uint32_t *prog, uint32_t qt)
 {
+  if (hist || !bf16_query_tiles_ok(dv.L, qt)) qt = 2;
   const uint32_t stride = shared_list ? 0 : dv.ntiles;
-  const uint32_t qpb = bf16_qpb(dv.L);
+  const uint32_t qpb = bf16_qpb(dv.L, qt);
   const uint32_t groups = (nq + qpb - 1) / qpb;
   dim3 grid(bf16_grid_x(groups), groups);
   launch_fill(st, prog, grid.x * grid.y, 0);
-#define HKG_BF16_LAUNCH(KS)                                                                                       \
-  if (hist)                                                                                                       \
-    hipLaunchKernelGGL((k_bf16_filter<KS, true>), grid, dim3(filter_waves(KS) * 64), 0, st, dv.n, nq, dv.Xb,     \
-                       dv.xn2, dv.xst, dv.tile_slack, Qb, qst, bound, qperm, tiles, stride, ntiles_g, scap,       \
-                       surv, scnt, thr0, hist, region, kRegionCap, region_cnt, prog);                             \
-  else                                                                                                            \
-    hipLaunchKernelGGL((k_bf16_filter<KS, false>), grid, dim3(filter_waves(KS) * 64), 0, st, dv.n, nq, dv.Xb,    \
-                       dv.xn2, dv.xst, dv.tile_slack, Qb, qst, bound, qperm, tiles, stride, ntiles_g, scap,       \
-                       surv, scnt, thr0, hist, region, kRegionCap, region_cnt, prog)
+#define HKG_BF16_LAUNCH(KS, HIST, QT)                                                                             \
+  hipLaunchKernelGGL((k_bf16_filter<KS, HIST, QT>), grid, dim3(filter_waves(KS) * 64), 0, st, dv.n, nq, dv.Xb,   \
+                     dv.xn2, dv.xst, dv.tile_slack, Qb, qst, bound, qperm, tiles, stride, ntiles_g, scap, surv,   \
+                     scnt, thr0, hist, region, kRegionCap, region_cnt, prog)
+  // the histogram pass (rare: the bound pass) runs in the default shape only
+#define HKG_BF16_LAUNCH_QT(KS)                   \
+  if (hist)                                      \
+    HKG_BF16_LAUNCH(KS, true, 2);                \
+  else if (qt == 3)                              \
+    HKG_BF16_LAUNCH(KS, false, 3);               \
+  else if (qt == 4 && KS <= 8)                   \
+    HKG_BF16_LAUNCH(KS, false, (KS <= 8 ? 4 : 2)); \
+  else                                           \
+    HKG_BF16_LAUNCH(KS, false, 2)
   switch (dv.L) {
-  case 96: HKG_BF16_LAUNCH(6); break;
-  case 128: HKG_BF16_LAUNCH(8); break;
-  default: HKG_BF16_LAUNCH(16); break;
+  case 96: HKG_BF16_LAUNCH_QT(6); break;
+  case 128: HKG_BF16_LAUNCH_QT(8); break;
+  default: HKG_BF16_LAUNCH_QT(16); break;
   }
+#undef HKG_BF16_LAUNCH_QT
 #undef HKG_BF16_LAUNCH
   if (!hist) {
     const uint32_t w = (uint32_t)filter_waves((int)dv.L / 16);
@@ -967,8 +989,11 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
   BufLease lease{this, b};  // handed to the caller on success
   const uint32_t cap_override = call_settings().cap_override;
   const uint32_t want_cap = cap_override ? std::max<uint32_t>(cap_override, 16) : kBf16SurvivorCap;
-  const uint32_t qpb = bf16_qpb(L);
-  const uint32_t groups_max = (b->B + qpb - 1) / qpb;
+  const uint32_t qt = bf16_query_tiles_ok(L, call_settings().bf16_qt) ? call_settings().bf16_qt : kBf16QueryTiles;
+  const uint32_t qpb = bf16_qpb(L, qt);
+  // sized for the smallest block group, whatever the shape of this call
+  const uint32_t qpb_min = bf16_qpb(L, 2);
+  const uint32_t groups_max = (b->B + qpb_min - 1) / qpb_min;
   if (b->bf_surv && b->bf_cap != want_cap) b->free_bf16();
   if (!b->bf_surv) {
     b->bf_cap = want_cap;
@@ -1046,7 +1071,8 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
       // within about one bin of the k-th smallest upper bound
       launch_fill(st, b->hist, nq * kBf16Bins, 0);
       launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, kSharedList,
-                         b->bf_cap, b->bf_surv, b->bf_cnt, thr0, b->hist, b->bf_region, b->bf_region_cnt, b->bf_prog);
+                         b->bf_cap, b->bf_surv, b->bf_cnt, thr0, b->hist, b->bf_region, b->bf_region_cnt, b->bf_prog,
+                         qt);
       HKG_HIP_TRY(hipMemcpyAsync(b->misc + 4, b->ntiles_g, lists * 4, hipMemcpyDeviceToDevice, st));
       hipLaunchKernelGGL(k_bf16_tau, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->kq, b->hist, thr0, b->bound);
       plan_tiles(true);
@@ -1054,7 +1080,7 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
     }
     // pass 2: collect the survivors of the certified lower bound
     launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, kSharedList, b->bf_cap,
-                       b->bf_surv, b->bf_cnt, thr0, nullptr, b->bf_region, b->bf_region_cnt, b->bf_prog);
+                       b->bf_surv, b->bf_cnt, thr0, nullptr, b->bf_region, b->bf_region_cnt, b->bf_prog, qt);
     if (p) p->mark(st, HKG_STAGE_REFINE);
     launch_bf16_refine(st, nq, dev.X, L, b->Q, b->kq, b->kmax, dev.ids, b->cap, b->cand, b->cand_cnt, b->bf_cap,
                        b->bf_surv, b->bf_cnt, thr0, b->bound, idmap64, b->out_ids, b->out_d2, overflow, b->counters);
@@ -1062,7 +1088,7 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
     return HKG_OK;
   };
   if (use_graph && !seed_host)
-    rc = run_captured(*b, 2, nq, bound_pass ? 1u : 0u, st, pf, enqueue);
+    rc = run_captured(*b, 2, nq, (bound_pass ? 1u : 0u) | qt << 4, st, pf, enqueue);
   else
     rc = enqueue(pf);
   if (rc != HKG_OK) return rc;

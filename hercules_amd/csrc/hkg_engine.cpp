@@ -946,6 +946,10 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
   if (path == HKG_PATH_BF16) {
     hkg_status rc = make_bf16();  // once per index
     if (rc != HKG_OK) return rc;
+    if (o.bf16_query_tiles && !bf16_query_tiles_ok(dev.L, o.bf16_query_tiles)) {
+      set_error("bf16_query_tiles must be 2, 3 or, for series of length 128 or less, 4");
+      return HKG_EINVAL;
+    }
   }
   if (!dev.has_tree && path != HKG_PATH_SCAN) path = HKG_PATH_SCAN;
   uint32_t batch = o.batch;
@@ -962,6 +966,7 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
   call_settings().cap_override = o.cand_cap;
   call_settings().bound_pass = o.bf16_bound_pass;
   call_settings().batched_filter = o.batched_filter;
+  call_settings().bf16_qt = o.bf16_query_tiles ? o.bf16_query_tiles : bf16_qt();
   call_settings().mode = mode;
   call_settings().mp = ModeParams{o.epsilon, o.eapca_threshold, o.sax_threshold, o.approx_leaves, o.nprobes, 0, 0};
   HKG_HIP_TRY(hipSetDevice(dev.device));
@@ -1168,12 +1173,14 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
   if (!dev.has_tree) path = HKG_PATH_SCAN;
   // a shape: queries per batch, the main kernel's grid (blocks per CU) and,
   // on the batched path, its filter (1 per item, 2 leaf-ordered; the grid
-  // only shapes the per-item filter)
+  // only shapes the per-item filter); on the BF16 path g is the filter's
+  // query tiles per wave, with batches of whole block groups of that shape
   struct Shape {
     uint32_t g, f;
   };
   std::vector<uint32_t> batches;
   std::vector<Shape> shapes;
+  std::vector<std::pair<uint32_t, Shape>> extra;  // (batch, shape) pairs beyond batches x shapes
   uint32_t def_batch = 1024, def_grid = 0;
   switch (path) {
     case HKG_PATH_INDEX:
@@ -1189,7 +1196,12 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
       break;
     case HKG_PATH_BF16:
       batches = {512, 1024, 2048, 4096};
-      shapes = {{0, 0}};
+      shapes = {{kBf16QueryTiles, 0}};
+      def_grid = kBf16QueryTiles;
+      // three tiles per wave: 384 queries per block group
+      for (uint32_t b : {768u, 1536u, 3072u}) extra.push_back({b, {3, 0}});
+      if (bf16_query_tiles_ok(dev.L, 4))
+        for (uint32_t b : {1024u, 2048u, 4096u}) extra.push_back({b, {4, 0}});
       break;
     default:
       batches = {1, 4, 16, 64, 256};
@@ -1202,15 +1214,22 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
   std::vector<uint32_t> bs;
   for (uint32_t b : batches)
     if (b <= nq || bs.empty()) bs.push_back(b);
+  std::vector<std::pair<uint32_t, Shape>> cands;
+  for (uint32_t b : bs)
+    for (const Shape &sh : shapes) cands.push_back({b, sh});
+  for (const auto &c : extra)
+    if (c.first <= nq) cands.push_back(c);
   struct Restore {
     CallSettings saved;
     ~Restore() { call_settings() = saved; }
   } restore{call_settings()};
   const bool scan_path = path == HKG_PATH_SCAN;
   const bool batched = path == HKG_PATH_BATCHED;
+  const bool bf16 = path == HKG_PATH_BF16;
   // the default shape is the untuned one
   tuned_batch[path].store(0, std::memory_order_relaxed);
   if (batched) tuned_filter.store(0, std::memory_order_relaxed);
+  if (bf16) tuned_qt.store(0, std::memory_order_relaxed);
   hkg_query_opts qo;
   hkg_default_query_opts(&qo);
   qo.path = path;
@@ -1220,7 +1239,9 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
   auto run = [&](uint32_t batch, uint32_t grid, uint32_t filt, uint64_t *I, float *D, double *ms) -> hkg_status {
     qo.batch = batch;
     qo.batched_filter = filt;
-    if (scan_path)
+    if (bf16)
+      qo.bf16_query_tiles = grid;
+    else if (scan_path)
       call_settings().sblocks = grid;
     else
       call_settings().fblocks = grid;
@@ -1245,18 +1266,19 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
   };
   std::vector<Timed> timed{{def_batch, def_filt == 2 ? 0u : def_grid, def_filt, ms_def}};
   uint32_t ncand = 1;
-  for (uint32_t b : bs) {
-    for (const Shape &sh : shapes) {
-      if (b == def_batch && sh.f == def_filt && (sh.f == 2 || sh.g == def_grid)) continue;
-      if ((rc = run(b, sh.g, sh.f, ids.data(), d2.data(), &ms)) != HKG_OK) return rc;
-      ++ncand;
-      if (memcmp(ids.data(), ids0.data(), m * 8) != 0 || memcmp(d2.data(), d20.data(), m * 4) != 0) {
-        set_error("autotune: a launch shape changed the answers (batch " + std::to_string(b) + ", grid " +
-                  std::to_string(sh.g) + " blocks per CU, filter " + std::to_string(sh.f) + ")");
-        return HKG_EINTERNAL;
-      }
-      timed.push_back({b, sh.g, sh.f, ms});
+  for (const auto &c : cands) {
+    const uint32_t b = c.first;
+    const Shape &sh = c.second;
+    if (b == def_batch && sh.f == def_filt && (sh.f == 2 || sh.g == def_grid)) continue;
+    if ((rc = run(b, sh.g, sh.f, ids.data(), d2.data(), &ms)) != HKG_OK) return rc;
+    ++ncand;
+    if (memcmp(ids.data(), ids0.data(), m * 8) != 0 || memcmp(d2.data(), d20.data(), m * 4) != 0) {
+      set_error("autotune: a launch shape changed the answers (batch " + std::to_string(b) + ", grid " +
+                std::to_string(sh.g) + (bf16 ? " query tiles per wave" : " blocks per CU") + ", filter " +
+                std::to_string(sh.f) + ")");
+      return HKG_EINTERNAL;
     }
+    timed.push_back({b, sh.g, sh.f, ms});
   }
   double fastest = ms_def;
   for (const Timed &t : timed) fastest = std::min(fastest, t.ms);
@@ -1267,15 +1289,19 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
     if (!best || t.b > best->b || (t.b == best->b && t.ms < best->ms)) best = &t;
   }
   tuned_batch[path].store(best->b, std::memory_order_relaxed);
-  if (best->g) (scan_path ? tuned_sblocks : tuned_fblocks).store(best->g, std::memory_order_relaxed);
+  if (bf16)
+    tuned_qt.store(best->g, std::memory_order_relaxed);
+  else if (best->g)
+    (scan_path ? tuned_sblocks : tuned_fblocks).store(best->g, std::memory_order_relaxed);
   if (batched) tuned_filter.store(best->f, std::memory_order_relaxed);
   memset(out, 0, sizeof(*out));
   out->path = path;
   out->length = dev.L;
   out->batch = best->b;
-  out->filter_blocks_per_cu = scan_path ? 0 : best->g;
+  out->filter_blocks_per_cu = scan_path || bf16 ? 0 : best->g;
   out->scan_blocks_per_cu = scan_path ? best->g : 0;
   out->batched_filter = best->f;
+  out->bf16_query_tiles = bf16 ? best->g : 0;
   out->candidates = ncand;
   out->ms_per_query = best->ms / nq;
   out->ms_per_query_default = ms_def / nq;

@@ -164,6 +164,12 @@ struct Index {
   std::atomic<uint32_t> tuned_fblocks{0};
   std::atomic<uint32_t> tuned_sblocks{0};  // full-scan blocks per CU
   std::atomic<uint32_t> tuned_filter{0};   // batched path's filter at its tuned batch (1 items, 2 leaf; 0 none)
+  std::atomic<uint32_t> tuned_qt{0};       // BF16 filter query tiles per wave (0: kBf16QueryTiles)
+  uint32_t bf16_qt() const
+  {
+    const uint32_t t = tuned_qt.load(std::memory_order_relaxed);
+    return t ? t : kBf16QueryTiles;
+  }
 
   ~Index();
   hkg_status init_device(int device);
@@ -223,13 +229,16 @@ struct CallSettings {
   uint32_t fblocks = 0;       // k_filter_refine blocks per CU (0: the index's tuned or default)
   uint32_t sblocks = 0;       // full-scan blocks per CU (0: the index's tuned or default)
   uint32_t batched_filter = 0;  // hkg_query_opts.batched_filter
+  uint32_t bf16_qt = 0;       // BF16 filter query tiles per wave of this call (0: the default)
   ModeParams mp{};            // its parameters (nleaves / nseries filled per index)
 };
 CallSettings &call_settings();
 // BF16 path scratch of one query beyond the FP32 buffer set
 uint64_t bf16_scratch_per_query(uint32_t L, uint64_t nleaf, uint64_t ntiles);
-// queries of one BF16 filter block group (a batch's unit of work)
-uint32_t bf16_group_queries(uint32_t L);
+// queries of one BF16 filter block group (a batch's unit of work) with qt
+// query tiles per wave, and the tile counts compiled for a length
+uint32_t bf16_group_queries(uint32_t L, uint32_t qt);
+bool bf16_query_tiles_ok(uint32_t L, uint32_t qt);
 // batched iSAX path (hkg_isax.hip): whether a batch of nq queries over nleaf
 // leaves takes the leaf-ordered filter by default, and its breakpoint table
 // (once per device)

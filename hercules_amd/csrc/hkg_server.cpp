@@ -142,7 +142,7 @@ void Server::worker(int wi)
       // partial group leaves filter blocks idle)
       uint32_t bmax = B;
       if (opt.path == HKG_PATH_BF16 && pending.size() < B) {
-        const uint32_t gq = bf16_group_queries(ix->dev.L);
+        const uint32_t gq = bf16_group_queries(ix->dev.L, ix->bf16_qt());
         if (pending.size() > gq) bmax = (uint32_t)(pending.size() / gq * gq);
       }
       while (!pending.empty() && batch.size() < bmax) {

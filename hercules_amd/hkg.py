@@ -40,7 +40,7 @@ class QueryOpts(C.Structure):
     _fields_ = [("path", C.c_int), ("batch", C.c_uint32), ("profile", C.c_uint32), ("cand_cap", C.c_uint32),
                 ("bf16_bound_pass", C.c_uint32), ("epsilon", C.c_float), ("delta", C.c_float),
                 ("nprobes", C.c_uint32), ("approx_leaves", C.c_uint32), ("eapca_threshold", C.c_float),
-                ("sax_threshold", C.c_float), ("batched_filter", C.c_uint32), ("reserved", C.c_uint32)]
+                ("sax_threshold", C.c_float), ("batched_filter", C.c_uint32), ("bf16_query_tiles", C.c_uint32)]
 
 
 BATCHED_FILTERS = {"auto": 0, "items": 1, "leaf": 2}
@@ -53,14 +53,16 @@ def query_opts(path: str = "auto", batch: int = 0, profile: bool = False, cand_c
     items) or "leaf" (leaf-ordered, one query per lane).  ``modes``:
     epsilon, delta, nprobes, approx_leaves, eapca_threshold, sax_threshold
     (the reference's --epsilon, --delta, --nprobes, --approx-stop-condition,
-    --eapca-threshold, --sax-threshold)."""
+    --eapca-threshold, --sax-threshold), and bf16_query_tiles (the BF16
+    filter's 32-query tiles per wave; 0: the tuned or default shape)."""
     qo = QueryOpts()
     lib().hkg_default_query_opts(C.byref(qo))
     qo.path, qo.batch, qo.profile, qo.cand_cap, qo.bf16_bound_pass = (PATHS[path], batch, int(profile), cand_cap,
                                                                        bf16_bound_pass)
     qo.batched_filter = BATCHED_FILTERS[batched_filter]
     for key, v in modes.items():
-        if key not in ("epsilon", "delta", "nprobes", "approx_leaves", "eapca_threshold", "sax_threshold"):
+        if key not in ("epsilon", "delta", "nprobes", "approx_leaves", "eapca_threshold", "sax_threshold",
+                       "bf16_query_tiles"):
             raise TypeError(f"unknown query option {key!r}")
         setattr(qo, key, v)
     return qo
@@ -89,7 +91,7 @@ class QueryStats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("path", C.c_uint32), ("length", C.c_uint32), ("batch", C.c_uint32),
                 ("filter_blocks_per_cu", C.c_uint32), ("candidates", C.c_uint32), ("scan_blocks_per_cu", C.c_uint32),
-                ("batched_filter", C.c_uint32), ("reserved", C.c_uint32),
+                ("batched_filter", C.c_uint32), ("bf16_query_tiles", C.c_uint32),
                 ("ms_per_query", C.c_double), ("ms_per_query_default", C.c_double)]
 
 
@@ -360,7 +362,7 @@ class HerculesIndex:
         return {"path": names[t.path], "length": t.length, "batch": t.batch,
                 "filter_blocks_per_cu": t.filter_blocks_per_cu, "scan_blocks_per_cu": t.scan_blocks_per_cu,
                 "batched_filter": {v: n for n, v in BATCHED_FILTERS.items()}.get(t.batched_filter, "auto"),
-                "candidates": t.candidates,
+                "candidates": t.candidates, "bf16_query_tiles": t.bf16_query_tiles,
                 "ms_per_query": t.ms_per_query, "ms_per_query_default": t.ms_per_query_default}
 
     def knn_device(self, dQ: DeviceArray, nq: int, dk: DeviceArray, kmax: int, dids: DeviceArray,

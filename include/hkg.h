@@ -115,7 +115,10 @@ typedef struct hkg_query_opts {
                               1 = per (query, leaf) work items,
                               2 = leaf-ordered: the queries of each leaf
                               together, one query per lane.  Same answers.  */
-  uint32_t reserved;
+  uint32_t bf16_query_tiles; /* HKG_PATH_BF16: 32-query MFMA tiles per filter
+                              wave (2, 3; 4 for L <= 128), so 128 x this many
+                              queries per filter block.  0 = the autotuner's
+                              choice, else 2.  Same answers.               */
 } hkg_query_opts;
 
 /* stages reported in hkg_query_stats.stage_ms / stage_bytes */
@@ -253,7 +256,9 @@ typedef struct hkg_tuning {
                                     otherwise)                                    */
   uint32_t batched_filter;       /* batched path: the filter chosen (1 per item,
                                     2 leaf-ordered; 0 on the other paths)         */
-  uint32_t reserved;
+  uint32_t bf16_query_tiles;     /* BF16 path: query tiles per filter wave chosen
+                                    (hkg_query_opts.bf16_query_tiles; 0 on the
+                                    other paths)                                  */
   double ms_per_query;           /* device time per query, the chosen shape       */
   double ms_per_query_default;   /* device time per query, the default shape      */
 } hkg_tuning;

@@ -73,7 +73,7 @@ def test_query_opts_layout_and_defaults(tmp_path):
 
     from hercules_amd import hkg
     fields = ["path", "batch", "profile", "cand_cap", "bf16_bound_pass", "epsilon", "delta", "nprobes",
-              "approx_leaves", "eapca_threshold", "sax_threshold", "batched_filter", "reserved"]
+              "approx_leaves", "eapca_threshold", "sax_threshold", "batched_filter", "bf16_query_tiles"]
     sfields = ["device_ms", "bf16_pairs", "stage_ms", "stage_launches", "mode_leaves_first", "mode_sequential"]
     tfields = [f for f, _ in hkg.Tuning._fields_]
     prog = tmp_path / "layout.c"

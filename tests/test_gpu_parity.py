@@ -284,6 +284,39 @@ def test_bf16_embeddings_l96(hkg, oracle, tmp_path):
             assert_same(r.ids, r.d2, ids, d2, f"L96 {path}")
 
 
+@pytest.mark.parametrize("kind,L", [(0, 256), (1, 96), (0, 128)])
+def test_bf16_query_tile_shapes_agree(hkg, oracle, kind, L):
+    """Every query-tile shape of the BF16 filter (2 or 3 32-query MFMA tiles
+    per wave, 4 for L <= 128) gives the oracle's answers, bit for bit, with
+    batches that fill whole block groups and batches that leave the last
+    group partial; a shape not compiled for the length is refused."""
+    import ctypes as C
+    n = 40000
+    _say(f"bf16 query tiles kind={kind} L={L}")
+    X = oracle.series(kind, 31, 0, n, L)
+    rng = np.random.default_rng(32)
+    nq = 900
+    Q, _ = oracle.noisy_queries(kind, 31, n, 33, rng.uniform(0.0, 0.2, nq).astype(np.float32), L)
+    kv = rng.integers(1, 40, nq).astype(np.uint32)
+    ids, d2 = oracle.knn(X, Q, 40)
+    dX = hkg.DeviceArray(X.nbytes)
+    dX.upload(X)
+    with hkg.HerculesIndex.build_device(dX, n, L, leaf_size=700) as ix:
+        for qt in ((2, 3) if L > 128 else (2, 3, 4)):
+            for batch in (0, 128 * qt, 333):
+                r = ix.knn(Q, kv, path="bf16", batch=batch, bf16_query_tiles=qt)
+                for i in range(nq):
+                    k = int(kv[i])
+                    assert np.array_equal(r.d2[i, :k].view(np.uint32), d2[i, :k].view(np.uint32)), (qt, batch, i)
+                    assert np.array_equal(r.ids[i, :k], ids[i, :k]), (qt, batch, i)
+        qo = hkg.query_opts("bf16", 0, False, 0, 0, bf16_query_tiles=4 if L > 128 else 5)
+        out_i = np.empty((4, 5), np.uint64)
+        out_d = np.empty((4, 5), np.float32)
+        rc = hkg.lib().hkg_knn(ix._h, Q[:4].ctypes.data, 4, None, 5, C.byref(qo), out_i.ctypes.data,
+                               out_d.ctypes.data, None)
+        assert rc == hkg.HKG_EINVAL and b"bf16_query_tiles" in hkg.lib().hkg_last_error()
+
+
 def test_bf16_overflow_rerun_stays_exact(hkg, oracle, small_data, ref_index):
     """A tiny survivor list forces the seeded BF16 re-runs."""
     X, Q = small_data

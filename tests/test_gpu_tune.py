@@ -11,8 +11,10 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-CANDIDATE_BATCHES = {"batched": {256, 512, 1024, 2048}, "bf16": {512, 1024, 2048, 4096}, "scan": {1, 4, 16, 64, 256},
-                     "index": {1}}
+CANDIDATE_BATCHES = {"batched": {256, 512, 1024, 2048}, "bf16": {512, 768, 1024, 1536, 2048, 3072, 4096},
+                     "scan": {1, 4, 16, 64, 256}, "index": {1}}
+# BF16 filter shapes: 32-query MFMA tiles per wave, and the batches timed with each
+BF16_SHAPES = {2: {512, 1024, 2048, 4096}, 3: {768, 1536, 3072}, 4: {1024, 2048, 4096}}
 
 
 @pytest.fixture(scope="module")
@@ -50,6 +52,11 @@ def test_autotune_paths(hkg, oracle, kind, L):
                 assert t["scan_blocks_per_cu"] in (3, 6, 8) and t["filter_blocks_per_cu"] == 0
             else:
                 assert t["filter_blocks_per_cu"] == 0 and t["scan_blocks_per_cu"] == 0
+                # a query-tile shape compiled for this length, at one of its batches
+                assert t["bf16_query_tiles"] in ((2, 3) if L > 128 else (2, 3, 4))
+                assert t["batch"] in BF16_SHAPES[t["bf16_query_tiles"]]
+            if path != "bf16":
+                assert t["bf16_query_tiles"] == 0
             assert t["candidates"] >= 1 and t["ms_per_query"] > 0 and t["ms_per_query_default"] > 0
             # the largest batch within 1% of the fastest shape is kept
             assert t["ms_per_query"] <= t["ms_per_query_default"] * 1.01
