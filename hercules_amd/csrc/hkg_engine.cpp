@@ -1153,8 +1153,9 @@ hkg_status Index::knn_host(const float *Q, uint32_t nq, const uint32_t *k, uint3
   return rc;
 }
 
-// Launch-shape tuner: each candidate (queries per batch x filter grid) runs
-// the sample three times and keeps its fastest device time; the default
+// Launch-shape tuner: each candidate (queries per batch x filter grid, or
+// on the BF16 path x query tiles per filter wave) runs the whole batches of
+// the sample three times and keeps its fastest device time per query; the default
 // shape runs first, after one untimed warm-up call (BF16 copy, buffer sets,
 // code objects).  Among shapes within kTuneMargin of the fastest, the one
 // with the largest batch is kept: repeated runs of one shape spread by about
@@ -1236,7 +1237,12 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
   const size_t m = (size_t)nq * kmax;
   std::vector<uint64_t> ids0(m), ids(m);
   std::vector<float> d20(m), d2(m);
-  auto run = [&](uint32_t batch, uint32_t grid, uint32_t filt, uint64_t *I, float *D, double *ms) -> hkg_status {
+  // a shape is timed on the whole batches the sample holds (a partial last
+  // batch would charge the shape for a launch it would not make at scale):
+  // device time per query over the first used(batch) queries
+  auto used = [&](uint32_t batch) { return batch <= nq ? nq / batch * batch : nq; };
+  auto run = [&](uint32_t batch, uint32_t grid, uint32_t filt, uint32_t n, uint64_t *I, float *D,
+                 double *ms) -> hkg_status {
     qo.batch = batch;
     qo.batched_filter = filt;
     if (bf16)
@@ -1248,18 +1254,19 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
     *ms = 1e300;
     for (int rep = 0; rep < 3; ++rep) {
       hkg_query_stats st;
-      hkg_status rc = knn_host(Q, nq, k, kmax, &qo, I, D, &st);
+      hkg_status rc = knn_host(Q, n, k, kmax, &qo, I, D, &st);
       if (rc != HKG_OK) return rc;
-      *ms = std::min(*ms, st.device_ms);
+      *ms = std::min(*ms, st.device_ms / n);
     }
     return HKG_OK;
   };
   // the default shape: default batch and grid, the filter its size picks
   const uint32_t def_filt = batched ? (isax_batched(def_batch, dev.nleaf) ? 2u : 1u) : 0u;
   double ms_def = 0.0, ms;
-  hkg_status rc = run(def_batch, def_grid, def_filt, ids0.data(), d20.data(), &ms);  // warm-up
+  // warm-up and the answers of every sample query, then the default's time
+  hkg_status rc = run(def_batch, def_grid, def_filt, nq, ids0.data(), d20.data(), &ms);
   if (rc != HKG_OK) return rc;
-  if ((rc = run(def_batch, def_grid, def_filt, ids0.data(), d20.data(), &ms_def)) != HKG_OK) return rc;
+  if ((rc = run(def_batch, def_grid, def_filt, used(def_batch), ids.data(), d2.data(), &ms_def)) != HKG_OK) return rc;
   struct Timed {
     uint32_t b, g, f;
     double ms;
@@ -1270,9 +1277,11 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
     const uint32_t b = c.first;
     const Shape &sh = c.second;
     if (b == def_batch && sh.f == def_filt && (sh.f == 2 || sh.g == def_grid)) continue;
-    if ((rc = run(b, sh.g, sh.f, ids.data(), d2.data(), &ms)) != HKG_OK) return rc;
+    const uint32_t n = used(b);
+    if ((rc = run(b, sh.g, sh.f, n, ids.data(), d2.data(), &ms)) != HKG_OK) return rc;
     ++ncand;
-    if (memcmp(ids.data(), ids0.data(), m * 8) != 0 || memcmp(d2.data(), d20.data(), m * 4) != 0) {
+    const size_t mu = (size_t)n * kmax;
+    if (memcmp(ids.data(), ids0.data(), mu * 8) != 0 || memcmp(d2.data(), d20.data(), mu * 4) != 0) {
       set_error("autotune: a launch shape changed the answers (batch " + std::to_string(b) + ", grid " +
                 std::to_string(sh.g) + (bf16 ? " query tiles per wave" : " blocks per CU") + ", filter " +
                 std::to_string(sh.f) + ")");
@@ -1303,8 +1312,8 @@ hkg_status Index::autotune(const float *Q, uint32_t nq, const uint32_t *k, uint3
   out->batched_filter = best->f;
   out->bf16_query_tiles = bf16 ? best->g : 0;
   out->candidates = ncand;
-  out->ms_per_query = best->ms / nq;
-  out->ms_per_query_default = ms_def / nq;
+  out->ms_per_query = best->ms;
+  out->ms_per_query_default = ms_def;
   return HKG_OK;
 }
 

@@ -21,7 +21,7 @@ inline uint32_t scan_group_size(uint32_t nq, uint32_t L)
 constexpr size_t kEapcaLdsBytes = 48 * 1024;
 constexpr int kWorkChunk = 8;            // work items taken per atomic
 constexpr int kFilterRows = 1024;        // rows per iSAX filter step of k_filter_refine
-constexpr int kBestLeaves = 4;           // leaves per query scanned in the approximate stage
+constexpr int kBestLeaves = 6;           // leaves per query scanned in the approximate stage
 constexpr int kBf16Bins = 32;            // threshold histogram bins per query (BF16 path)
 constexpr uint32_t kBf16QueryTiles = 2;   // default 32-query MFMA tiles per BF16 filter wave
 
