@@ -470,21 +470,20 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     for (int r = 0; r < 16; ++r) pt_elem(r, a, m);
   };
   // Pairs of a sub-tile whose pre-test passed (rare; reads no LDS, so it
-  // may run after the tile's buffer is reused); only the query tiles t with
-  // a pass (h) are looked at.  HIST: the pairs whose upper bound with
-  // their own slack is within the threshold go to the query's histogram,
-  // one ballot per pair.  Otherwise each lane with a passing pair writes one
-  // record to the wave's slice of the region: {series of its register 0,
-  // query, slot | pass bits << 16, 0} and its 16 values xn2 - 2 s, five
-  // 16-byte stores at a position from one ballot; k_bf16_scatter tests each
-  // passing pair with its own slack.  The pass bits cost three VALU
-  // instructions per pair and no scalar round trip, so a wave that finds a
-  // survivor holds up its block's barrier for a few hundred cycles, not for
-  // a ballot and branch per pair.
-  auto slow = [&](int crow, uint64_t r0, const Acc &a, const float (&C)[QT], const bool (&h)[QT]) __attribute__((always_inline)) {
+  // may run after the tile's buffer is reused); only the query tiles t whose
+  // ballot hm[t] (lanes with m_t <= C_t) is not empty are looked at.  HIST:
+  // the pairs whose upper bound with their own slack is within the threshold
+  // go to the query's histogram, one ballot per pair.  Otherwise each lane of
+  // the ballot writes one record to the wave's slice of the region: {series
+  // of its register 0, query, slot, 0} and its 16 values xn2 - 2 s, five
+  // 16-byte stores at a position from the ballot; k_bf16_scatter tests each
+  // of the 16 pairs with its own slack.  A wave that finds a survivor spends
+  // a few instructions and stores on it, so it holds up its block's barrier
+  // only briefly.
+  auto slow = [&](int crow, uint64_t r0, const Acc &a, const float (&C)[QT], const uint64_t (&hm)[QT]) __attribute__((always_inline)) {
 #pragma unroll
     for (int t = 0; t < QT; ++t) {
-      if (!h[t]) continue;
+      if (hm[t] == 0) continue;
       const float Ct = C[t];
       if constexpr (HIST) {
 #pragma unroll
@@ -504,33 +503,26 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
           }
         }
       } else {
-        float v[16];
-        uint32_t bits = 0;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          v[r] = a[t][r];  // xn2 - 2 s
-          bits |= v[r] <= Ct ? 1u << r : 0u;
-        }
-        const uint64_t m = __ballot(bits != 0);
+        const uint64_t m = hm[t];
         const uint32_t pos = wcnt + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
         wcnt += (uint32_t)__popcll(m);
         const uint64_t rowb = r0 + crow + 4 * half;  // series of register 0; register r: + 8 (r / 4) + r % 4
-        if (bits != 0) {
+        if ((m >> lane) & 1u) {
           if (pos < wrec) {
             uint4 *rec = region + wbase + (uint64_t)pos * kRecU4;
-            rec[0] = make_uint4((uint32_t)rowb, qid[t], (uint32_t)(t * 32 + l32) | (bits << 16), 0u);
+            rec[0] = make_uint4((uint32_t)rowb, qid[t], (uint32_t)(t * 32 + l32), 0u);
 #pragma unroll
             for (int gi = 0; gi < 4; ++gi)
-              rec[1 + gi] = make_uint4(__float_as_uint(v[4 * gi]), __float_as_uint(v[4 * gi + 1]),
-                                       __float_as_uint(v[4 * gi + 2]), __float_as_uint(v[4 * gi + 3]));
+              rec[1 + gi] = make_uint4(__float_as_uint(a[t][4 * gi]), __float_as_uint(a[t][4 * gi + 1]),
+                                       __float_as_uint(a[t][4 * gi + 2]), __float_as_uint(a[t][4 * gi + 3]));
           } else {
             // the wave's slice is full: tested here, straight to the query's list
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
               const uint64_t grow = rowb + 8 * (r >> 2) + (r & 3);
-              if (((bits >> r) & 1u) && grow < n) {
+              if (a[t][r] <= Ct && grow < n) {
                 const float4 xsv = xst[grow];
-                const float lb = (v[r] + qn2[t]) - (qe[t] * xsv.y + qbn[t] * xsv.z + xsv.w + kEvalSlack);
+                const float lb = (a[t][r] + qn2[t]) - (qe[t] * xsv.y + qbn[t] * xsv.z + xsv.w + kEvalSlack);
                 if (lb <= thr[t]) {
                   const uint32_t qp = atomicAdd(&scnt[qid[t]], 1u);
                   if (qp < scap)
@@ -588,15 +580,15 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     Cp[t] = -FLT_MAX;
   }
   uint64_t r0p = 0;
-  // which query tiles have a pair within their pre-test constant
-  auto ballots = [&](const float (&m)[QT], const float (&C)[QT], bool (&h)[QT]) __attribute__((always_inline)) {
-    bool any = false;
+  // the lanes of each query tile with a pair within their pre-test constant
+  auto ballots = [&](const float (&m)[QT], const float (&C)[QT], uint64_t (&hm)[QT]) __attribute__((always_inline)) {
+    uint64_t any = 0;
 #pragma unroll
     for (int t = 0; t < QT; ++t) {
-      h[t] = __ballot(m[t] <= C[t]) != 0;
-      any |= h[t];
+      hm[t] = __ballot(m[t] <= C[t]);
+      any |= hm[t];
     }
-    return any;
+    return any != 0;
   };
   auto step = [&](auto Bc, uint32_t j) __attribute__((always_inline)) {
     constexpr int B = decltype(Bc)::value;
@@ -633,7 +625,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     using S2 = std::integral_constant<int, 2>;
     Acc A, X;
     float m[QT], C[QT];
-    bool h[QT];
+    uint64_t h[QT];
     // sub-tiles 0..2, the MFMAs of one beside the pre-test of the one
     // before (for sub-tile 0: the previous step's sub-tile 2)
     bf16x8 opc[G];  // first operand batch of the next sub-tile
@@ -666,7 +658,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   // the last tile's sub-tile 2
   {
     float m[QT];
-    bool h[QT];
+    uint64_t h[QT];
     pretest(Y, m);
     if (ballots(m, Cp, h)) slow(64, r0p, Y, Cp, h);
   }
@@ -679,10 +671,11 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
 }
 
 // The survivor records every filter wave wrote (one region slice each, one
-// record per lane and query tile with a pair that passed the pre-test): each
-// passing pair's lower bound with the row's own slack, and the pairs within
-// the query's threshold into its list, keyed by that bound.  One thread per
-// record (usually one or two passing pairs); a global atomic per kept pair.
+// record per lane and query tile whose pre-test passed, with the 16 values
+// xn2 - 2 s of the lane's series): each pair's lower bound with the row's own
+// slack, and the pairs within the query's threshold into its list, keyed by
+// that bound.  One thread per record (usually one or two of its 16 pairs
+// pass); a global atomic per kept pair.
 __global__ void __launch_bounds__(256) k_bf16_scatter(const uint4 *__restrict__ region, uint32_t slice_cap,
                                                       const uint32_t *__restrict__ region_cnt, uint64_t n,
                                                       const float4 *__restrict__ xst, const float4 *__restrict__ qst,
@@ -698,17 +691,21 @@ __global__ void __launch_bounds__(256) k_bf16_scatter(const uint4 *__restrict__ 
     const uint32_t q = h.y;
     const float4 qs = qst[q];
     const float th = thr0[q];
-    for (uint32_t bits = h.z >> 16; bits != 0; bits &= bits - 1) {
-      const int r = __ffs(bits) - 1;
-      const uint64_t row = (uint64_t)h.x + 8 * (r >> 2) + (r & 3);
-      if (row >= n) continue;  // a padding row of the last tile
-      const uint4 vv = rec[1 + (r >> 2)];
-      const uint32_t vb = (r & 3) == 0 ? vv.x : (r & 3) == 1 ? vv.y : (r & 3) == 2 ? vv.z : vv.w;
-      const float4 xsv = xst[row];
-      const float lb = (__uint_as_float(vb) + qs.x) - (qs.y * xsv.y + qs.z * xsv.z + xsv.w + kEvalSlack);
-      if (lb <= th) {
-        const uint32_t p = atomicAdd(&scnt[q], 1u);
-        if (p < scap) surv[(uint64_t)q * scap + p] = ((uint64_t)__float_as_uint(fmaxf(lb, 0.0f)) << 32) | (uint32_t)row;
+#pragma unroll
+    for (int gi = 0; gi < 4; ++gi) {
+      const uint4 vv = rec[1 + gi];
+      const uint32_t vb[4] = {vv.x, vv.y, vv.z, vv.w};
+#pragma unroll
+      for (int i4 = 0; i4 < 4; ++i4) {
+        const uint64_t row = (uint64_t)h.x + 8 * gi + i4;
+        if (row >= n) continue;  // a padding row of the last tile
+        const float v = __uint_as_float(vb[i4]);
+        const float4 xsv = xst[row];
+        const float lb = (v + qs.x) - (qs.y * xsv.y + qs.z * xsv.z + xsv.w + kEvalSlack);
+        if (lb <= th) {
+          const uint32_t p = atomicAdd(&scnt[q], 1u);
+          if (p < scap) surv[(uint64_t)q * scap + p] = ((uint64_t)__float_as_uint(fmaxf(lb, 0.0f)) << 32) | (uint32_t)row;
+        }
       }
     }
   }
