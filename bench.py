@@ -671,7 +671,9 @@ def main() -> int:
         loads = [args.serve_qps] if args.serve_qps > 0 else [f * qps for f in map(float, args.serve_loads.split(","))]
         serving = []
         for inflight in sorted({1, args.serve_inflight}):
-            args_i = argparse.Namespace(**{**vars(args), "serve_inflight": inflight})
+            # batches of up to the step's (tuned) size, so that the server can
+            # carry the throughput the offered loads are fractions of
+            args_i = argparse.Namespace(**{**vars(args), "serve_inflight": inflight, "batch": B})
             for sq in loads:
                 leg = serving_leg(hkg, ix, args_i, Qh, kq, sq)
                 leg["offered_fraction_of_throughput"] = round(sq / qps, 3)

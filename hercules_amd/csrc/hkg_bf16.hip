@@ -48,7 +48,7 @@ namespace {
 __host__ __device__ constexpr int filter_waves(int) { return 4; }
 constexpr int kRowTile = 96;               // series per LDS tile
 constexpr int kBuf = 3;                    // LDS tile buffers (tiles loaded two ahead)
-constexpr uint32_t kRegionCap = 262144;    // 16-byte units per filter block for survivor records (4 MB)
+constexpr uint32_t kRegionCap = 131072;    // 16-byte units per filter block for survivor records (2 MB)
 constexpr uint32_t kRecU4 = 5;             // 16-byte units per survivor record
 constexpr float kEvalSlack = 1.0e-3f;      // FP32 evaluation of the bound
 constexpr uint32_t kBf16SurvivorCap = 524288;  // survivors kept per query (8 B each)
@@ -970,25 +970,15 @@ hkg_status Index::make_bf16()
   return HKG_OK;
 }
 
-hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax, hipStream_t st,
-                                 QueryBuffers **used, Counters *ctr, Prof *pf, const uint32_t *seed_host,
-                                 bool bound_pass, bool use_graph)
+// The BF16 scratch of a buffer set (survivor lists at the call's cap,
+// BF16 queries, block-group tables, survivor region), allocated on first use
+hkg_status Index::ensure_bf16_scratch(QueryBuffers &bs)
 {
-  if (!dev.has_tree) {
-    set_error("the BF16 path needs an index (its first bound comes from the best leaf)");
-    return HKG_EINVAL;
-  }
+  QueryBuffers *b = &bs;
   const uint32_t L = dev.L, nleaf = dev.nleaf;
-  const uint32_t cap = cand_cap(HKG_PATH_BATCHED, nq, kmax);
-  QueryBuffers *b = nullptr;
-  hkg_status rc = get_buffers(nq, kmax, cap, &b);
-  if (rc != HKG_OK) return rc;
-  BufLease lease{this, b};  // handed to the caller on success
   const uint32_t cap_override = call_settings().cap_override;
   const uint32_t want_cap = cap_override ? std::max<uint32_t>(cap_override, 16) : kBf16SurvivorCap;
-  const uint32_t qt = bf16_query_tiles_ok(L, call_settings().bf16_qt) ? call_settings().bf16_qt : kBf16QueryTiles;
-  const uint32_t qpb = bf16_qpb(L, qt);
-  // sized for the smallest block group, whatever the shape of this call
+  // sized for the smallest block group, whatever the shape of a call
   const uint32_t qpb_min = bf16_qpb(L, 2);
   const uint32_t groups_max = (b->B + qpb_min - 1) / qpb_min;
   if (b->bf_surv && b->bf_cap != want_cap) b->free_bf16();
@@ -1016,6 +1006,30 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
       return HKG_ENOMEM;
     }
   }
+  return HKG_OK;
+}
+
+hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t nq, uint32_t kmax, hipStream_t st,
+                                 QueryBuffers **used, Counters *ctr, Prof *pf, const uint32_t *seed_host,
+                                 bool bound_pass, bool use_graph)
+{
+  if (!dev.has_tree) {
+    set_error("the BF16 path needs an index (its first bound comes from the best leaf)");
+    return HKG_EINVAL;
+  }
+  const uint32_t L = dev.L, nleaf = dev.nleaf;
+  // the approximate stage's candidates: the per-query-path size, which does
+  // not depend on the batch (a server's batches of different sizes then
+  // share buffer sets)
+  const uint32_t cap = cand_cap(HKG_PATH_INDEX, nq, kmax);
+  QueryBuffers *b = nullptr;
+  hkg_status rc = get_buffers(nq, kmax, cap, &b);
+  if (rc != HKG_OK) return rc;
+  BufLease lease{this, b};  // handed to the caller on success
+  const uint32_t qt = bf16_query_tiles_ok(L, call_settings().bf16_qt) ? call_settings().bf16_qt : kBf16QueryTiles;
+  const uint32_t qpb = bf16_qpb(L, qt);
+  rc = ensure_bf16_scratch(*b);
+  if (rc != HKG_OK) return rc;
   HKG_HIP_TRY(hipMemcpyAsync(b->Q, dQ, (size_t)nq * L * 4, hipMemcpyDeviceToDevice, st));
   HKG_HIP_TRY(hipMemcpyAsync(b->kq, dk, (size_t)nq * 4, hipMemcpyDeviceToDevice, st));
   uint32_t *overflow = b->misc + 2;

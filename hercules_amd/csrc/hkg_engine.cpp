@@ -394,9 +394,11 @@ QueryBuffers::~QueryBuffers()
   dfree(lab);
 }
 
+// buffer sets kept alive per index (each a batch's scratch: up to a few GB)
+constexpr size_t kMaxBufferSets = 8;
+
 hkg_status Index::get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuffers **out)
 {
-  const uint32_t L = dev.L, nleaf = dev.nleaf;
   B = std::max(B, call_settings().min_buffer_batch);
   const auto key = std::make_tuple(B, kmax, cap);
   std::lock_guard<std::mutex> lock(mu_bufs);
@@ -404,20 +406,42 @@ hkg_status Index::get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuff
   // of a whole batch asks for the shape of its caller)
   auto range = bufs.equal_range(key);
   for (auto it = range.first; it != range.second; ++it) {
-    const QueryBuffers &c = *it->second;
+    QueryBuffers &c = *it->second;
     if (!c.busy && c.B == B && c.kmax == kmax && c.cap == cap) {
-      it->second->busy = true;
-      *out = it->second.get();
+      c.busy = true;
+      c.last_use = ++buf_clock;
+      *out = &c;
       return HKG_OK;
     }
   }
-  // keep at most a few buffer sets alive; never one a call holds
-  if (bufs.size() >= 4) {
-    for (auto e = bufs.begin(); e != bufs.end();) {
-      if (!e->second->busy) e = bufs.erase(e);
-      else ++e;
-    }
+  // keep at most kMaxBufferSets sets alive, dropping the least recently
+  // used one a call does not hold (a server's workers lease several shapes
+  // in turn: freeing them all at once would re-allocate gigabytes per batch)
+  auto evict_lru = [&]() {
+    auto lru = bufs.end();
+    for (auto e = bufs.begin(); e != bufs.end(); ++e)
+      if (!e->second->busy && (lru == bufs.end() || e->second->last_use < lru->second->last_use)) lru = e;
+    if (lru == bufs.end()) return false;
+    bufs.erase(lru);
+    return true;
+  };
+  while (bufs.size() >= kMaxBufferSets && evict_lru()) {
   }
+  hkg_status rc = alloc_buffers(B, kmax, cap, out);
+  if (rc == HKG_ENOMEM) {
+    // device memory held by idle sets: drop them all and try once more
+    while (evict_lru()) {
+    }
+    rc = alloc_buffers(B, kmax, cap, out);
+  }
+  return rc;
+}
+
+// a new buffer set (mu_bufs held), leased to the caller
+hkg_status Index::alloc_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuffers **out)
+{
+  const uint32_t L = dev.L, nleaf = dev.nleaf;
+  const auto key = std::make_tuple(B, kmax, cap);
   std::unique_ptr<QueryBuffers> b(new QueryBuffers());
   b->B = B;
   b->kmax = kmax;
@@ -461,9 +485,31 @@ hkg_status Index::get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuff
 #undef A
   b->bytes = acct;
   b->busy = true;
+  b->last_use = ++buf_clock;
   *out = b.get();
   bufs.emplace(key, std::move(b));
   return HKG_OK;
+}
+
+hkg_status Index::prepare_buffers(hkg_path path, uint32_t B, uint32_t kmax, uint32_t count)
+{
+  if (path == HKG_PATH_AUTO) path = HKG_PATH_BATCHED;
+  if (!dev.has_tree) path = HKG_PATH_SCAN;
+  hkg_status rc = HKG_OK;
+  if (path == HKG_PATH_BF16 && (rc = make_bf16()) != HKG_OK) return rc;
+  HKG_HIP_TRY(hipSetDevice(dev.device));
+  // the caps the batch paths compute for a set of B queries
+  const uint32_t cap = path == HKG_PATH_BF16 ? cand_cap(HKG_PATH_INDEX, B, kmax) : cand_cap(path, B, kmax);
+  std::vector<QueryBuffers *> held;
+  for (uint32_t i = 0; i < count && rc == HKG_OK; ++i) {
+    QueryBuffers *b = nullptr;
+    rc = get_buffers(B, kmax, cap, &b);
+    if (rc != HKG_OK) break;
+    held.push_back(b);
+    if (path == HKG_PATH_BF16) rc = ensure_bf16_scratch(*b);
+  }
+  for (QueryBuffers *b : held) release_buffers(b);
+  return rc;
 }
 
 void Index::release_buffers(QueryBuffers *b)
@@ -681,7 +727,10 @@ hkg_status Index::run_batch(hkg_path path, const float *dQ, const uint32_t *dk, 
                             const uint32_t *seed_host)
 {
   const bool scan = (path == HKG_PATH_SCAN) || !dev.has_tree;
-  const uint32_t cap = cand_cap(scan ? HKG_PATH_SCAN : path, nq, kmax);
+  // sized for the buffer set's batch, not this call's: a server's batches of
+  // different sizes then lease the same sets
+  const uint32_t cap = scan ? cand_cap(HKG_PATH_SCAN, nq, kmax)
+                            : cand_cap(path, std::max(nq, call_settings().min_buffer_batch), kmax);
   QueryBuffers *b = nullptr;
   hkg_status rc = get_buffers(nq, kmax, cap, &b);
   if (rc != HKG_OK) return rc;

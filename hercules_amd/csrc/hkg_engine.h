@@ -134,6 +134,7 @@ struct QueryBuffers {
   uint64_t *vtop = nullptr;     // B x kmax: their answers after the approximate phase
   unsigned long long *lab = nullptr;  // B: their iSAX counts
   bool busy = false;            // held by a query call (guarded by Index::mu_bufs)
+  uint64_t last_use = 0;        // Index::buf_clock at the last lease (guarded by Index::mu_bufs)
   // captured stage sequence of a batched path (FP32 or BF16), and the shape
   // it was captured for
   hipGraphExec_t graph_exec = nullptr;
@@ -158,6 +159,7 @@ struct Index {
   bool bf16_ready = false;      // BF16 copy, norms and tile tables built (guarded by mu_bf16)
   // key: (queries, kmax, candidate entries per query)
   std::multimap<std::tuple<uint32_t, uint32_t, uint32_t>, std::unique_ptr<QueryBuffers>> bufs;
+  uint64_t buf_clock = 0;  // lease counter for the least-recently-used eviction (guarded by mu_bufs)
   // launch shapes chosen by autotune (0: the defaults): queries per batch
   // of each path, and k_filter_refine blocks per CU
   std::atomic<uint32_t> tuned_batch[HKG_PATH_BF16 + 1] = {};
@@ -176,6 +178,12 @@ struct Index {
   hkg_status upload_tree();
   hkg_status set_ids(const uint64_t *ids, uint64_t n);
   hkg_status make_bf16();
+  hkg_status alloc_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuffers **out);
+  // the BF16 scratch of a buffer set, allocated on first use
+  hkg_status ensure_bf16_scratch(QueryBuffers &b);
+  // allocate `count` buffer sets of the shape a batch of B queries at kmax
+  // leases on `path` (a server's workers, before their first batch)
+  hkg_status prepare_buffers(hkg_path path, uint32_t B, uint32_t kmax, uint32_t count);
   hkg_status get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuffers **out);
   void release_buffers(QueryBuffers *b);
   // device scratch one query of a batch needs on a path (buffer sets)

@@ -276,7 +276,7 @@ hkg_status Index::run_isax_batch(const float *dQ, const uint32_t *dk, uint32_t n
                                  bool use_graph)
 {
   const uint32_t L = dev.L, nleaf = dev.nleaf;
-  const uint32_t cap = cand_cap(HKG_PATH_BATCHED, nq, kmax);
+  const uint32_t cap = cand_cap(HKG_PATH_BATCHED, std::max(nq, call_settings().min_buffer_batch), kmax);
   QueryBuffers *b = nullptr;
   hkg_status rc = get_buffers(nq, kmax, cap, &b);
   if (rc != HKG_OK) return rc;

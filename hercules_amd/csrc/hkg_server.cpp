@@ -247,6 +247,13 @@ hkg_status hkg_server_start(hkg_index *index, const hkg_server_opts *so, hkg_ser
     hkg_status rc = s->ix->make_bf16();
     if (rc != HKG_OK) return rc;
   }
+  {
+    // the workers' buffer sets for a full batch at the largest k class,
+    // allocated now rather than by the first requests (a few GB each)
+    const uint32_t bq = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(o.max_batch, s->batch_scratch / s->per_query(o.max_k)));
+    hkg_status rc = s->ix->prepare_buffers((hkg_path)o.path, bq, o.max_k, o.max_inflight);
+    if (rc != HKG_OK) return rc;
+  }
   for (uint32_t i = 0; i < o.max_inflight; ++i) s->workers.emplace_back(&Server::worker, s.get(), (int)i);
   *out = reinterpret_cast<hkg_server *>(s.release());
   return HKG_OK;
