@@ -130,3 +130,19 @@ def test_graph_replays_equal_eager_runs(hkg, setup, path, filt):
         # eager work of other shapes between replays
         ix.knn(Q[n:n + 17], 5, path="scan")
         ix.knn(Q[n:n + 9], 7, path=path, profile=True, batched_filter=filt)
+
+
+@pytest.mark.parametrize("path", ["batched", "bf16"])
+def test_buffer_sets_evicted_and_rebuilt_keep_answers(hkg, setup, path):
+    """More buffer-set shapes than the index keeps alive (each distinct
+    (batch, k) leases its own set; the least recently used idle one is
+    dropped): every call, on a fresh or a re-allocated set, returns the
+    oracle's answers."""
+    ix, Q, kv, ids, d2 = setup
+    _say(f"buffer-set eviction path={path}")
+    shapes = [(b, k) for b in (16, 48) for k in (1, 2, 5, 10, 37, 64)]  # 12 shapes > 8 kept
+    for rep in range(2):
+        for b, k in shapes:
+            r = ix.knn(Q[:b], k, path=path, batch=b)
+            assert np.array_equal(r.d2.view(np.uint32), d2[:b, :k].view(np.uint32)), (rep, b, k)
+            assert np.array_equal(r.ids, ids[:b, :k]), (rep, b, k)
