@@ -242,7 +242,10 @@ hkg_status hkg_scan_knn_device_noalloc(hkg_index *ix, const float *dQ, uint32_t 
    (blocks per CU): the iSAX filter + refine kernel on the FP32 index paths,
    the scan kernel on the full scan; on the batched path also its filter
    (per item or leaf-ordered, hkg_query_opts.batched_filter), kept for calls
-   that leave batched_filter at 0 and whose batch is the tuned one.  Every
+   that leave batched_filter at 0 and whose batch is the tuned one; on the
+   BF16 path the filter's 32-query MFMA tiles per wave
+   (hkg_query_opts.bf16_query_tiles), with batches of whole block groups.
+   Each shape is timed on the whole batches the sample holds.  Every
    candidate must return the same answers, bit for bit (HKG_EINTERNAL
    otherwise). */
 typedef struct hkg_tuning {
