@@ -1059,7 +1059,7 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
     if (p) p->mark(st, HKG_STAGE_EAPCA);
     launch_best_leaf(st, nq, nleaf, b->lb, b->best);
     if (p) p->mark(st, HKG_STAGE_SELECT_LEAF);
-    launch_scan_best_bf16(st, nq, dev.X, dev.Xb, dev.xn2, dev.max_leaf, L, b->Q, b->kq, b->best, dev.leaf_pos,
+    launch_scan_best_bf16(st, nq, dev.X, dev.Xb, dev.xn2, dev.max_leaf, L, b->Q, b->kq, dev.sax, b->paa, b->best, dev.leaf_pos,
                           dev.leaf_size, dev.ids, b->cap, b->cand, b->cand_cnt, b->bound, b->counters);
     if (p) p->mark(st, HKG_STAGE_APPROX);
     uint32_t *skeys = b->qperm + b->B, *sidx = b->qperm + 2 * b->B;

@@ -81,8 +81,12 @@ void launch_scan_best(hipStream_t st, uint32_t nq, const float *X, uint32_t L, c
 // for the best-ranked rows (falls back to launch_scan_best when the largest
 // leaf's ranking keys exceed kApproxMaxDynLds)
 constexpr size_t kApproxMaxDynLds = 64 * 1024;
+// the best leaves' rows ranked by a BF16 dot, the best k + 32 of each refined
+// exactly: the first leaf of every query, then the others with the rows whose
+// iSAX bound (sax words, query PAA) is within the k-th distance so far
 void launch_scan_best_bf16(hipStream_t st, uint32_t nq, const float *X, const uint16_t *Xb, const float *xn2,
-                           uint32_t max_leaf, uint32_t L, const float *Q, const uint32_t *kq, const uint32_t *best,
+                           uint32_t max_leaf, uint32_t L, const float *Q, const uint32_t *kq, const uint8_t *sax,
+                           const float *paa, const uint32_t *best,
                            const uint64_t *leaf_pos, const uint32_t *leaf_size, const uint32_t *ids, uint32_t cap,
                            uint64_t *cand, uint32_t *cand_cnt, uint32_t *bound, unsigned long long *counters);
 void launch_collect(hipStream_t st, uint32_t nq, uint32_t nleaf, const float *lb, const uint32_t *best,

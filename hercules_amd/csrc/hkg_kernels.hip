@@ -462,11 +462,14 @@ __global__ void __launch_bounds__(kBlock) k_scan_best(const float *__restrict__ 
 // row of the leaf, a region the exact refinement reuses once the rows are
 // chosen (so three blocks fit a CU).
 constexpr uint32_t kApproxMargin = 32;  // rows refined beyond k
+template <bool PRUNE>
 __global__ void __launch_bounds__(kBlock) k_scan_best_bf16(const float *__restrict__ X,
                                                            const uint16_t *__restrict__ Xb,
                                                            const float *__restrict__ xn2, uint32_t L,
                                                            const float *__restrict__ Q,
                                                            const uint32_t *__restrict__ kq,
+                                                           const uint8_t *__restrict__ sax,
+                                                           const float *__restrict__ paa, uint32_t first_leaf,
                                                            const uint32_t *__restrict__ best,
                                                            const uint64_t *__restrict__ leaf_pos,
                                                            const uint32_t *__restrict__ leaf_size,
@@ -484,47 +487,82 @@ __global__ void __launch_bounds__(kBlock) k_scan_best_bf16(const float *__restri
   __shared__ uint64_t rows[kBlock];
   __shared__ uint32_t scan[kBlock];
   __shared__ uint32_t hist[256];
-  __shared__ uint32_t s_prefix, s_rank, s_nsel, s_ties;
+  __shared__ uint32_t s_prefix, s_rank, s_nsel, s_ties, s_kept;
+  __shared__ float sbp[PRUNE ? 256 : 1];
+  __shared__ float spaa[kPaaSegments];
   const uint32_t q = blockIdx.x;
   const int k = (int)kq[q];
-  const uint32_t leaf = best[q * kBestLeaves + blockIdx.y];
+  const uint32_t leaf = best[q * kBestLeaves + first_leaf + blockIdx.y];
   if (leaf == 0xFFFFFFFFu) return;  // uniform for the block
   load_query(Q, q, L, qs);
   topk_reset(tk);
+  if (PRUNE) {
+    for (int i = threadIdx.x; i < 256; i += kBlock) sbp[i] = c_bp[i];
+    if (threadIdx.x < kPaaSegments) spaa[threadIdx.x] = paa[q * kPaaSegments + threadIdx.x];
+  }
+  if (threadIdx.x == 0) s_kept = 0;
   const uint64_t pos = leaf_pos[leaf];
   const uint32_t size = leaf_size[leaf];
+  // PRUNE: the query's k-th distance bound after its best leaf (an earlier
+  // launch); a row whose iSAX lower bound is beyond it cannot enter the
+  // top-k, so it is not ranked (its key stays the largest)
+  const uint32_t tau = PRUNE ? __atomic_load_n(&bound[q], __ATOMIC_RELAXED) : kInfBits;
+  const float ratio = (float)(L / kPaaSegments);
   __syncthreads();
   // ranking keys: one row per thread, read 256 bytes (16 loads) at a time;
   // the float is mapped to an unsigned key of the same order
   const uint32_t nc = L / 8;  // 16-byte chunks per row
+  uint32_t kept = 0;
   for (uint32_t i = threadIdx.x; i < size; i += kBlock) {
-    const uint4 *xr = reinterpret_cast<const uint4 *>(Xb + (pos + i) * L);
-    float dot = 0.0f;
-    for (uint32_t c0 = 0; c0 < nc; c0 += 16) {
-      uint4 v[16];
-#pragma unroll
-      for (int u = 0; u < 16; ++u) v[u] = c0 + u < nc ? xr[c0 + u] : make_uint4(0, 0, 0, 0);
-#pragma unroll
-      for (int u = 0; u < 16; ++u) {
-        if (c0 + u >= nc) break;
-        const float4 qa = *reinterpret_cast<const float4 *>(qs + 8 * (c0 + u));
-        const float4 qb = *reinterpret_cast<const float4 *>(qs + 8 * (c0 + u) + 4);
-        dot = fmaf(qa.x, __uint_as_float(v[u].x << 16), dot);
-        dot = fmaf(qa.y, __uint_as_float(v[u].x & 0xFFFF0000u), dot);
-        dot = fmaf(qa.z, __uint_as_float(v[u].y << 16), dot);
-        dot = fmaf(qa.w, __uint_as_float(v[u].y & 0xFFFF0000u), dot);
-        dot = fmaf(qb.x, __uint_as_float(v[u].z << 16), dot);
-        dot = fmaf(qb.y, __uint_as_float(v[u].z & 0xFFFF0000u), dot);
-        dot = fmaf(qb.z, __uint_as_float(v[u].w << 16), dot);
-        dot = fmaf(qb.w, __uint_as_float(v[u].w & 0xFFFF0000u), dot);
-      }
+    bool keep = true;
+    if (PRUNE) {
+      const uint4 w = *reinterpret_cast<const uint4 *>(sax + (pos + i) * kPaaSegments);
+      keep = lb_keep(lb_sax_word(spaa, w, sbp, ratio), tau);
     }
-    const uint32_t b = __float_as_uint(fmaf(-2.0f, dot, xn2[pos + i]));
-    key[i] = b ^ ((b >> 31) ? 0xFFFFFFFFu : 0x80000000u);
+    uint32_t kk = 0xFFFFFFFFu;
+    if (keep) {
+      const uint4 *xr = reinterpret_cast<const uint4 *>(Xb + (pos + i) * L);
+      float dot = 0.0f;
+      for (uint32_t c0 = 0; c0 < nc; c0 += 16) {
+        uint4 v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = c0 + u < nc ? xr[c0 + u] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          if (c0 + u >= nc) break;
+          const float4 qa = *reinterpret_cast<const float4 *>(qs + 8 * (c0 + u));
+          const float4 qb = *reinterpret_cast<const float4 *>(qs + 8 * (c0 + u) + 4);
+          dot = fmaf(qa.x, __uint_as_float(v[u].x << 16), dot);
+          dot = fmaf(qa.y, __uint_as_float(v[u].x & 0xFFFF0000u), dot);
+          dot = fmaf(qa.z, __uint_as_float(v[u].y << 16), dot);
+          dot = fmaf(qa.w, __uint_as_float(v[u].y & 0xFFFF0000u), dot);
+          dot = fmaf(qb.x, __uint_as_float(v[u].z << 16), dot);
+          dot = fmaf(qb.y, __uint_as_float(v[u].z & 0xFFFF0000u), dot);
+          dot = fmaf(qb.z, __uint_as_float(v[u].w << 16), dot);
+          dot = fmaf(qb.w, __uint_as_float(v[u].w & 0xFFFF0000u), dot);
+        }
+      }
+      const uint32_t b = __float_as_uint(fmaf(-2.0f, dot, xn2[pos + i]));
+      kk = b ^ ((b >> 31) ? 0xFFFFFFFFu : 0x80000000u);
+      ++kept;
+    }
+    key[i] = kk;
+  }
+  if (PRUNE) {
+    for (int off = 32; off > 0; off >>= 1) kept += __shfl_down(kept, off);
+    if ((threadIdx.x & 63) == 0 && kept) atomicAdd(&s_kept, kept);
   }
   __syncthreads();
+  if (PRUNE && s_kept == 0) {
+    // every row of the leaf is beyond the bound: nothing to refine
+    if (threadIdx.x == 0) {
+      atomicAdd(&counters[kCtrApprox], (unsigned long long)size);
+      atomicAdd(&counters[kCtrApproxLeaves], 1ull);
+    }
+    return;
+  }
   // the M-th smallest key: radix select, 8 bits per pass from the top
-  const uint32_t M = min(size, (uint32_t)min(k + (int)kApproxMargin, kBlock));
+  const uint32_t M = min(PRUNE ? s_kept : size, (uint32_t)min(k + (int)kApproxMargin, kBlock));
   if (threadIdx.x == 0) {
     s_prefix = 0;
     s_rank = M;
@@ -1088,7 +1126,8 @@ void launch_best_leaf(hipStream_t st, uint32_t nq, uint32_t nleaf, const float *
 }
 
 void launch_scan_best_bf16(hipStream_t st, uint32_t nq, const float *X, const uint16_t *Xb, const float *xn2,
-                           uint32_t max_leaf, uint32_t L, const float *Q, const uint32_t *kq, const uint32_t *best,
+                           uint32_t max_leaf, uint32_t L, const float *Q, const uint32_t *kq, const uint8_t *sax,
+                           const float *paa, const uint32_t *best,
                            const uint64_t *leaf_pos, const uint32_t *leaf_size, const uint32_t *ids, uint32_t cap,
                            uint64_t *cand, uint32_t *cand_cnt, uint32_t *bound, unsigned long long *counters)
 {
@@ -1098,8 +1137,13 @@ void launch_scan_best_bf16(hipStream_t st, uint32_t nq, const float *X, const ui
     launch_scan_best(st, nq, X, L, Q, kq, best, leaf_pos, leaf_size, ids, cap, cand, cand_cnt, bound, counters);
     return;
   }
-  hipLaunchKernelGGL(k_scan_best_bf16, dim3(nq, kBestLeaves), dim3(kBlock), dyn, st, X, Xb, xn2, L, Q, kq, best,
-                     leaf_pos, leaf_size, ids, cap, cand, cand_cnt, bound, counters);
+  // the best leaf of every query first, then the others with the rows whose
+  // iSAX bound is within the k-th distance that leaf gave
+  hipLaunchKernelGGL((k_scan_best_bf16<false>), dim3(nq, 1), dim3(kBlock), dyn, st, X, Xb, xn2, L, Q, kq, sax, paa,
+                     0u, best, leaf_pos, leaf_size, ids, cap, cand, cand_cnt, bound, counters);
+  if (kBestLeaves > 1)
+    hipLaunchKernelGGL((k_scan_best_bf16<true>), dim3(nq, kBestLeaves - 1), dim3(kBlock), dyn, st, X, Xb, xn2, L, Q,
+                       kq, sax, paa, 1u, best, leaf_pos, leaf_size, ids, cap, cand, cand_cnt, bound, counters);
 }
 
 void launch_scan_best(hipStream_t st, uint32_t nq, const float *X, uint32_t L, const float *Q,
