@@ -4,7 +4,8 @@ dataset in seconds:
 - 25M x 256 random walks, 1,000 1-NN queries: dataset series plus noise of
   sigma 0.01, 0.05 and 0.1 (333 / 333 / 334), renormalised;
 - 50M x 256 random walks, 10,000 queries with sigma ~ U[0.01, 0.2] and k
-  drawn from {1, 10, 100};
+  drawn from {1, 10, 100}, on the batch sizes and kernel shapes the autotuner
+  picks for the BF16 and the batched path;
 - 100M x 96 embeddings (N(0,1) -> L2 -> z-normalised), 10,000 10-NN
   queries of the same kind (sigma ~ U[0.01, 0.2]);
 - 100M x 256 random walks (the serving workload), 4,096 queries with
@@ -56,7 +57,7 @@ WORKLOADS = {
     "25Mx256_k1": dict(kind=0, n=25_000_000, L=256, leaf=10000, nq=1000, qseed=3, batched_all=True,
                        sigma=lambda rng, nq: np.repeat(np.float32([0.01, 0.05, 0.1]), [333, 333, 334]),
                        k=lambda rng, nq: np.ones(nq, np.uint32)),
-    "50Mx256_kmix": dict(kind=0, n=50_000_000, L=256, leaf=10000, nq=10000, qseed=4, batched_all=True,
+    "50Mx256_kmix": dict(kind=0, n=50_000_000, L=256, leaf=10000, nq=10000, qseed=4, batched_all=True, tune=True,
                          sigma=lambda rng, nq: rng.uniform(0.01, 0.2, nq).astype(np.float32),
                          k=lambda rng, nq: rng.choice(np.uint32([1, 10, 100]), nq)),
     "100Mx96_k10": dict(kind=1, n=100_000_000, L=96, leaf=10000, nq=10000, qseed=6, batched_all=False,
@@ -91,6 +92,14 @@ def test_full_size_properties(hkg, oracle, name):
     with hkg.HerculesIndex.build_device(dX, n, L, leaf_size=w["leaf"]) as ix:
         _say(f"sizes {name}: built, {ix.info()['n_leaves']} leaves, {time.perf_counter() - t0:.1f} s")
 
+        if w.get("tune"):
+            # the batched paths in the shapes the autotuner picks on this index
+            # (later calls leave the batch at 0: tuned batch, BF16 query tiles,
+            # FP32 filter; full batches replay their captured graph)
+            for path in ("bf16", "batched"):
+                t = ix.autotune(Q[:2048], kq[:2048], path=path)
+                _say(f"sizes {name}: tuned {path}: {t}")
+                assert t["candidates"] > 1 and t["ms_per_query"] <= t["ms_per_query_default"] * 1.01
         # every query on the BF16 path, and on the batched path where its iSAX
         # bound prunes (random walks; on the embeddings it would scan nearly
         # every series per query, so there it runs on the sample)
