@@ -207,7 +207,8 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
               const uint32_t *__restrict__ tiles, uint32_t tile_stride, const uint32_t *__restrict__ ntiles_g,
               uint32_t scap, uint64_t *__restrict__ surv, uint32_t *__restrict__ scnt, const float *__restrict__ thr0,
               uint32_t *__restrict__ hist, uint4 *__restrict__ region, uint32_t region_cap,
-              uint32_t *__restrict__ region_cnt, uint32_t *prog)
+              uint32_t *__restrict__ region_cnt, uint32_t *prog, const uint32_t *__restrict__ slot_need,
+              uint32_t slot_words, unsigned long long *__restrict__ counters)
 {
   constexpr int L = KS * 16;
   constexpr int VPR = L / 8;                       // 16-byte vectors per row
@@ -247,6 +248,9 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar loops and copy bases
   const uint32_t g = blockIdx.y;
   const uint32_t blk = blockIdx.y * gridDim.x + blockIdx.x;
+  // this wave's slot: QPW consecutive queries of the batch order (waves are
+  // dealt to the block groups round-robin)
+  const uint32_t wslot = wave * gridDim.y + g;
 
   // this lane's query of each query tile, and its constants
   uint32_t qid[QT];
@@ -258,7 +262,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     // queries (their survivors fall in the same tiles); waves are dealt to
     // the block groups round-robin, so each group gets a share of every
     // region and the groups' survivor loads even out
-    const uint32_t slot = (wave * gridDim.y + g) * QPW + t * 32 + l32;
+    const uint32_t slot = wslot * QPW + t * 32 + l32;
     const uint32_t q = slot < nq ? qperm[slot] : 0xFFFFFFFFu;
     qid[t] = q;
     const uint32_t qs = q < nq ? q : 0;
@@ -319,9 +323,17 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     const uint32_t j = c * 64 + lane;
     return j < my_n ? tlist[blockIdx.x + (uint64_t)j * gridDim.x] : 0u;
   };
+  // slot_need (nullable): per tile, one bit per wave slot, set when a query
+  // of the slot needs a leaf the tile overlaps (EAPCA bound within its k-th
+  // distance bound); a wave whose bit is clear skips the tile's MFMAs.  The
+  // wave's bit rides in bit 31 of the tile id it holds (tile ids are far
+  // below 2^31), so it costs no register.
+  const uint32_t nw_off = wslot >> 5, nw_bit = wslot & 31;
   auto fetch_chunk = [&](uint32_t c, uint32_t &id, float2 &ts) {
     id = id_of(c);
     ts = tslack[id];
+    const uint32_t nw = slot_need ? slot_need[(uint64_t)id * slot_words + nw_off] : 0xFFFFFFFFu;
+    id |= ((nw >> nw_bit) & 1u) << 31;
     asm volatile("" ::"v"(id), "v"(ts.x), "v"(ts.y));
   };
   uint32_t cid, nid;
@@ -330,7 +342,8 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   fetch_chunk(1, nid, nts);
   // id of the block's j-th tile, from the chunk of step jc or the next
   auto tile_at = [&](uint32_t j, uint32_t jc) -> uint32_t {
-    return (j >> 6) == (jc >> 6) ? __builtin_amdgcn_readlane(cid, j & 63) : __builtin_amdgcn_readlane(nid, j & 63);
+    return ((j >> 6) == (jc >> 6) ? __builtin_amdgcn_readlane(cid, j & 63) : __builtin_amdgcn_readlane(nid, j & 63)) &
+           0x7FFFFFFFu;
   };
   auto slack_at = [&](uint32_t j) -> float2 {
     return make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(cts.x), j & 63)),
@@ -590,6 +603,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     }
     return any != 0;
   };
+  uint32_t skipped = 0;  // steps this wave skipped
   auto step = [&](auto Bc, uint32_t j) __attribute__((always_inline)) {
     constexpr int B = decltype(Bc)::value;
     using BN = std::integral_constant<int, (B + 2) % kBuf>;
@@ -617,6 +631,22 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
     // copies every step, and no branch among the MFMAs)
     const uint64_t r0n = (uint64_t)tile_at(j + 2, j) * kRowTile;
     const uint64_t r0 = (uint64_t)tile_at(j, j) * kRowTile;
+    if (!(__builtin_amdgcn_readlane(cid, j & 63) >> 31)) {
+      // no query of this wave needs the tile: its copies of tile j + 2 (the
+      // same count as a computing step issues), the previous step's sub-tile
+      // 2, and no MFMAs
+#pragma unroll
+      for (int i = 0; i < NCI; ++i) issue_chunk(r0n, BN{}, i);
+      issue_stats(r0n, BN{});
+      float m[QT];
+      uint64_t h[QT];
+      pretest(Y, m);
+      if (__builtin_expect(ballots(m, Cp, h), 0)) slow(64, r0p, Y, Cp, h);
+#pragma unroll
+      for (int t = 0; t < QT; ++t) Cp[t] = -FLT_MAX;  // nothing pending for the next step
+      ++skipped;
+      return;
+    }
     using PT = std::true_type;
     using T_ = std::true_type;
     using F_ = std::false_type;
@@ -668,6 +698,10 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   if (pace_grid && wave == 0 && lane == 0)
     __hip_atomic_store(&prog[blk], 0xFFFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (!HIST && lane == 0) region_cnt[(uint64_t)blk * kWaves + wave] = wcnt < wrec ? wcnt : wrec;
+  // (query, series) pairs of this wave's real queries it skipped
+  const uint32_t nreal = nq > wslot * QPW ? min((uint32_t)QPW, nq - wslot * QPW) : 0u;
+  if (counters && lane == 0 && skipped && nreal)
+    atomicAdd(&counters[kCtrBf16Skip], (unsigned long long)skipped * nreal * kRowTile);
 }
 
 // The survivor records every filter wave wrote (one region slice each, one
@@ -772,7 +806,8 @@ void launch_bf16_filter(hipStream_t st, const DevIndex &dv, uint32_t nq, const u
                         const uint32_t *bound, const uint32_t *qperm, const uint32_t *tiles,
                         const uint32_t *ntiles_g, bool shared_list, uint32_t scap, uint64_t *surv, uint32_t *scnt,
                         const float *thr0, uint32_t *hist, uint4 *region, uint32_t *region_cnt,
-                        uint32_t *prog, uint32_t qt)
+                        uint32_t *prog, uint32_t qt, const uint32_t *slot_need, uint32_t slot_words,
+                        unsigned long long *counters)
 {
   if (hist || !bf16_query_tiles_ok(dv.L, qt)) qt = 2;
   const uint32_t stride = shared_list ? 0 : dv.ntiles;
@@ -783,7 +818,7 @@ void launch_bf16_filter(hipStream_t st, const DevIndex &dv, uint32_t nq, const u
 #define HKG_BF16_LAUNCH(KS, HIST, QT)                                                                             \
   hipLaunchKernelGGL((k_bf16_filter<KS, HIST, QT>), grid, dim3(filter_waves(KS) * 64), 0, st, dv.n, nq, dv.Xb,   \
                      dv.xn2, dv.xst, dv.tile_slack, Qb, qst, bound, qperm, tiles, stride, ntiles_g, scap, surv,   \
-                     scnt, thr0, hist, region, kRegionCap, region_cnt, prog)
+                     scnt, thr0, hist, region, kRegionCap, region_cnt, prog, slot_need, slot_words, counters)
   // the histogram pass (rare: the bound pass) runs in the default shape only
 #define HKG_BF16_LAUNCH_QT(KS)                   \
   if (hist)                                      \
@@ -970,6 +1005,33 @@ hkg_status Index::make_bf16()
   return HKG_OK;
 }
 
+// filter wave slots of a batch of nq queries (at the smallest slot, 64
+// queries) and the 32-bit words of their per-tile need bits
+static uint32_t bf16_max_slots(uint32_t nq) { return (nq + 63) / 64; }
+static uint32_t bf16_slot_words(uint32_t nq) { return (bf16_max_slots(nq) + 31) / 32; }
+
+// Per row tile, one bit per filter wave slot (qpw consecutive queries of the
+// batch order): set when a query of the slot needs a leaf the tile overlaps
+// (need8, per slot and leaf, from k_group_leaf_need over the slots)
+__global__ void k_tile_slot_bits(uint32_t ntiles, uint32_t nslots, uint32_t words, uint32_t nleaf,
+                                 const uint32_t *__restrict__ tile_leaf0, const uint32_t *__restrict__ tile_leaf1,
+                                 const uint8_t *__restrict__ need8, uint32_t *__restrict__ bits)
+{
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const uint32_t l0 = tile_leaf0[t], l1 = tile_leaf1[t];
+  for (uint32_t w = 0; w < words; ++w) {
+    uint32_t v = 0;
+    for (uint32_t b = 0; b < 32 && w * 32 + b < nslots; ++b) {
+      const uint8_t *nd = need8 + (uint64_t)(w * 32 + b) * nleaf;
+      bool any = false;
+      for (uint32_t lf = l0; lf <= l1 && !any; ++lf) any = nd[lf] != 0;
+      v |= any ? 1u << b : 0u;
+    }
+    bits[(uint64_t)t * words + w] = v;
+  }
+}
+
 // The BF16 scratch of a buffer set (survivor lists at the call's cap,
 // BF16 queries, block-group tables, survivor region), allocated on first use
 hkg_status Index::ensure_bf16_scratch(QueryBuffers &bs)
@@ -1000,7 +1062,9 @@ hkg_status Index::ensure_bf16_scratch(QueryBuffers &bs)
         hipMalloc((void **)&b->hist, (size_t)b->B * kBf16Bins * 4) != hipSuccess ||
         hipMalloc((void **)&b->bf_region, (size_t)bf16_max_blocks() * kRegionCap * sizeof(uint4)) != hipSuccess ||
         hipMalloc((void **)&b->bf_region_cnt, (size_t)bf16_max_blocks() * 4 * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc((void **)&b->bf_prog, (size_t)bf16_max_blocks() * sizeof(uint32_t)) != hipSuccess) {
+        hipMalloc((void **)&b->bf_prog, (size_t)bf16_max_blocks() * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void **)&b->slot_need8, (size_t)bf16_max_slots(b->B) * nleaf) != hipSuccess ||
+        hipMalloc((void **)&b->slot_need, (size_t)dev.ntiles * bf16_slot_words(b->B) * 4) != hipSuccess) {
       (void)hipGetLastError();
       set_error("no device memory for the BF16 path scratch");
       return HKG_ENOMEM;
@@ -1083,15 +1147,25 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
       launch_fill(st, b->hist, nq * kBf16Bins, 0);
       launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, kSharedList,
                          b->bf_cap, b->bf_surv, b->bf_cnt, thr0, b->hist, b->bf_region, b->bf_region_cnt, b->bf_prog,
-                         qt);
+                         qt, nullptr, 0, nullptr);
       HKG_HIP_TRY(hipMemcpyAsync(b->misc + 4, b->ntiles_g, lists * 4, hipMemcpyDeviceToDevice, st));
       hipLaunchKernelGGL(k_bf16_tau, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->kq, b->hist, thr0, b->bound);
       plan_tiles(true);
       if (p) p->mark(st, HKG_STAGE_BOUND);
     }
+    // the filter wave slots (32 qt consecutive queries of the batch order)
+    // that need each row tile: a wave whose queries need none of the
+    // tile's leaves skips its MFMAs (those series lie beyond every one of
+    // its queries' k-th distance bound)
+    const uint32_t qpw = 32 * qt, nslots = (nq + qpw - 1) / qpw, swords = (nslots + 31) / 32;
+    hipLaunchKernelGGL(k_group_leaf_need, dim3((nleaf + 255) / 256, nslots), dim3(256), 0, st, nq, qpw, nleaf, b->lb,
+                       b->bound, b->qperm, b->slot_need8, nullptr);
+    hipLaunchKernelGGL(k_tile_slot_bits, dim3((dev.ntiles + 255) / 256), dim3(256), 0, st, dev.ntiles, nslots, swords,
+                       nleaf, dev.tile_leaf0, dev.tile_leaf1, b->slot_need8, b->slot_need);
     // pass 2: collect the survivors of the certified lower bound
     launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, kSharedList, b->bf_cap,
-                       b->bf_surv, b->bf_cnt, thr0, nullptr, b->bf_region, b->bf_region_cnt, b->bf_prog, qt);
+                       b->bf_surv, b->bf_cnt, thr0, nullptr, b->bf_region, b->bf_region_cnt, b->bf_prog, qt,
+                       b->slot_need, swords, b->counters);
     if (p) p->mark(st, HKG_STAGE_REFINE);
     launch_bf16_refine(st, nq, dev.X, L, b->Q, b->kq, b->kmax, dev.ids, b->cap, b->cand, b->cand_cnt, b->bf_cap,
                        b->bf_surv, b->bf_cnt, thr0, b->bound, idmap64, b->out_ids, b->out_d2, overflow, b->counters);

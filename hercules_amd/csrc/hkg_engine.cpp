@@ -360,6 +360,8 @@ void QueryBuffers::free_bf16()
   dfree(bf_region);
   dfree(bf_region_cnt);
   dfree(bf_prog);
+  dfree(slot_need8);
+  dfree(slot_need);
 }
 
 QueryBuffers::~QueryBuffers()
@@ -1110,8 +1112,11 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
         scan ? 0
              : (path == HKG_PATH_BF16 ? total.v[kCtrLeafPairs] : total.nwork + total.v[kCtrApproxLeaves]);
     stats->series_total = (uint64_t)nq * dev.n;
-    stats->series_lb_checked = scan ? 0 : total.v[kCtrSaxChecked] + total.v[kCtrBf16];
-    stats->bf16_pairs = total.v[kCtrBf16];
+    // BF16: the pairs of the row tiles the filter walked, less those of the
+    // tiles a wave skipped (no query of it needs them)
+    const uint64_t bf16_seen = total.v[kCtrBf16] - std::min(total.v[kCtrBf16], total.v[kCtrBf16Skip]);
+    stats->series_lb_checked = scan ? 0 : total.v[kCtrSaxChecked] + bf16_seen;
+    stats->bf16_pairs = bf16_seen;
     stats->series_refined = scan ? (uint64_t)nq * dev.n : total.v[kCtrRefined];
     stats->device_ms = ms;
     stats->overflow_retries = retries;

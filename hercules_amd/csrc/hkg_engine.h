@@ -118,7 +118,9 @@ struct QueryBuffers {
   uint32_t *ntiles_g = nullptr; // groups
   uint32_t *thr = nullptr;      // B: filter threshold (float bits)
   uint32_t *hist = nullptr;     // B x kBf16Bins: counts of series by BF16 upper bound
-  uint4 *bf_region = nullptr;   // per filter block: pre-test passes {row, xn2 - 2 s bits, query, 0}
+  uint8_t *slot_need8 = nullptr;  // wave slots x nleaf: leaf needed by a filter wave's queries
+  uint32_t *slot_need = nullptr;  // ntiles x words: per row tile, one bit per wave slot that needs it
+  uint4 *bf_region = nullptr;   // per filter block: survivor records (header + 16 values xn2 - 2 s)
   uint32_t *bf_region_cnt = nullptr;  // per filter wave: pre-test passes in its region slice
   uint32_t *bf_prog = nullptr;  // per filter block: tiles done (sibling blocks keep pace)
   void free_bf16();

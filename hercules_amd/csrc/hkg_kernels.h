@@ -39,7 +39,8 @@ enum Counter : int {
   kCtrApproxLeaves = 9,  // (query, leaf) pairs scanned in the approximate stage
   kCtrLeafPairs = 10,  // BF16 path: (query, leaf) pairs within the k-th bound after the approximate stage
   kCtrSaxWords = 11,   // batched iSAX path: iSAX words streamed (once per leaf per wave chunk)
-  kNumCounters = 12
+  kCtrBf16Skip = 12,   // BF16 path: (query, series) pairs of tiles a filter wave skipped (no query needs them)
+  kNumCounters = 13
 };
 
 // parameters of the approximate modes (hkg_query_opts)
