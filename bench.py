@@ -510,7 +510,7 @@ def main() -> int:
     stage_bytes = {}
     stage_launch = {}
     step_ms = []
-    refined = checked = 0
+    refined = checked = pairs = 0
     t = time.perf_counter()
     for s in range(warm, warm + steps):
         ts = time.perf_counter()
@@ -522,6 +522,7 @@ def main() -> int:
             stage_launch[kk] = stage_launch.get(kk, 0) + st["stage_launches"][kk]
         refined += st["series_refined"]
         checked += st["series_lb_checked"]
+        pairs += st.get("bf16_pairs", 0)
         log(f"step {s}: {step_ms[-1]:.1f} ms, refined {st['series_refined']}, "
             f"lb-checked {st['series_lb_checked']}, leaves {st['leaves_after_eapca']}")
     hkg.lib().hkg_device_sync()
@@ -548,6 +549,10 @@ def main() -> int:
         # the BF16 filter is a contraction: 2 L N B FLOPs per batch pass
         flops_per_launch = 2.0 * L * N * B
         achieved_tf = flops_per_launch / (avg_ms * 1e-3) / 1e12
+        # the MFMAs actually issued: waves skip the row tiles none of their
+        # queries needs, so the executed pairs fall below N B
+        exec_flops = 2.0 * L * pairs / launches
+        exec_tf = exec_flops / (avg_ms * 1e-3) / 1e12
     kernel_names = {"refine": "k_filter_refine", "approx": "k_scan_best", "eapca": "k_eapca", "scan": "k_scan",
                     "topk": "k_select", "collect": "k_collect", "prep": "k_prep", "select_leaf": "k_best_leaf"}
     if args.path == "bf16":
@@ -712,7 +717,9 @@ def main() -> int:
         "roofline": ({"bound": "mfma", "kernel": kernel_names.get(dom, dom), "achieved": achieved_tf,
                       "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved_tf / MFMA_BF16_PEAK_TFLOPS,
                       "traffic": tb, "traffic_gbs": traffic_gbs, "avg_launch_ms": avg_ms,
-                      "flops_per_launch": flops_per_launch, "tile_bytes_per_launch": bytes_per_launch,
+                      "flops_per_launch": flops_per_launch, "executed_flops_per_launch": exec_flops,
+                      "executed_tflops": exec_tf, "executed_frac": exec_tf / MFMA_BF16_PEAK_TFLOPS,
+                      "tile_bytes_per_launch": bytes_per_launch,
                       "tile_bytes_gbs": achieved}
                      if mfma else
                      {"bound": "hbm", "kernel": kernel_names.get(dom, dom), "achieved": achieved,
