@@ -71,7 +71,7 @@ def parse() -> argparse.Namespace:
     p.add_argument("--serve-loads", default="0.5,0.8,0.95",
                    help="serving sweep: offered loads as fractions of the measured throughput")
     p.add_argument("--serve-inflight", type=int, default=2, help="serving: batches on the device at once")
-    p.add_argument("--serve-requests", type=int, default=4096, help="requests of the serving leg (0: skip)")
+    p.add_argument("--serve-requests", type=int, default=16384, help="requests of the serving leg (0: skip)")
     p.add_argument("--mode-epsilon", type=float, default=1.0,
                    help="epsilon of the approximate-mode leg (0: skip that leg)")
     p.add_argument("--mode-nprobes", type=int, default=10, help="nprobes of the ng-mode leg (0: skip that leg)")
@@ -273,7 +273,11 @@ def traffic_bytes(kernel: str, n: int, L: int, batch: int):
 def serving_leg(hkg, ix, args, Qh: np.ndarray, kq: np.ndarray, qps: float) -> dict:
     """Poisson stream of single-query requests through the continuous-
     batching server; latency = submit -> result ready, as the server reports
-    it.  A sample of the served answers is compared with direct queries."""
+    it.  A sample of the served answers is compared with direct queries.
+    Before the stream the new server is warmed, untimed: two full batches at
+    once (every worker runs one) and one request of each k class (their
+    buffer sets), as a server takes some traffic before it is put in
+    service."""
     import threading
 
     nreq = args.serve_requests
@@ -283,6 +287,15 @@ def serving_leg(hkg, ix, args, Qh: np.ndarray, kq: np.ndarray, qps: float) -> di
     kmax = int(kq.max())
     srv = hkg.Server(ix, max_batch=args.batch, path=args.path, max_inflight=args.serve_inflight, max_k=kmax,
                      bf16_bound_pass=args.bf16_bound_pass, scratch_bytes=32 << 30)
+    t_warm = time.perf_counter()
+    warm = [(i % Qh.shape[0], kmax) for i in range(args.serve_inflight * args.batch)]
+    warm_cls = sorted({min(1 << i, kmax) for i in range(kmax.bit_length() + 1)})
+    tw =[srv.submit(Qh[q], k) for q, k in warm]
+    for t_, (q, k) in zip(tw, warm):
+        srv.wait(t_, k)
+    for c in warm_cls:
+        srv.wait(srv.submit(Qh[0], c), c)
+    warm_s = time.perf_counter() - t_warm
     tickets = [None] * nreq
     ready = threading.Semaphore(0)
 
@@ -321,6 +334,7 @@ def serving_leg(hkg, ix, args, Qh: np.ndarray, kq: np.ndarray, qps: float) -> di
             "p50_ms": round(float(np.percentile(lat, 50)) / 1e3, 3),
             "p99_ms": round(float(np.percentile(lat, 99)) / 1e3, 3),
             "max_batch": args.batch, "inflight": args.serve_inflight, "path": args.path, "scratch_gib": 32,
+            "warm_up_s": round(warm_s, 3), "warm_up_requests": len(warm) + len(warm_cls),
             "answers_identical_to_direct_queries": same, "sampled": int(len(sample))}
 
 

@@ -50,6 +50,7 @@ constexpr int kRowTile = 96;               // series per LDS tile
 constexpr int kBuf = 3;                    // LDS tile buffers (tiles loaded two ahead)
 constexpr uint32_t kRegionCap = 131072;    // 16-byte units per filter block for survivor records (2 MB)
 constexpr uint32_t kRecU4 = 5;             // 16-byte units per survivor record
+constexpr uint32_t kScatterBlocks = 4;     // k_bf16_scatter blocks per region slice
 constexpr float kEvalSlack = 1.0e-3f;      // FP32 evaluation of the bound
 constexpr uint32_t kBf16SurvivorCap = 524288;  // survivors kept per query (8 B each)
 constexpr float kHistLo = 0.6f;            // histogram range: [0.6 bound, threshold]
@@ -719,7 +720,8 @@ __global__ void __launch_bounds__(256) k_bf16_scatter(const uint4 *__restrict__ 
   const uint32_t sl = blockIdx.x;
   const uint32_t cnt = min(region_cnt[sl], slice_cap / kRecU4);
   const uint4 *reg = region + (uint64_t)sl * slice_cap;
-  for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+  // grid.y blocks share a slice (the slices' record counts are uneven)
+  for (uint32_t i = blockIdx.y * blockDim.x + threadIdx.x; i < cnt; i += blockDim.x * gridDim.y) {
     const uint4 *rec = reg + (uint64_t)i * kRecU4;
     const uint4 h = rec[0];
     const uint32_t q = h.y;
@@ -785,8 +787,10 @@ uint32_t bf16_group_queries(uint32_t L, uint32_t qt) { return bf16_qpb(L, qt); }
 
 uint64_t bf16_scratch_per_query(uint32_t L, uint64_t nleaf, uint64_t ntiles)
 {
+  // ... + the block groups' leaf needs and tile lists, the 32-query slots'
+  // leaf needs and the wave slots' tile bits
   return (uint64_t)kBf16SurvivorCap * 8 + 4 + (uint64_t)L * 2 + 16 + 16 + 4 + 4ull * kBf16Bins +
-         (nleaf + 4 * ntiles) / bf16_qpb(L, kBf16QueryTiles) + 16;
+         (nleaf + 4 * ntiles) / bf16_qpb(L, kBf16QueryTiles) + nleaf / 32 + ntiles / 16 + 18;
 }
 
 // one resident block per CU, split over the block groups; a multiple of 8
@@ -838,7 +842,7 @@ void launch_bf16_filter(hipStream_t st, const DevIndex &dv, uint32_t nq, const u
 #undef HKG_BF16_LAUNCH
   if (!hist) {
     const uint32_t w = (uint32_t)filter_waves((int)dv.L / 16);
-    hipLaunchKernelGGL(k_bf16_scatter, dim3(grid.x * grid.y * w), dim3(256), 0, st, region, kRegionCap / w,
+    hipLaunchKernelGGL(k_bf16_scatter, dim3(grid.x * grid.y * w, kScatterBlocks), dim3(256), 0, st, region, kRegionCap / w,
                        region_cnt, dv.n, dv.xst, qst, thr0, scap, surv, scnt);
   }
 }
@@ -1007,15 +1011,29 @@ hkg_status Index::make_bf16()
 
 // filter wave slots of a batch of nq queries (at the smallest slot, 64
 // queries) and the 32-bit words of their per-tile need bits
-static uint32_t bf16_max_slots(uint32_t nq) { return (nq + 63) / 64; }
-static uint32_t bf16_slot_words(uint32_t nq) { return (bf16_max_slots(nq) + 31) / 32; }
+// 32-query slots of a batch (rows of the per-slot leaf needs), and 32-bit
+// words of filter wave slots (at least 64 queries each)
+static uint32_t bf16_max_slots(uint32_t nq) { return (nq + 31) / 32; }
+static uint32_t bf16_slot_words(uint32_t nq) { return ((nq + 63) / 64 + 31) / 32; }
 
-// Per row tile, one bit per filter wave slot (qpw consecutive queries of the
-// batch order): set when a query of the slot needs a leaf the tile overlaps
-// (need8, per slot and leaf, from k_group_leaf_need over the slots)
-__global__ void k_tile_slot_bits(uint32_t ntiles, uint32_t nslots, uint32_t words, uint32_t nleaf,
-                                 const uint32_t *__restrict__ tile_leaf0, const uint32_t *__restrict__ tile_leaf1,
-                                 const uint8_t *__restrict__ need8, uint32_t *__restrict__ bits)
+// leaves some query of the batch needs: OR over the 32-query slots
+__global__ void k_leaf_need_any(uint32_t nleaf, uint32_t nslot32, const uint8_t *__restrict__ need8,
+                                uint8_t *__restrict__ need)
+{
+  const uint32_t lf = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lf >= nleaf) return;
+  uint8_t any = 0;
+  for (uint32_t s = 0; s < nslot32 && !any; ++s) any = need8[(uint64_t)s * nleaf + lf];
+  need[lf] = any ? 1 : 0;
+}
+
+// Per row tile, one bit per filter wave slot (sub consecutive 32-query
+// slots of the batch order): set when a query of the slot needs a leaf the
+// tile overlaps (need8, per 32-query slot and leaf, from k_group_leaf_need)
+__global__ void k_tile_slot_bits(uint32_t ntiles, uint32_t nslots, uint32_t words, uint32_t nleaf, uint32_t sub,
+                                 uint32_t nslot32, const uint32_t *__restrict__ tile_leaf0,
+                                 const uint32_t *__restrict__ tile_leaf1, const uint8_t *__restrict__ need8,
+                                 uint32_t *__restrict__ bits)
 {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
@@ -1023,9 +1041,12 @@ __global__ void k_tile_slot_bits(uint32_t ntiles, uint32_t nslots, uint32_t word
   for (uint32_t w = 0; w < words; ++w) {
     uint32_t v = 0;
     for (uint32_t b = 0; b < 32 && w * 32 + b < nslots; ++b) {
-      const uint8_t *nd = need8 + (uint64_t)(w * 32 + b) * nleaf;
+      const uint32_t s0 = (w * 32 + b) * sub, s1 = min(s0 + sub, nslot32);
       bool any = false;
-      for (uint32_t lf = l0; lf <= l1 && !any; ++lf) any = nd[lf] != 0;
+      for (uint32_t s = s0; s < s1 && !any; ++s) {
+        const uint8_t *nd = need8 + (uint64_t)s * nleaf;
+        for (uint32_t lf = l0; lf <= l1 && !any; ++lf) any = nd[lf] != 0;
+      }
       v |= any ? 1u << b : 0u;
     }
     bits[(uint64_t)t * words + w] = v;
@@ -1091,20 +1112,17 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
   if (rc != HKG_OK) return rc;
   BufLease lease{this, b};  // handed to the caller on success
   const uint32_t qt = bf16_query_tiles_ok(L, call_settings().bf16_qt) ? call_settings().bf16_qt : kBf16QueryTiles;
-  const uint32_t qpb = bf16_qpb(L, qt);
   rc = ensure_bf16_scratch(*b);
   if (rc != HKG_OK) return rc;
   HKG_HIP_TRY(hipMemcpyAsync(b->Q, dQ, (size_t)nq * L * 4, hipMemcpyDeviceToDevice, st));
   HKG_HIP_TRY(hipMemcpyAsync(b->kq, dk, (size_t)nq * 4, hipMemcpyDeviceToDevice, st));
   uint32_t *overflow = b->misc + 2;
   if (seed_host) HKG_HIP_TRY(hipMemcpyAsync(b->seed, seed_host, (size_t)nq * 4, hipMemcpyHostToDevice, st));
-  // batch order by first bound, leaves and row tiles each block group needs
-  const uint32_t groups = (nq + qpb - 1) / qpb;
   // row tiles the batch needs under the current bounds (one list that
   // every block group walks), and the filter threshold of every query
   float *thr0 = reinterpret_cast<float *>(b->thr);
   constexpr bool kSharedList = true;
-  const uint32_t lists = kSharedList ? 1 : groups, list_qpb = kSharedList ? nq : qpb;
+  const uint32_t lists = 1;
   // the first pass's tile count is kept on the device (misc[4]) for the
   // counters read after the batch: one list
   static_assert(kSharedList, "the pass-1 tile count has one slot");
@@ -1130,11 +1148,16 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
     hipLaunchKernelGGL(k_iota_best, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->best, skeys, sidx);
     HKG_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(b->bf_sort_tmp, b->bf_sort_bytes, skeys, b->qperm + 3 * b->B,
                                                    sidx, b->qperm, (int)nq, 0, 32, st));
-    // the last plan also counts the (query, leaf) pairs it keeps
+    // the leaves each 32-query slot of the batch order needs (the last plan
+    // also counts the (query, leaf) pairs it keeps), the leaves any query
+    // needs, and the row tiles overlapping those
+    const uint32_t nslot32 = bf16_max_slots(nq);
     auto plan_tiles = [&](bool count) {
       launch_fill(st, b->ntiles_g, lists, 0);
-      hipLaunchKernelGGL(k_group_leaf_need, dim3((nleaf + 255) / 256, lists), dim3(256), 0, st, nq, list_qpb, nleaf,
-                         b->lb, b->bound, b->qperm, b->need, count ? b->counters + kCtrLeafPairs : nullptr);
+      hipLaunchKernelGGL(k_group_leaf_need, dim3((nleaf + 255) / 256, nslot32), dim3(256), 0, st, nq, 32u, nleaf,
+                         b->lb, b->bound, b->qperm, b->slot_need8, count ? b->counters + kCtrLeafPairs : nullptr);
+      hipLaunchKernelGGL(k_leaf_need_any, dim3((nleaf + 255) / 256), dim3(256), 0, st, nleaf, nslot32, b->slot_need8,
+                         b->need);
       hipLaunchKernelGGL(k_group_tiles, dim3((dev.ntiles + 255) / 256, lists), dim3(256), 0, st, dev.ntiles, nleaf,
                          dev.tile_leaf0, dev.tile_leaf1, b->need, b->tiles, b->ntiles_g);
       hipLaunchKernelGGL(k_bf16_thr_init, dim3((nq + 255) / 256), dim3(256), 0, st, nq, b->bound, thr0);
@@ -1153,15 +1176,13 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
       plan_tiles(true);
       if (p) p->mark(st, HKG_STAGE_BOUND);
     }
-    // the filter wave slots (32 qt consecutive queries of the batch order)
-    // that need each row tile: a wave whose queries need none of the
-    // tile's leaves skips its MFMAs (those series lie beyond every one of
-    // its queries' k-th distance bound)
+    // the filter wave slots (32 qt consecutive queries of the batch order,
+    // qt of the 32-query slots of the last plan) that need each row tile: a
+    // wave whose queries need none of the tile's leaves skips its MFMAs
+    // (those series lie beyond every one of its queries' k-th distance bound)
     const uint32_t qpw = 32 * qt, nslots = (nq + qpw - 1) / qpw, swords = (nslots + 31) / 32;
-    hipLaunchKernelGGL(k_group_leaf_need, dim3((nleaf + 255) / 256, nslots), dim3(256), 0, st, nq, qpw, nleaf, b->lb,
-                       b->bound, b->qperm, b->slot_need8, nullptr);
     hipLaunchKernelGGL(k_tile_slot_bits, dim3((dev.ntiles + 255) / 256), dim3(256), 0, st, dev.ntiles, nslots, swords,
-                       nleaf, dev.tile_leaf0, dev.tile_leaf1, b->slot_need8, b->slot_need);
+                       nleaf, qt, nslot32, dev.tile_leaf0, dev.tile_leaf1, b->slot_need8, b->slot_need);
     // pass 2: collect the survivors of the certified lower bound
     launch_bf16_filter(st, dev, nq, b->Qb, b->qst, b->bound, b->qperm, b->tiles, b->ntiles_g, kSharedList, b->bf_cap,
                        b->bf_surv, b->bf_cnt, thr0, nullptr, b->bf_region, b->bf_region_cnt, b->bf_prog, qt,
@@ -1192,14 +1213,11 @@ hkg_status Index::run_bf16_batch(const float *dQ, const uint32_t *dk, uint32_t n
     HKG_HIP_TRY(hipStreamSynchronize(st));
     // (query, series) pairs and row tiles the filter passes went through
     // (rows counted once per block group that streamed them)
-    for (uint32_t g = 0; g < groups; ++g) {
-      const uint32_t gl = lists == 1 ? 0 : g;
-      uint64_t qn = std::min<uint32_t>(qpb, nq - g * qpb);
-      const uint64_t t1 = nt_pass1.empty() ? 0 : nt_pass1[gl];
-      ctr->v[kCtrBf16] += (unsigned long long)(nt[gl] + t1) * kRowTile * qn;
-      ctr->bf16_rows += (uint64_t)nt[gl] * kRowTile;
-      ctr->bf16_rows_bound += t1 * kRowTile;
-    }
+    const uint64_t groups = (nq + bf16_qpb(L, qt) - 1) / bf16_qpb(L, qt);
+    const uint64_t t1 = nt_pass1.empty() ? 0 : nt_pass1[0];
+    ctr->v[kCtrBf16] += (unsigned long long)(nt[0] + t1) * kRowTile * nq;
+    ctr->bf16_rows += (uint64_t)nt[0] * kRowTile * groups;
+    ctr->bf16_rows_bound += t1 * kRowTile * groups;
   }
   *used = b;
   lease.b = nullptr;

@@ -397,7 +397,7 @@ QueryBuffers::~QueryBuffers()
 }
 
 // buffer sets kept alive per index (each a batch's scratch: up to a few GB)
-constexpr size_t kMaxBufferSets = 8;
+constexpr size_t kMaxBufferSets = 16;
 
 hkg_status Index::get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuffers **out)
 {
@@ -425,10 +425,12 @@ hkg_status Index::get_buffers(uint32_t B, uint32_t kmax, uint32_t cap, QueryBuff
       if (!e->second->busy && (lru == bufs.end() || e->second->last_use < lru->second->last_use)) lru = e;
     if (lru == bufs.end()) return false;
     bufs.erase(lru);
+    ++sets_evicted;
     return true;
   };
   while (bufs.size() >= kMaxBufferSets && evict_lru()) {
   }
+  ++sets_allocated;
   hkg_status rc = alloc_buffers(B, kmax, cap, out);
   if (rc == HKG_ENOMEM) {
     // device memory held by idle sets: drop them all and try once more
@@ -1083,6 +1085,8 @@ hkg_status Index::knn_device(const float *dQ, uint32_t nq, const uint32_t *dk, u
     if (!dbg_cnt.empty()) {
       HKG_HIP_TRY(hipStreamSynchronize(st));
       print_bf16_summary(b, n, kmax, dbg_cnt, dbg_thr);
+      fprintf(stderr, "hkg:   approximate stage: %llu series in its leaves, %llu ranked\n",
+              (unsigned long long)c.v[kCtrApprox], (unsigned long long)c.v[kCtrApproxRanked]);
     }
     HKG_HIP_TRY(hipMemcpyAsync(dids + (uint64_t)q0 * kmax, b->out_ids, (size_t)n * kmax * 8,
                                hipMemcpyDeviceToDevice, st));

@@ -162,6 +162,7 @@ struct Index {
   // key: (queries, kmax, candidate entries per query)
   std::multimap<std::tuple<uint32_t, uint32_t, uint32_t>, std::unique_ptr<QueryBuffers>> bufs;
   uint64_t buf_clock = 0;  // lease counter for the least-recently-used eviction (guarded by mu_bufs)
+  uint64_t sets_allocated = 0, sets_evicted = 0;  // buffer sets made and dropped (guarded by mu_bufs)
   // launch shapes chosen by autotune (0: the defaults): queries per batch
   // of each path, and k_filter_refine blocks per CU
   std::atomic<uint32_t> tuned_batch[HKG_PATH_BF16 + 1] = {};

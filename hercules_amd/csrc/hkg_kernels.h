@@ -40,7 +40,8 @@ enum Counter : int {
   kCtrLeafPairs = 10,  // BF16 path: (query, leaf) pairs within the k-th bound after the approximate stage
   kCtrSaxWords = 11,   // batched iSAX path: iSAX words streamed (once per leaf per wave chunk)
   kCtrBf16Skip = 12,   // BF16 path: (query, series) pairs of tiles a filter wave skipped (no query needs them)
-  kNumCounters = 13
+  kCtrApproxRanked = 13,  // BF16 path: series of the approximate stage's leaves ranked (within the iSAX bound)
+  kNumCounters = 14
 };
 
 // parameters of the approximate modes (hkg_query_opts)

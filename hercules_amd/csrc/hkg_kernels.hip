@@ -553,6 +553,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_best_bf16(const float *__restri
     if ((threadIdx.x & 63) == 0 && kept) atomicAdd(&s_kept, kept);
   }
   __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&counters[kCtrApproxRanked], (unsigned long long)(PRUNE ? s_kept : size));
   if (PRUNE && s_kept == 0) {
     // every row of the leaf is beyond the bound: nothing to refine
     if (threadIdx.x == 0) {

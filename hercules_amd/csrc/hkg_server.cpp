@@ -17,6 +17,8 @@
 // The reference answers a query file in a loop, one query at a time
 // (hercules_file_buffer_manager / query answering in hercules.c); this
 // loop has no single counterpart there.
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -106,7 +108,9 @@ struct Slot {
 
 void Server::worker(int wi)
 {
-  (void)wi;
+  // HKG_SERVER_TRACE: one line per batch on stderr (size, k class, time,
+  // overflow re-runs, buffer sets made and dropped)
+  const bool trace = getenv("HKG_SERVER_TRACE") != nullptr;
   const uint32_t L = ix->dev.L, B = opt.max_batch, K = opt.max_k;
   if (hipSetDevice(ix->dev.device) != hipSuccess) return;
   Slot s;
@@ -158,6 +162,12 @@ void Server::worker(int wi)
     // requests on the device are not touched by the client (their state
     // is 1), and unordered_map never moves its elements
     const uint32_t n = (uint32_t)batch.size();
+    // a batch that fits the share at the largest class runs at it: the sets
+    // the server prepared at start then serve every such batch (a set per
+    // class would be allocated, and under memory pressure dropped and
+    // re-allocated, by the batches of each class).  Only a batch that the
+    // share admits at its own class alone runs at that class.
+    if ((uint64_t)n * per_query(K) <= batch_scratch) kc = K;
     // buffer sets sized for what the share admits at this class (batches of
     // fewer queries reuse them)
     call_settings().min_buffer_batch =
@@ -173,7 +183,15 @@ void Server::worker(int wi)
       rc = HKG_EDEVICE;
     }
     // results at the class's stride (kc ranks per query)
-    if (rc == HKG_OK) rc = ix->knn_device(s.dQ, n, s.dk, kc, &qo, s.dids, s.dd2, s.st, nullptr);
+    const Clock::time_point t_run = Clock::now();
+    uint64_t sets0 = 0, evict0 = 0;
+    hkg_query_stats stt{};
+    if (trace) {
+      std::lock_guard<std::mutex> lock(ix->mu_bufs);
+      sets0 = ix->sets_allocated;
+      evict0 = ix->sets_evicted;
+    }
+    if (rc == HKG_OK) rc = ix->knn_device(s.dQ, n, s.dk, kc, &qo, s.dids, s.dd2, s.st, trace ? &stt : nullptr);
     if (rc == HKG_OK &&
         (hipMemcpyAsync(s.hids, s.dids, (size_t)n * kc * 8, hipMemcpyDeviceToHost, s.st) != hipSuccess ||
          hipMemcpyAsync(s.hd2, s.dd2, (size_t)n * kc * 4, hipMemcpyDeviceToHost, s.st) != hipSuccess ||
@@ -181,6 +199,17 @@ void Server::worker(int wi)
       rc = HKG_EDEVICE;
     }
     const Clock::time_point now = Clock::now();
+    if (trace) {
+      uint64_t sets1, evict1;
+      {
+        std::lock_guard<std::mutex> lock(ix->mu_bufs);
+        sets1 = ix->sets_allocated;
+        evict1 = ix->sets_evicted;
+      }
+      fprintf(stderr, "hkg server: worker %d batch %u k class %u: %.2f ms (device %.2f ms), %u overflow re-runs, %llu buffer sets made, %llu dropped\n",
+              wi, n, kc, std::chrono::duration<double, std::milli>(now - t_run).count(), stt.device_ms,
+              stt.overflow_retries, (unsigned long long)(sets1 - sets0), (unsigned long long)(evict1 - evict0));
+    }
     std::lock_guard<std::mutex> lock(mu);
     for (uint32_t i = 0; i < n; ++i) {
       Request &r = *batch[i];

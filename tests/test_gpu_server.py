@@ -140,7 +140,7 @@ def test_buffer_sets_evicted_and_rebuilt_keep_answers(hkg, setup, path):
     oracle's answers."""
     ix, Q, kv, ids, d2 = setup
     _say(f"buffer-set eviction path={path}")
-    shapes = [(b, k) for b in (16, 48) for k in (1, 2, 5, 10, 37, 64)]  # 12 shapes > 8 kept
+    shapes = [(b, k) for b in (16, 48, 80) for k in (1, 2, 5, 10, 37, 64)]  # 18 shapes > 16 kept
     for rep in range(2):
         for b, k in shapes:
             r = ix.knn(Q[:b], k, path=path, batch=b)
