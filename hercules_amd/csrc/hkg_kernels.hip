@@ -297,7 +297,12 @@ __global__ void __launch_bounds__(64) k_prep(const float *__restrict__ Q, uint32
 // LB_EAPCA (hercules_node.c:879-924) for every (query, leaf): the query's
 // per-segment mean and sd come from FP64 prefix sums instead of the
 // reference's FP32 sums; the bound is used only with the certified slack.
-__global__ void __launch_bounds__(kBlock) k_eapca(uint32_t nleaf, uint32_t L,
+// One leaf per thread, kEapcaQueries queries per block (grid.y: the query
+// groups): a leaf's segment ends and ranges are read once for the group's
+// queries, whose prefix sums sit in LDS (global memory when they do not
+// fit); per query the arithmetic is the per-segment formula as it stands.
+constexpr uint32_t kEapcaQueries = 4;
+__global__ void __launch_bounds__(kBlock) k_eapca(uint32_t nleaf, uint32_t L, uint32_t nq,
                                                   const uint32_t *__restrict__ seg_off,
                                                   const int16_t *__restrict__ seg_end,
                                                   const float4 *__restrict__ ind,
@@ -305,40 +310,51 @@ __global__ void __launch_bounds__(kBlock) k_eapca(uint32_t nleaf, uint32_t L,
                                                   float *__restrict__ lb)
 {
   extern __shared__ double sp[];
-  const uint32_t q = blockIdx.y;
-  const double *gp = psum + (uint64_t)q * 2 * (L + 1);
-  const bool use_lds = 2 * (L + 1) * sizeof(double) <= kEapcaLdsBytes;
+  const uint32_t q0 = blockIdx.y * kEapcaQueries;
+  const uint32_t nqb = min(kEapcaQueries, nq - q0);
+  const uint32_t stride = 2 * (L + 1);
+  const double *gp = psum + (uint64_t)q0 * stride;
+  const bool use_lds = kEapcaQueries * stride * sizeof(double) <= kEapcaLdsBytes;
   if (use_lds) {
-    for (uint32_t i = threadIdx.x; i < 2 * (L + 1); i += kBlock) sp[i] = gp[i];
+    for (uint32_t i = threadIdx.x; i < nqb * stride; i += kBlock) sp[i] = gp[i];
     __syncthreads();
   }
-  const double *p1 = use_lds ? sp : gp;
-  const double *p2 = p1 + (L + 1);
+  const double *pb = use_lds ? sp : gp;
   uint32_t leaf = blockIdx.x * kBlock + threadIdx.x;
   if (leaf >= nleaf) return;
   uint32_t b = seg_off[leaf], e = seg_off[leaf + 1];
-  double sum = 0.0;
+  double sum[kEapcaQueries];
+#pragma unroll
+  for (uint32_t j = 0; j < kEapcaQueries; ++j) sum[j] = 0.0;
   int start = 0;
   for (uint32_t s = b; s < e; ++s) {
     int end = seg_end[s];
     double n = (double)(end - start);
-    double mu = (p1[end] - p1[start]) / n;
-    double var = (p2[end] - p2[start]) / n - mu * mu;
-    double sd = var > 0.0 ? sqrt(var) : 0.0;
     float4 g = ind[s];  // {max mean, min mean, max sd, min sd}
-    double t = 0.0;
-    if ((sd - g.z) * (sd - g.w) > 0.0) {
-      double u = fmin(fabs(sd - (double)g.z), fabs(sd - (double)g.w));
-      t += u * u;
+#pragma unroll
+    for (uint32_t j = 0; j < kEapcaQueries; ++j) {
+      if (j >= nqb) break;
+      const double *p1 = pb + j * stride;
+      const double *p2 = p1 + (L + 1);
+      double mu = (p1[end] - p1[start]) / n;
+      double var = (p2[end] - p2[start]) / n - mu * mu;
+      double sd = var > 0.0 ? sqrt(var) : 0.0;
+      double t = 0.0;
+      if ((sd - g.z) * (sd - g.w) > 0.0) {
+        double u = fmin(fabs(sd - (double)g.z), fabs(sd - (double)g.w));
+        t += u * u;
+      }
+      if ((mu - g.x) * (mu - g.y) > 0.0) {
+        double u = fmin(fabs(mu - (double)g.x), fabs(mu - (double)g.y));
+        t += u * u;
+      }
+      sum[j] += t * n;
     }
-    if ((mu - g.x) * (mu - g.y) > 0.0) {
-      double u = fmin(fabs(mu - (double)g.x), fabs(mu - (double)g.y));
-      t += u * u;
-    }
-    sum += t * n;
     start = end;
   }
-  lb[(uint64_t)q * nleaf + leaf] = (float)sum;
+#pragma unroll
+  for (uint32_t j = 0; j < kEapcaQueries; ++j)
+    if (j < nqb) lb[(uint64_t)(q0 + j) * nleaf + leaf] = (float)sum[j];
 }
 
 // per query: the kBestLeaves leaves with the smallest bounds (ties: smaller
@@ -1115,10 +1131,10 @@ void launch_prep(hipStream_t st, const float *Q, uint32_t nq, uint32_t L, float 
 void launch_eapca(hipStream_t st, uint32_t nq, uint32_t nleaf, uint32_t L, const uint32_t *seg_off,
                   const int16_t *seg_end, const float4 *ind, const double *psum, float *lb)
 {
-  size_t lds = 2 * (L + 1) * sizeof(double);
+  size_t lds = kEapcaQueries * 2 * (L + 1) * sizeof(double);
   if (lds > kEapcaLdsBytes) lds = 0;
-  hipLaunchKernelGGL(k_eapca, dim3((nleaf + kBlock - 1) / kBlock, nq), dim3(kBlock), lds, st, nleaf, L,
-                     seg_off, seg_end, ind, psum, lb);
+  hipLaunchKernelGGL(k_eapca, dim3((nleaf + kBlock - 1) / kBlock, (nq + kEapcaQueries - 1) / kEapcaQueries),
+                     dim3(kBlock), lds, st, nleaf, L, nq, seg_off, seg_end, ind, psum, lb);
 }
 
 void launch_best_leaf(hipStream_t st, uint32_t nq, uint32_t nleaf, const float *lb, uint32_t *best)
