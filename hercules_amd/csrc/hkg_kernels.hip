@@ -301,7 +301,7 @@ __global__ void __launch_bounds__(64) k_prep(const float *__restrict__ Q, uint32
 // groups): a leaf's segment ends and ranges are read once for the group's
 // queries, whose prefix sums sit in LDS (global memory when they do not
 // fit); per query the arithmetic is the per-segment formula as it stands.
-constexpr uint32_t kEapcaQueries = 4;
+constexpr uint32_t kEapcaQueries = 8;
 __global__ void __launch_bounds__(kBlock) k_eapca(uint32_t nleaf, uint32_t L, uint32_t nq,
                                                   const uint32_t *__restrict__ seg_off,
                                                   const int16_t *__restrict__ seg_end,
@@ -595,15 +595,34 @@ __global__ void __launch_bounds__(kBlock) k_scan_best_bf16(const float *__restri
       if ((kk & hi) == prefix) atomicAdd(&hist[(kk >> shift) & 255u], 1u);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t cum = 0, d = 0;
-      const uint32_t r = s_rank;
-      for (; d < 255; ++d) {
-        if (cum + hist[d] >= r) break;
-        cum += hist[d];
+    if (threadIdx.x < 64) {
+      // the first digit d (< 255) whose cumulative count reaches the rank:
+      // lane l holds digits 4l..4l+3, a scan over the lanes finds the lane
+      const uint32_t l = threadIdx.x, r = s_rank;
+      const uint32_t h0 = hist[4 * l], h1 = hist[4 * l + 1], h2 = hist[4 * l + 2], h3 = hist[4 * l + 3];
+      const uint32_t tot = h0 + h1 + h2 + h3;
+      uint32_t inc = tot;
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(inc, off);
+        if (l >= (uint32_t)off) inc += v;
       }
-      s_prefix = prefix | (d << shift);
-      s_rank = r - cum;
+      const uint64_t m = __ballot(inc >= r);
+      const uint32_t at = m ? (uint32_t)(__ffsll((long long)m) - 1) : 63u;
+      if (l == at) {
+        uint32_t cum = inc - tot, d;
+        if (cum + h0 >= r) {
+          d = 4 * l;
+        } else if (cum += h0, cum + h1 >= r) {
+          d = 4 * l + 1;
+        } else if (cum += h1, cum + h2 >= r) {
+          d = 4 * l + 2;
+        } else {
+          cum += h2;
+          d = 4 * l + 3;
+        }
+        s_prefix = prefix | (d << shift);
+        s_rank = r - cum;
+      }
     }
     __syncthreads();
   }
