@@ -192,7 +192,8 @@ hkg_status hkg_open_dir(const char *index_dir, const hkg_options *opt, hkg_index
 hkg_status hkg_open_raw(const float *X, uint64_t n, uint32_t L, const hkg_options *opt,
                         hkg_index **out);
 
-/* Attach an id map (leaf-order row -> public id, n entries, each < 2^32);
+/* Attach an id map (leaf-order row -> public id, n distinct 64-bit
+   entries; the device ranks them, so ties in d2 still break by public id);
    without one the public id of a series is its leaf-order position
    (file_pos + offset in leaf, the reference's "Pos"). */
 hkg_status hkg_set_ids(hkg_index *ix, const uint64_t *ids, uint64_t n);

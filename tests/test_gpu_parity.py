@@ -221,6 +221,24 @@ def test_bf16_path_matches_oracle(hkg, oracle, small_data, ref_index, k, bound_p
         assert Q.shape[0] < st["leaves_after_eapca"] <= Q.shape[0] * nleaf
 
 
+@pytest.mark.parametrize("path", ["scan", "index", "batched", "bf16"])
+def test_ids_beyond_32_bits(hkg, oracle, small_data, ref_index, path):
+    """Public ids of 2^32 and above (the reference's positions are 64-bit):
+    an id map of original ordinal * 2^33 + 7 (the same order) returns the
+    oracle's ids mapped the same way, with the same d2."""
+    X, Q = small_data
+    idx, _ = ref_index
+    k = 10
+    _say(f"64-bit ids path={path}")
+    ids, d2 = oracle.knn(X, Q, k)
+    big = lambda v: v.astype(np.uint64) * np.uint64(1 << 33) + np.uint64(7)
+    with hkg.HerculesIndex.open(idx, keep_bf16=path == "bf16") as ix:
+        ix.set_ids(big(row_map(idx, X)))
+        r = ix.knn(Q, k, path=path)
+    assert int(r.ids.max()) >= 1 << 33
+    assert_same(r.ids, r.d2, big(ids), d2, f"64-bit ids {path}")
+
+
 def ix_leaves(idx: str) -> int:
     from hercules_amd import hkg
     return int(hkg.inspect_dir(idx)["n_leaves"])
