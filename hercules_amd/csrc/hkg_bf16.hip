@@ -710,10 +710,14 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
 // xn2 - 2 s of the lane's series): each pair's lower bound with the row's own
 // slack, and the pairs within the query's threshold into its list, keyed by
 // that bound.  One thread per record (usually one or two of its 16 pairs
-// pass); a global atomic per kept pair.
+// pass); a global atomic per kept pair.  A value above the filter's own
+// pre-test constant for the row tile (the tile's largest slack terms, the
+// same float arithmetic as the filter) cannot pass with the row's own
+// slack, so its row's terms are not read.
 __global__ void __launch_bounds__(256) k_bf16_scatter(const uint4 *__restrict__ region, uint32_t slice_cap,
                                                       const uint32_t *__restrict__ region_cnt, uint64_t n,
-                                                      const float4 *__restrict__ xst, const float4 *__restrict__ qst,
+                                                      const float4 *__restrict__ xst, const float2 *__restrict__ tslack,
+                                                      float acc_g, const float4 *__restrict__ qst,
                                                       const float *__restrict__ thr0, uint32_t scap,
                                                       uint64_t *__restrict__ surv, uint32_t *__restrict__ scnt)
 {
@@ -727,6 +731,9 @@ __global__ void __launch_bounds__(256) k_bf16_scatter(const uint4 *__restrict__ 
     const uint32_t q = h.y;
     const float4 qs = qst[q];
     const float th = thr0[q];
+    const float2 tsl = tslack[h.x / kRowTile];
+    const float dmax = acc_g * (0.25f * tsl.x * tsl.x);
+    const float C = (th - qs.x) + (qs.y * tsl.x + qs.z * tsl.y + dmax + kEvalSlack);
 #pragma unroll
     for (int gi = 0; gi < 4; ++gi) {
       const uint4 vv = rec[1 + gi];
@@ -736,6 +743,7 @@ __global__ void __launch_bounds__(256) k_bf16_scatter(const uint4 *__restrict__ 
         const uint64_t row = (uint64_t)h.x + 8 * gi + i4;
         if (row >= n) continue;  // a padding row of the last tile
         const float v = __uint_as_float(vb[i4]);
+        if (!(v <= C)) continue;
         const float4 xsv = xst[row];
         const float lb = (v + qs.x) - (qs.y * xsv.y + qs.z * xsv.z + xsv.w + kEvalSlack);
         if (lb <= th) {
@@ -843,7 +851,7 @@ void launch_bf16_filter(hipStream_t st, const DevIndex &dv, uint32_t nq, const u
   if (!hist) {
     const uint32_t w = (uint32_t)filter_waves((int)dv.L / 16);
     hipLaunchKernelGGL(k_bf16_scatter, dim3(grid.x * grid.y * w, kScatterBlocks), dim3(256), 0, st, region, kRegionCap / w,
-                       region_cnt, dv.n, dv.xst, qst, thr0, scap, surv, scnt);
+                       region_cnt, dv.n, dv.xst, dv.tile_slack, acc_gamma(dv.L), qst, thr0, scap, surv, scnt);
   }
 }
 
