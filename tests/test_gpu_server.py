@@ -88,6 +88,28 @@ def test_server_same_alone_and_in_the_mix(hkg, setup, path):
         srv.stop()
 
 
+def test_server_trace_and_top_k_class(hkg, setup, monkeypatch, capfd):
+    """HKG_SERVER_TRACE prints one line per batch; a batch of small-k
+    requests that fits the scratch share at max_k runs at max_k (the sets
+    prepared at start), and its answers are those of each query alone."""
+    ix, Q, kv, ids, d2 = setup
+    _say("server trace, k class")
+    monkeypatch.setenv("HKG_SERVER_TRACE", "1")
+    srv = hkg.Server(ix, max_batch=64, path="bf16", max_inflight=1, max_k=100)
+    try:
+        rows = list(range(0, 40))
+        tickets = [(i, srv.submit(Q[i], 2)) for i in rows]
+        for i, t in tickets:
+            rid, rd2, _ = srv.wait(t, 2)
+            _check(i, 2, rid, rd2, ids, d2, "trace, k = 2")
+    finally:
+        srv.stop()
+    err = capfd.readouterr().err
+    lines = [l for l in err.splitlines() if l.startswith("hkg server: worker")]
+    assert lines, err[-2000:]
+    assert all(" k class 100:" in l for l in lines), lines
+
+
 def test_server_rejects_bad_requests(hkg, setup):
     ix, Q, kv, _, _ = setup
     srv = hkg.Server(ix, max_batch=16, max_k=10)
