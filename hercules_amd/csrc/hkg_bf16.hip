@@ -557,7 +557,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
   // one tile: j in buffer Bc (= j % 3); issues tile j + 2 into the buffer
   // that held tile j - 1
   // Sibling blocks (the same x in the other block groups, so on the same
-  // XCD) walk the same tiles.  Every 64 tiles a block publishes its
+  // XCD) walk the same tiles.  Every 128 tiles a block publishes its
   // progress and waits until its siblings have reached the same tile, so
   // that one block's copies from HBM are the others' L2 hits and the
   // batch reads the series about once instead of once per group.  The
@@ -625,7 +625,7 @@ k_bf16_filter(uint64_t n, uint32_t nq, const uint16_t *__restrict__ Xb, const fl
       cid = nid;
       cts = nts;
       fetch_chunk((j >> 6) + 1, nid, nts);
-      keep_pace(j);
+      if ((j & 127) == 0) keep_pace(j);
     }
     // the tile two ahead, copied during this step's MFMAs (past the end
     // of the list, tile 0 again into the free buffer: the same count of
