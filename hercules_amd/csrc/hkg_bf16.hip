@@ -171,9 +171,10 @@ __device__ __forceinline__ void load_lds4(const void *g, lds_void *lds)
   __builtin_amdgcn_global_load_lds(g, lds, 4, 0, 0);
 }
 
-// Filter.  Block = W waves x 64 queries (two 32-query tiles per wave) of
-// block group g (grid.y), wave w taking the queries qperm[64 (w G + g) ..]
-// (G groups); it loops over the
+// Filter.  Block = W waves x 32 QT queries (QT 32-query tiles per wave) of
+// block group g (grid.y), wave w taking the queries qperm[32 QT (w G + g) ..]
+// (G groups: a group's waves hold slots spread over the batch, so the
+// groups' loads even out between pacing points); it loops over the
 // row tiles of the batch's tile list (tiles overlapping a leaf whose EAPCA
 // bound is within the k-th distance bound of one of the batch's queries).
 // Tiles of kRowTile series go through a ring of kBuf LDS buffers, loaded
